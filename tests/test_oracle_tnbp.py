@@ -1,0 +1,150 @@
+"""Pins of the BP-PEPS restatement (oracle/tnbp.py, oracle/circuits.py).
+
+Each check is chosen so that a plausible mistake fails it: a swapped message
+direction, a transposed gate, a missing conjugate or a wrong fusion order all
+break the tree-exactness checks against the brute-force statevector, which
+use random single-qubit gates so that no lattice symmetry hides them.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import circuits, tci, tnbp
+from statevector import StateVector, X, Y, Z, ZZ, kicked_layer
+
+
+def test_lattices_and_layering():
+    e = tnbp.bond_square_lattice(3, 3)
+    assert len(e) == 2 * 9 - 3 - 3
+    assert e[:3] == [(0, 1), (0, 3), (1, 2)]
+    # interior site 4: bonds ordered (down, left, right, up) by global edge id
+    assert [e[m] for m in tnbp.surrounding_bond_index(4, e)] == [(1, 4), (3, 4), (4, 5), (4, 7)]
+    layers = tnbp.greedy_edge_layering(tnbp.bond_square_lattice(16, 16))
+    assert len(layers) == 4 and [len(l) for l in layers] == [128, 128, 126, 98]
+    assert tnbp.find_shortest_path(e, 0, 8)[0] == 0 and len(tnbp.find_shortest_path(e, 0, 8)) == 5
+
+
+def test_square_root_and_inverse():
+    rng = np.random.default_rng(5)
+    a = rng.standard_normal((6, 4)) + 1j * rng.standard_normal((6, 4))
+    m = a @ a.conj().T  # Hermitian PSD of rank 4
+    r, s = tnbp.square_root_and_inverse(m, 1e-8)
+    np.testing.assert_allclose(r @ r, m, atol=1e-10)
+    p = r @ s  # projector onto the support
+    np.testing.assert_allclose(p @ p, p, atol=1e-10)
+    np.testing.assert_allclose(p @ m, m, atol=1e-10)
+    assert abs(np.trace(p).real - 4) < 1e-10
+
+
+def test_product_state_closed_form():
+    g = 0.3 / 2  # RX(0.3) per step
+    r = circuits.run_kicked(3, 4, 0.0, 0.0, g / math.pi, 1.0, 50, 1e-12, 4, 1e-8)
+    for s in r["steps"]:
+        n = s["step"] + 1
+        np.testing.assert_allclose(s["z"], math.cos(0.3 * n), atol=1e-13)
+        np.testing.assert_allclose(s["x"], 0.0, atol=1e-13)
+        np.testing.assert_allclose(s["zz"], math.cos(0.3 * n) ** 2, atol=1e-13)
+
+
+def test_kicked_chain_exact():
+    L, a, hz, hx = 8, math.pi / 8, 0.1, 0.7 / (2 * math.pi)
+    r = circuits.run_kicked(L, 4, 0.125, hz, hx, 1.0, 200, 1e-13, 64, 1e-12, lattice="oned")
+    edges = tnbp.bond_oned_lattice(L)
+    sv = StateVector(L)
+    for s in r["steps"]:
+        kicked_layer(sv, edges, a, hz * math.pi, hx * math.pi)
+        np.testing.assert_allclose(s["z"], [sv.expect1(Z, q).real for q in range(L)], atol=1e-12)
+        np.testing.assert_allclose(s["x"], [sv.expect1(X, q).real for q in range(L)], atol=1e-12)
+        np.testing.assert_allclose(s["zz"], [sv.expect2(ZZ, i, j).real for i, j in edges], atol=1e-12)
+
+
+def _tree_edges():
+    # a small tree with a degree-3 vertex and both edge orientations
+    return [(0, 1), (1, 2), (1, 3), (3, 4), (5, 4)]
+
+
+def test_random_circuit_on_tree_exact():
+    rng = np.random.default_rng(11)
+    edges = _tree_edges()
+    n = 6
+    sv = StateVector(n)
+    tpos = []
+    for _ in range(3):
+        gates = circuits.random_layer_gates(rng, edges, n)
+        tpos.append(circuits.layer_tpo(edges, gates))
+        for u, q in gates:
+            if len(q) == 1:
+                sv.apply1(u, q[0])
+            else:
+                sv.apply2(u, q[0], q[1])
+    st, recs = circuits.run_tpo_layers(edges, tpos, 100, 1e-14, 64, 1e-12)
+    for op in (X, Y, Z):
+        got = tnbp.measure_sites(st, list(range(n)), [circuits.gate_tensor(op)] * n)
+        np.testing.assert_allclose(got, [sv.expect1(op, q) for q in range(n)], atol=1e-12)
+    zz = circuits.gate_tensor(ZZ)
+    xy = circuits.gate_tensor(np.kron(Y, X))  # X on the first qubit, Y on the second
+    got = tnbp.measure_edges(st, edges, [zz] * len(edges))
+    np.testing.assert_allclose(got, [sv.expect2(ZZ, a, b) for a, b in edges], atol=1e-12)
+    got = tnbp.measure_edges(st, edges, [xy] * len(edges))
+    np.testing.assert_allclose(got, [sv.expect2(np.kron(Y, X), a, b) for a, b in edges], atol=1e-12)
+
+
+def test_two_site_gate_on_path_matches_statevector():
+    # a CX between sites two bonds apart runs through an identity on site 1
+    edges = [(0, 1), (1, 2)]
+    sv = StateVector(3)
+    gates = [(circuits.gate_matrix("h"), (0,)), (circuits.gate_matrix("cx"), (0, 2)),
+             (circuits.gate_matrix("u3", (0.3, 0.2, 0.1)), (2,))]
+    for u, q in gates:
+        sv.apply1(u, q[0]) if len(q) == 1 else sv.apply2(u, *q)
+    st, _ = circuits.run_tpo_layers(edges, [circuits.layer_tpo(edges, gates)], 50, 1e-14, 16, 1e-12)
+    got = tnbp.measure_sites(st, [0, 1, 2], [circuits.gate_tensor(Z)] * 3)
+    np.testing.assert_allclose(got, [sv.expect1(Z, q) for q in range(3)], atol=1e-12)
+
+
+def test_t2_equals_t4_config1():
+    kw = dict(L=4, num_steps=5, Jz=0.125, hz=0.0, hx=0.3 / (2 * math.pi), dt=1.0,
+              max_bp_iterations=10000, bp_tolerance=1e-10, max_bond_dim=4, sv_min=1e-8)
+    r2 = circuits.run_kicked(two_site_svd="trunc", **kw)
+    r4 = circuits.run_kicked(two_site_svd="full", **kw)
+    for a, b in zip(r2["steps"], r4["steps"]):
+        assert a["bp_iterations"] == b["bp_iterations"]
+        np.testing.assert_allclose(a["z"], b["z"], atol=1e-12)
+        np.testing.assert_allclose(a["zz"], b["zz"], atol=1e-12)
+
+
+def test_gauge_invariance_of_bp_and_measure():
+    """A random invertible gauge on every bond leaves converged BP observables unchanged."""
+    rng = np.random.default_rng(7)
+    edges = tnbp.bond_square_lattice(3, 3)
+    r = circuits.run_kicked(3, 3, 0.125, 0.0, 0.05, 1.0, 500, 1e-13, 8, 1e-12)
+    st = r["state"]
+    layers = tnbp.greedy_edge_layering(edges)
+    st2 = st.copy()
+    for m, (a, b) in enumerate(edges):
+        ka = tnbp.surrounding_bond_index(a, edges).index(m)
+        kb = tnbp.surrounding_bond_index(b, edges).index(m)
+        n = st2.V[a].shape[ka]
+        g = np.eye(n) + 0.2 * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+        gi = np.linalg.inv(g)
+        la = list(range(st2.V[a].ndim)); la[ka] = -1
+        st2.V[a] = tci.contract(st2.V[a], la, g, [-1, ka], list(range(st2.V[a].ndim)))
+        lb = list(range(st2.V[b].ndim)); lb[kb] = -1
+        st2.V[b] = tci.contract(gi, [kb, -1], st2.V[b], lb, list(range(st2.V[b].ndim)))
+    for s in (st, st2):
+        s.E = [np.eye(e.shape[0], dtype=complex) for e in s.E]
+        tnbp.bp_loop(s, layers, 2000, 1e-13)
+    sites, _, Oz, medges, Oj = circuits.measops(edges)
+    np.testing.assert_allclose(tnbp.measure_sites(st, sites, Oz), tnbp.measure_sites(st2, sites, Oz), atol=1e-9)
+    np.testing.assert_allclose(tnbp.measure_edges(st, medges, Oj), tnbp.measure_edges(st2, medges, Oj), atol=1e-9)
+
+
+def test_residual_is_zero_at_fixed_point_and_positive_otherwise():
+    edges = tnbp.bond_square_lattice(3, 3)
+    r = circuits.run_kicked(3, 3, 0.125, 0.0, 0.05, 1.0, 500, 1e-13, 8, 1e-12)
+    st = r["state"]
+    tnbp.bp_loop(st, tnbp.greedy_edge_layering(edges), 500, 1e-14)
+    assert tnbp.belief_propagation_condition(st) < 1e-13
+    st.E[0] = st.E[0] + 0.1 * np.ones_like(st.E[0])
+    assert tnbp.belief_propagation_condition(st) > 1e-3
