@@ -1,0 +1,190 @@
+/*
+ * hbp.h - C-ABI of the MI355X-native belief-propagation PEPS backend.
+ *
+ * Two layers, both with plain pointers and sizes only:
+ *
+ *  1. Tensor layer: the Tensor Computing Interface (TCI) operations that
+ *     tnbp's algorithms issue, on device-resident column-major tensors.  Each
+ *     entry point names the min-tci function it replaces
+ *     (external/min-tci/include/tci/gqten/...); a C++ TCI header binds these
+ *     so that include/tnbp and the apps compile unchanged.
+ *
+ *  2. PEPS layer: fused entry points behind tnbp's algorithm functions
+ *     (include/tnbp/ptns/function/{init,apply,bp,truncation,measure}.h) that
+ *     keep the whole state (site tensors V, messages E) resident in HBM and
+ *     run each step as a few hand-written CDNA4 kernels.
+ *
+ * Conventions (same as the reference):
+ *  - storage is column-major, first index fastest (min-tci constructor.h:109-110);
+ *  - complex numbers are interleaved (re, im) pairs of double or float;
+ *  - messages E[m] (m = position in the sorted edge list) run first -> second,
+ *    E[m + n_e] second -> first (tnbp bp.h:170-174); index 0 = ket, 1 = bra;
+ *  - a site tensor has its bonds in ascending global edge order, then the
+ *    physical index (graph.h:40-55, init.h:57-60).
+ *
+ * Every call returns an hbp_status; on failure hbp_last_error() holds a
+ * message.  Calls on one context are issued in order on one HIP stream;
+ * calls that return host values synchronise that stream.  A context must not
+ * be used from two host threads at once.
+ */
+#ifndef HBP_H
+#define HBP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HBP_MAX_ORDER 8
+
+typedef int hbp_status;
+enum {
+  HBP_OK = 0,
+  HBP_ERR_INVALID_ARGUMENT = 1, /* std::invalid_argument in min-tci */
+  HBP_ERR_LOGIC = 2,            /* std::logic_error */
+  HBP_ERR_BAD_ALLOC = 3,        /* std::bad_alloc */
+  HBP_ERR_OUT_OF_RANGE = 4,     /* std::out_of_range */
+  HBP_ERR_HIP = 5,              /* HIP runtime failure */
+  HBP_ERR_UNSUPPORTED = 6       /* shape or option outside what the kernels take */
+};
+
+enum { HBP_C128 = 0, HBP_F64 = 1, HBP_C64 = 2, HBP_F32 = 3 };
+
+/* Arithmetic of the PEPS layer.  FP64: everything in double.  MIXED: site
+ * tensors, contractions and gate applies in float on MFMA with float
+ * accumulation; messages, normalisation, residual, truncation and <O> in double. */
+enum { HBP_PREC_FP64 = 0, HBP_PREC_MIXED = 1 };
+
+typedef struct hbp_ctx hbp_ctx;
+typedef struct hbp_tensor hbp_tensor;
+typedef struct hbp_peps hbp_peps;
+
+const char *hbp_last_error(void);
+const char *hbp_version(void);
+
+/* ---- context: tci::create_context (min-tci gqten/traits.h:25-27) ---- */
+hbp_status hbp_ctx_create(int device, hbp_ctx **out);
+hbp_status hbp_ctx_destroy(hbp_ctx *ctx);
+hbp_status hbp_ctx_synchronize(hbp_ctx *ctx);
+/* The HIP stream all work of this context runs on (a hipStream_t). */
+hbp_status hbp_ctx_stream(hbp_ctx *ctx, void **stream);
+/* Bytes currently held by the context's device pool. */
+hbp_status hbp_ctx_memory(hbp_ctx *ctx, size_t *in_use, size_t *peak);
+
+/* ==================================================== tensor layer ==== */
+/* Zero tensor of the given shape (tci::zeros / gqten tensor(shape)). */
+hbp_status hbp_tensor_create(hbp_ctx *ctx, int dtype, int order, const int32_t *shape,
+                             hbp_tensor **out);
+hbp_status hbp_tensor_destroy(hbp_ctx *ctx, hbp_tensor *t);
+/* tci::order / tci::shape / tci::size (min-tci getter.h). shape holds HBP_MAX_ORDER. */
+hbp_status hbp_tensor_info(const hbp_tensor *t, int *dtype, int *order, int32_t *shape,
+                           size_t *size);
+/* Host <-> device copies of the column-major element array
+ * (tci::assign_from_range constructor.h:114-163 / tci::to_range). */
+hbp_status hbp_tensor_upload(hbp_ctx *ctx, hbp_tensor *t, const void *host);
+hbp_status hbp_tensor_download(hbp_ctx *ctx, const hbp_tensor *t, void *host);
+/* tci::copy (constructor.h:192-204): deep copy, dst takes src's shape. */
+hbp_status hbp_tensor_copy(hbp_ctx *ctx, const hbp_tensor *src, hbp_tensor *dst);
+/* tci::reshape (manipulator.h:37-55): metadata only, same size required. */
+hbp_status hbp_tensor_reshape(hbp_ctx *ctx, hbp_tensor *t, int order, const int32_t *shape);
+/* tci::transpose (manipulator.h:61-79): axis i of out = axis new_order[i] of in. */
+hbp_status hbp_transpose(hbp_ctx *ctx, const hbp_tensor *in, const int32_t *new_order,
+                         hbp_tensor *out);
+/* tci::cplx_conj (manipulator.h:85-101), in place. */
+hbp_status hbp_conj(hbp_ctx *ctx, hbp_tensor *t);
+/* tci::fill (constructor.h:82-104). */
+hbp_status hbp_fill(hbp_ctx *ctx, hbp_tensor *t, double re, double im);
+/* tci::contract (min-tci linalg.h:311-323): labels shared by a and b are
+ * summed, survivors are laid out in the order of lc (label rules of
+ * GqtenCntLabelHelper linalg.h:205-305).  c may alias a or b. */
+hbp_status hbp_contract(hbp_ctx *ctx, const hbp_tensor *a, const int32_t *la,
+                        const hbp_tensor *b, const int32_t *lb, hbp_tensor *c,
+                        const int32_t *lc, int nc);
+/* tci::norm (linalg.h:67-72): Frobenius norm. */
+hbp_status hbp_norm(hbp_ctx *ctx, const hbp_tensor *t, double *out);
+/* tci::normalize (linalg.h:77-84): t /= ||t||_F, returns the norm. */
+hbp_status hbp_normalize(hbp_ctx *ctx, hbp_tensor *t, double *norm_out);
+/* tci::scale (linalg.h:103-110): t *= (re + i im). */
+hbp_status hbp_scale(hbp_ctx *ctx, hbp_tensor *t, double re, double im);
+/* tci::linear_combine (linalg.h:468-493): out = sum_i coefs[i] * ins[i];
+ * coefs holds n complex numbers as (re, im) pairs. */
+hbp_status hbp_linear_combine(hbp_ctx *ctx, int n, const hbp_tensor *const *ins,
+                              const double *coefs, hbp_tensor *out);
+/* tci::trace (linalg.h:142-149): trace over bond pairs (pairs[2k], pairs[2k+1]). */
+hbp_status hbp_trace(hbp_ctx *ctx, const hbp_tensor *in, int npairs, const int32_t *pairs,
+                     hbp_tensor *out);
+/* tci::diag (linalg.h:49-62): vector of n -> n x n diagonal matrix; a real
+ * vector gives a complex matrix when out is complex (tci::to_cplx + diag). */
+hbp_status hbp_diag(hbp_ctx *ctx, const hbp_tensor *vec, hbp_tensor *out);
+/* tci::get_elem (getter.h:50-80); an order-0 tensor returns its scalar. */
+hbp_status hbp_get_elem(hbp_ctx *ctx, const hbp_tensor *t, const int32_t *coors, double *re,
+                        double *im);
+/* tci::svd (linalg.h:500-512): thin SVD of the matricisation with nrows row
+ * bonds; u (rows..., k), s real (k), vdag (k, cols...), k = min(m, n),
+ * singular values descending.  Batched one-sided Jacobi in FP64 on the GPU. */
+hbp_status hbp_svd(hbp_ctx *ctx, const hbp_tensor *a, int nrows, hbp_tensor *u, hbp_tensor *s,
+                   hbp_tensor *vdag);
+/* tci::trunc_svd(..., chi_max, s_min) (linalg.h:552-569): keeps the singular
+ * values > s_min, at most chi_max and at least one; trunc_err = discarded
+ * s^2 / total s^2. */
+hbp_status hbp_trunc_svd(hbp_ctx *ctx, const hbp_tensor *a, int nrows, int chi_max,
+                         double s_min, hbp_tensor *u, hbp_tensor *s, hbp_tensor *vdag,
+                         double *trunc_err);
+
+/* ====================================================== PEPS layer ==== */
+/* tnbp::InitTensorProductState (init.h:17-95) on one rank: |0...0> with all
+ * bonds of dimension 1 and 1x1 messages equal to 1.  edges holds n_edges
+ * (first, second) pairs; sites are the sorted distinct endpoints.
+ * precision is HBP_PREC_FP64 or HBP_PREC_MIXED. */
+hbp_status hbp_peps_create(hbp_ctx *ctx, int n_edges, const int32_t *edges,
+                           const int32_t *phys_dims, int precision, hbp_peps **out);
+hbp_status hbp_peps_destroy(hbp_ctx *ctx, hbp_peps *p);
+hbp_status hbp_peps_num_sites(const hbp_peps *p, int *n_sites, int *n_edges);
+/* Shape of site tensor s (order = degree + 1) and of message m (2 x D). */
+hbp_status hbp_peps_site_shape(const hbp_peps *p, int site, int *order, int32_t *shape);
+/* Download / upload a site tensor or message as complex128 column-major
+ * (site index = position in the sorted site list; message index 0..2n_e-1). */
+hbp_status hbp_peps_get_site(hbp_ctx *ctx, hbp_peps *p, int site, double *host);
+hbp_status hbp_peps_set_site(hbp_ctx *ctx, hbp_peps *p, int site, int order,
+                             const int32_t *shape, const double *host);
+hbp_status hbp_peps_get_message(hbp_ctx *ctx, hbp_peps *p, int m, int32_t *dim, double *host);
+hbp_status hbp_peps_set_message(hbp_ctx *ctx, hbp_peps *p, int m, int32_t dim,
+                                const double *host);
+
+/* tnbp::AbsorbTPO (apply.h:14-177): per site O_s(t_1..t_z, d_in, d_out) given
+ * as complex128 column-major host arrays (ops[s] with shape tpo_shapes[s*8..]),
+ * V'[(v o)...,p'] = sum_p O[o..,p,p'] V[v..,p]; messages become E (x) I_t. */
+hbp_status hbp_peps_absorb_tpo(hbp_ctx *ctx, hbp_peps *p, const double *const *ops,
+                               const int32_t *tpo_shapes);
+
+/* tnbp::BeliefPropagation over all layers followed by E <- F
+ * (bp.h:65-261, apps/kicked_ising_lattice/main.cc:128-141). */
+hbp_status hbp_peps_bp_sweep(hbp_ctx *ctx, hbp_peps *p);
+/* tnbp::BeliefPropagationCondition (bp.h:266-355): mean Frobenius distance
+ * of recomputed and stored normalised messages. */
+hbp_status hbp_peps_bp_condition(hbp_ctx *ctx, hbp_peps *p, double *residual);
+/* The driver's BP loop (main.cc:121-159): repeat sweep + condition until
+ * residual < tol or max_iter.  The condition's recomputed messages are the
+ * next sweep's messages, so each iteration after the first costs one sweep.
+ * residuals (may be NULL) receives max_iter values; iters the count run. */
+hbp_status hbp_peps_bp_run(hbp_ctx *ctx, hbp_peps *p, int max_iter, double tol, int *iters,
+                           double *residuals);
+/* tnbp::Truncation (truncation.h:17-222 with SquareRootAndInverse root.h:65-148):
+ * bond_dims / trunc_errs (may be NULL) receive one value per edge. */
+hbp_status hbp_peps_truncate(hbp_ctx *ctx, hbp_peps *p, int chi_max, double sv_min,
+                             int32_t *bond_dims, double *trunc_errs);
+/* tnbp::Measure for single sites (measure.h:14-94): ops are n complex 2x2
+ * (d x d) operators O(in, out) column-major; out receives n complex values. */
+hbp_status hbp_peps_measure_sites(hbp_ctx *ctx, hbp_peps *p, int n, const int32_t *sites,
+                                  const double *ops, double *out);
+/* tnbp::Measure for nearest-neighbour pairs (measure.h:99-300): ops are n
+ * complex (d,d,d,d) operators indexed (a_in, b_in, a_out, b_out). */
+hbp_status hbp_peps_measure_edges(hbp_ctx *ctx, hbp_peps *p, int n, const int32_t *pairs,
+                                  const double *ops, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HBP_H */

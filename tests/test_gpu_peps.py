@@ -1,0 +1,163 @@
+"""Fused PEPS layer on the GPU against the oracle and its golden fixtures.
+
+Tolerances (stated per test): the plain FP64 path must match the oracle to
+1e-10 relative per expectation value (floor 1 for values near zero); the mixed
+path to 1e-5.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from gpu_util import announce, rel_close
+from oracle import circuits as oc
+from oracle import tnbp as ot
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from tnbp_amd.tci import Context
+    c = Context(0)
+    yield c
+    c.close()
+
+
+def _golden(name):
+    return json.load(open(os.path.join(GOLDEN, name)))
+
+
+def _run_gpu(ctx, args, precision=0):
+    from tnbp_amd import driver
+    a = dict(args)
+    keep_zero = a.pop("two_site_svd", "trunc") == "full"
+    return driver.run_kicked(ctx, two_site_keep_zero=keep_zero, precision=precision, **a)
+
+
+@pytest.mark.parametrize("name", ["config1_kicked_4x4_chi4.json", "config1_fixed3_kicked_4x4_chi4.json",
+                                  "config1_t4_kicked_4x4_chi4.json"])
+def test_config1_fp64_matches_golden(ctx, name):
+    announce(f"config1 fp64 {name}")
+    g = _golden(name)
+    r = _run_gpu(ctx, g["args"])
+    for a, b in zip(r["steps"], g["steps"]):
+        assert a["bp_iterations"] == b["bp_iterations"], (a["step"], a["residuals"], b["residuals"])
+        assert a["bond_dims"] == b["bond_dims"]
+        for key in ("x", "z", "zz"):
+            assert rel_close(a[key], b[key], 1e-10) <= 1.0, (key, a["step"])
+        np.testing.assert_allclose(a["trunc_errs"], b["trunc_errs"], rtol=1e-6, atol=1e-14)
+        # residuals are gauge invariant; compare where they are above rounding
+        for ra, rb in zip(a["residuals"], b["residuals"]):
+            if rb > 1e-12:
+                assert abs(ra - rb) <= 1e-6 * rb
+
+
+def test_config1_site11(ctx):
+    g = _golden("config1_kicked_4x4_chi4.json")
+    r = _run_gpu(ctx, g["args"])
+    want = g["steps"][-1]["z"][5]
+    got = r["steps"][-1]["z"][5]
+    announce(f"<Z_(1,1)> gpu {got!r} oracle {want!r}")
+    assert abs(got - want) <= 1e-10 * max(abs(want), 1.0)
+
+
+def test_config1_mixed_within_1e5(ctx):
+    announce("config1 mixed")
+    g = _golden("config1_fixed3_kicked_4x4_chi4.json")
+    r = _run_gpu(ctx, g["args"], precision=1)
+    for a, b in zip(r["steps"], g["steps"]):
+        for key in ("x", "z", "zz"):
+            assert rel_close(a[key], b[key], 1e-5) <= 1.0, (key, a["step"])
+
+
+def test_chain_random_gates_exact(ctx):
+    """Open chain (a tree): BP is exact; random U3 breaks every symmetry."""
+    from tnbp_amd import circuits as pc
+    from tnbp_amd import driver
+    from tnbp_amd.peps import Measure, MeasureEdges
+    announce("chain random gates")
+    rng = np.random.default_rng(21)
+    edges = pc.chain(7)
+    layers = [pc.random_layer(rng, edges, 7) for _ in range(4)]
+    peps, recs = driver.run_layers(ctx, edges, layers, 100, 1e-13, 64, 1e-12, keep=True)
+    st, orecs = oc.run_tpo_layers(edges, [oc.layer_tpo(edges, g) for g in layers], 100, 1e-13, 64, 1e-12)
+    for op in ("X", "Y", "Z"):
+        o = pc.op_tensor(pc.PAULI[op])
+        got = Measure(ctx, peps, list(range(7)), [o] * 7)
+        want = ot.measure_sites(st, list(range(7)), [o] * 7)
+        np.testing.assert_allclose(got, want, atol=1e-10)
+    o2 = pc.op_tensor(np.kron(pc.PAULI["Y"], pc.PAULI["X"]))
+    got = MeasureEdges(ctx, peps, edges, [o2] * len(edges))
+    want = ot.measure_edges(st, edges, [o2] * len(edges))
+    np.testing.assert_allclose(got, want, atol=1e-10)
+    peps.close()
+
+
+def test_square_random_gates_fixed_iterations(ctx):
+    """3x3 loopy lattice with random gates and truncation, fixed BP iterations."""
+    from tnbp_amd import circuits as pc
+    from tnbp_amd import driver
+    from tnbp_amd.peps import Measure, MeasureEdges
+    announce("3x3 random gates")
+    rng = np.random.default_rng(5)
+    edges = pc.square_lattice(3, 3)
+    layers = [pc.random_layer(rng, edges, 9) for _ in range(4)]
+    peps, recs = driver.run_layers(ctx, edges, layers, 4, 0.0, 4, 1e-8, keep=True)
+    st, orecs = oc.run_tpo_layers(edges, [oc.layer_tpo(edges, g) for g in layers], 4, 0.0, 4, 1e-8)
+    for a, b in zip(recs, orecs):
+        assert a["bond_dims"] == b["bond_dims"]
+        np.testing.assert_allclose(a["residuals"], b["residuals"], rtol=1e-6, atol=1e-13)
+    zop = pc.op_tensor(pc.PAULI["Z"])
+    got = Measure(ctx, peps, list(range(9)), [zop] * 9)
+    want = ot.measure_sites(st, list(range(9)), [zop] * 9)
+    assert rel_close(got, want, 1e-10) <= 1.0
+    o2 = pc.op_tensor(np.kron(pc.PAULI["Z"], pc.PAULI["X"]))
+    got = MeasureEdges(ctx, peps, edges, [o2] * len(edges))
+    want = ot.measure_edges(st, edges, [o2] * len(edges))
+    assert rel_close(got, want, 1e-10) <= 1.0
+    peps.close()
+
+
+def test_site_and_message_round_trip(ctx):
+    from tnbp_amd.peps import InitTensorProductState
+    announce("round trip")
+    p = InitTensorProductState(ctx, [(0, 1), (1, 2)])
+    v = p.get_site(1)
+    assert v.shape == (1, 1, 2) and v[0, 0, 0] == 1 and v[0, 0, 1] == 0
+    e = p.get_message(3)
+    assert e.shape == (1, 1) and e[0, 0] == 1
+    with pytest.raises(ValueError):
+        p.set_site(1, np.zeros((2, 1, 2)))
+    p.close()
+
+
+def test_residual_matches_condition(ctx):
+    """bp_run's fused residual equals BeliefPropagationCondition's."""
+    from tnbp_amd import circuits as pc
+    from tnbp_amd.peps import (AbsorbTPO, BeliefPropagation, BeliefPropagationCondition,
+                               InitTensorProductState)
+    announce("residual vs condition")
+    edges = pc.square_lattice(3, 3)
+    tpo = pc.kicked_ising_tpo(edges, 0.125, 0.0, 0.05, math.pi)
+    p = InitTensorProductState(ctx, edges)
+    AbsorbTPO(ctx, tpo, p)
+    AbsorbTPO(ctx, tpo, p)
+    st = ot.init_tensor_product_state(edges, [2] * 9)
+    otpo = oc.circuit_tpo(edges, 0.125, 0.0, 0.05, math.pi, "trunc")
+    ot.absorb_tpo(st, otpo)
+    ot.absorb_tpo(st, otpo)
+    layers = ot.greedy_edge_layering(edges)
+    for _ in range(3):
+        BeliefPropagation(ctx, p)
+        F = [None] * (2 * len(edges))
+        for J in layers:
+            ot.belief_propagation(st, J, F)
+        st.E = F
+        r_gpu = BeliefPropagationCondition(ctx, p)
+        r_ref = ot.belief_propagation_condition(st)
+        assert abs(r_gpu - r_ref) <= 1e-8 * r_ref
+    p.close()

@@ -1,0 +1,149 @@
+"""TCI layer on the GPU (hbp_contract & co.) against the oracle's restatement of min-tci.
+
+Tolerance: complex128 results agree with the oracle to 1e-12 relative to the
+operands' scale (they differ by summation order only).
+"""
+import numpy as np
+import pytest
+
+from gpu_util import announce
+from oracle import tci as ot
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from tnbp_amd.tci import Context
+    c = Context(0)
+    yield c
+    c.close()
+
+
+def _rand(rng, shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+def _contract(ctx, a, la, b, lb, lc):
+    from tnbp_amd import tci as T
+    ta, tb = T.Tensor.from_numpy(ctx, a), T.Tensor.from_numpy(ctx, b)
+    return T.contract(ctx, ta, la, tb, lb, lc).to_numpy()
+
+
+CASES = [
+    # mode products at every bond position (bp.h:148), square and projector-shaped E
+    ((8, 8, 8, 8, 2), [-1, 1, 2, 3, 4], (8, 8), [-1, 0], [0, 1, 2, 3, 4]),
+    ((8, 8, 8, 8, 2), [0, -1, 2, 3, 4], (8, 8), [-1, 1], [0, 1, 2, 3, 4]),
+    ((16, 16, 16, 2), [0, 1, -1, 3], (16, 16), [-1, 2], [0, 1, 2, 3]),
+    ((64, 64, 2), [0, -1, 2], (64, 64), [-1, 1], [0, 1, 2]),
+    ((64, 32, 2), [-1, 1, 2], (64, 32), [-1, 0], [0, 1, 2]),          # projector D -> chi
+    ((6, 5, 7, 3), [0, 1, -1, 3], (5, 7), [2, -1], [0, 1, 2, 3]),     # shared bond is E's axis 1
+    ((3, 4, 5), [0, 1, -1], (5, 100), [-1, 2], [0, 1, 2]),
+    ((96, 96, 2), [-1, 1, 2], (96, 96), [-1, 0], [0, 1, 2]),
+    # all-but-one message contraction (bp.h:165)
+    ((8, 8, 8, 2), [0, -2, -3, -4], (8, 8, 8, 2), [1, -2, -3, -4], [0, 1]),
+    ((8, 8, 8, 2), [-1, -2, 0, -4], (8, 8, 8, 2), [-1, -2, 1, -4], [0, 1]),
+    ((32, 32, 32, 2), [-1, 0, -3, -4], (32, 32, 32, 2), [-1, 1, -3, -4], [0, 1]),
+    ((5, 7, 3), [-1, 0, -3], (5, 7, 3), [-1, 1, -3], [1, 0]),
+    # full contraction (measure.h:83)
+    ((4, 5, 6, 2), [-1, -2, -3, -4], (4, 5, 6, 2), [-1, -2, -3, -4], []),
+    # general: edge environment (measure.h:204), AbsorbTPO interleave (apply.h:64), outer product
+    ((3, 4, 5, 2), [-1, 2, -3, 0], (3, 4, 5, 2), [-1, 3, -3, 1], [0, 1, 2, 3]),
+    ((2, 3, 2, 2), [1, 3, -1, 5], (4, 5, 2), [0, 2, -1], [0, 1, 2, 3, 5]),
+    ((4, 4), [0, 2], (3, 3), [1, 3], [0, 1, 2, 3]),
+    ((150, 7), [0, -1], (7, 140), [-1, 1], [0, 1]),
+    ((3, 4), [-1, 0], (3, 5), [-1, 1], [1, 0]),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[str(i) for i in range(len(CASES))])
+def test_contract(ctx, case):
+    sa, la, sb, lb, lc = case
+    announce(f"contract {sa} {la} x {sb} {lb} -> {lc}")
+    rng = np.random.default_rng(hash(str(case)) % 2**32)
+    a, b = _rand(rng, sa), _rand(rng, sb)
+    got = _contract(ctx, a, la, b, lb, lc)
+    want = ot.contract(a, la, b, lb, lc)
+    scale = np.sqrt(np.sum(np.abs(a) ** 2) * np.sum(np.abs(b) ** 2))
+    assert got.shape == want.shape
+    assert np.max(np.abs(got - want)) <= 1e-12 * scale
+
+
+def test_contract_errors(ctx):
+    from tnbp_amd import tci as T
+    a = T.Tensor.from_numpy(ctx, np.ones((2, 2), dtype=complex))
+    with pytest.raises(ValueError):
+        T.contract(ctx, a, [0, 0], a, [1, 2], [0, 1, 2])
+    with pytest.raises(ValueError):
+        T.contract(ctx, a, [0, -1], a, [-1, 1], [0, 2])
+
+
+def test_contract_in_place_alias(ctx):
+    from tnbp_amd import tci as T
+    rng = np.random.default_rng(8)
+    a, e = _rand(rng, (4, 5, 2)), _rand(rng, (5, 5))
+    ta, te = T.Tensor.from_numpy(ctx, a), T.Tensor.from_numpy(ctx, e)
+    T.contract(ctx, ta, [0, -1, 2], te, [-1, 1], [0, 1, 2], out=ta)
+    np.testing.assert_allclose(ta.to_numpy(), ot.contract(a, [0, -1, 2], e, [-1, 1], [0, 1, 2]), atol=1e-12)
+
+
+def test_elementwise_and_reductions(ctx):
+    from tnbp_amd import tci as T
+    announce("elementwise")
+    rng = np.random.default_rng(9)
+    a = _rand(rng, (3, 4, 5))
+    t = T.Tensor.from_numpy(ctx, a)
+    assert abs(T.norm(ctx, t) - ot.norm(a)) < 1e-12 * ot.norm(a)
+    n = T.normalize(ctx, t)
+    assert abs(n - ot.norm(a)) < 1e-12 * n
+    np.testing.assert_allclose(t.to_numpy(), a / ot.norm(a), atol=1e-14)
+    T.cplx_conj(ctx, t)
+    np.testing.assert_allclose(t.to_numpy(), np.conj(a) / ot.norm(a), atol=1e-14)
+    t2 = T.Tensor.from_numpy(ctx, a)
+    T.transpose(ctx, t2, [2, 0, 1])
+    np.testing.assert_array_equal(t2.to_numpy(), np.transpose(a, [2, 0, 1]))
+    T.reshape(ctx, t2, [5, 12])
+    np.testing.assert_array_equal(t2.to_numpy(), ot.reshape(np.transpose(a, [2, 0, 1]), (5, 12)))
+    b = _rand(rng, (3, 4, 5))
+    lc = T.linear_combine(ctx, [T.Tensor.from_numpy(ctx, a), T.Tensor.from_numpy(ctx, b)], [1.0, -1.0 + 0.5j])
+    np.testing.assert_allclose(lc.to_numpy(), a + (-1.0 + 0.5j) * b, atol=1e-13)
+    c = _rand(rng, (2, 2, 3, 3))
+    tr = T.trace(ctx, T.Tensor.from_numpy(ctx, c), [(0, 1)])
+    np.testing.assert_allclose(tr.to_numpy(), ot.trace(c, [(0, 1)]), atol=1e-13)
+    v = T.Tensor.from_numpy(ctx, np.array([1.0, 2.0, 3.0]))
+    vc = T.to_cplx(ctx, v)
+    T.diag(ctx, vc)
+    np.testing.assert_allclose(vc.to_numpy(), np.diag([1.0, 2.0, 3.0]), atol=0)
+    assert T.get_elem(ctx, T.Tensor.from_numpy(ctx, a), [1, 2, 3]) == a[1, 2, 3]
+
+
+@pytest.mark.parametrize("shape,nrows", [((6, 6), 1), ((4, 3, 5), 2), ((64, 64), 1), ((2, 7), 1), ((9, 4), 1)])
+def test_svd(ctx, shape, nrows):
+    from tnbp_amd import tci as T
+    announce(f"svd {shape} {nrows}")
+    rng = np.random.default_rng(10)
+    a = _rand(rng, shape)
+    u, s, vd = T.svd(ctx, T.Tensor.from_numpy(ctx, a), nrows)
+    U, S, Vd = u.to_numpy(), s.to_numpy(), vd.to_numpy()
+    _, s_ref, _ = ot.svd(a, nrows)
+    np.testing.assert_allclose(S, s_ref, rtol=1e-12, atol=1e-13 * s_ref[0])
+    k = S.size
+    rec = ot.contract(ot.contract(U, list(range(nrows)) + [-1], np.diag(S).astype(complex), [-1, 100], list(range(nrows)) + [100]),
+                      list(range(nrows)) + [-1], Vd, [-1] + list(range(nrows, a.ndim)), list(range(a.ndim)))
+    np.testing.assert_allclose(rec, a, atol=1e-12 * s_ref[0])
+    Um = U.reshape(-1, k, order="F")
+    np.testing.assert_allclose(Um.conj().T @ Um, np.eye(k), atol=1e-12)
+
+
+def test_trunc_svd_rule(ctx):
+    from tnbp_amd import tci as T
+    rng = np.random.default_rng(3)
+    q1, _ = np.linalg.qr(_rand(rng, (6, 6)))
+    q2, _ = np.linalg.qr(_rand(rng, (6, 6)))
+    s_true = np.array([1.0, 0.5, 0.25, 1e-3, 1e-9, 0.0])
+    m = q1 @ np.diag(s_true) @ q2
+    u, s, vd, err = T.trunc_svd(ctx, T.Tensor.from_numpy(ctx, m), 1, 3, 1e-8)
+    np.testing.assert_allclose(s.to_numpy(), s_true[:3], rtol=1e-12)
+    assert abs(err - (1e-6 + 1e-18) / np.sum(s_true ** 2)) < 1e-12
+    u, s, vd, err = T.trunc_svd(ctx, T.Tensor.from_numpy(ctx, m), 1, 10, 1e-8)
+    assert s.to_numpy().size == 4

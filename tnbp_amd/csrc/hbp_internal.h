@@ -1,0 +1,143 @@
+// Internal declarations shared by the hbp HIP sources (not part of the C-ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "hbp.h"
+
+namespace hbp {
+
+struct Error : std::runtime_error {
+  hbp_status code;
+  Error(hbp_status c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define HBP_HIP(call)                                                                        \
+  do {                                                                                       \
+    hipError_t e_ = (call);                                                                  \
+    if (e_ != hipSuccess)                                                                    \
+      throw ::hbp::Error(HBP_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));  \
+  } while (0)
+
+#define HBP_REQUIRE(cond, code, msg)                                                         \
+  do {                                                                                       \
+    if (!(cond)) throw ::hbp::Error((code), (msg));                                          \
+  } while (0)
+
+inline size_t dtype_size(int dtype) {
+  switch (dtype) {
+    case HBP_C128: return 16;
+    case HBP_F64: return 8;
+    case HBP_C64: return 8;
+    case HBP_F32: return 4;
+  }
+  throw Error(HBP_ERR_INVALID_ARGUMENT, "unknown dtype");
+}
+inline bool dtype_complex(int dtype) { return dtype == HBP_C128 || dtype == HBP_C64; }
+
+// ------------------------------------------------------------- context
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipMemPool_t pool = nullptr;
+  size_t in_use = 0, peak = 0;
+  // grow-only scratch for split-K partials and reductions
+  void *scratch = nullptr;
+  size_t scratch_bytes = 0;
+  // pinned host staging for small results
+  double *host_small = nullptr;
+
+  void *alloc(size_t bytes);
+  void release(void *p, size_t bytes);
+  void *get_scratch(size_t bytes);
+  void sync();
+  // host -> device copy of a small host buffer that may go out of scope on
+  // return: enqueued on the context stream and waited for
+  void upload(void *dst, const void *src, size_t bytes);
+};
+
+struct Tensor {
+  int dtype = HBP_C128;
+  int order = 0;
+  int32_t shape[HBP_MAX_ORDER] = {0};
+  size_t size = 1;
+  void *data = nullptr;
+  size_t bytes = 0;
+};
+
+size_t shape_size(int order, const int32_t *shape);
+void tensor_resize(Ctx *ctx, Tensor *t, int dtype, int order, const int32_t *shape);
+
+// ------------------------------------------------------------ kernels
+// Out[q, j] = scale * sum_i T[q, i] E(i, j), with the row q = l + L r of a
+// column-major (L, Din, R) tensor, E(i, j) = E[i + Din j] (e_trans = 0) or
+// E[j + Dout i] (e_trans = 1), complex<double> E; T / Out of scalar S.
+// norm_partials (may be null) receives one sum |Out|^2 per block.
+template <typename S>
+void launch_mode_product(hipStream_t st, const void *T, const double2 *E, void *Out, int64_t L,
+                         int Din, int Dout, int64_t R, int e_trans, double scale,
+                         double *norm_partials, int *n_partials);
+
+// Partial sums of M[i, j] = sum_q T[q, i] conj(V[q, j]) (conjV = 1) or
+// T[q, i] V[q, j] (conjV = 0) over row chunks; T is a (L, D1, R) view and V a
+// (L, D2, R) view of scalar S, with r strides ldT_R, ldV_R (normally L*D).
+// Returns the number of chunks written (each D1*D2 complex<double>).
+template <typename S>
+int message_partials(hipStream_t st, const void *T, const void *V, double2 *partials,
+                     int64_t L, int D1, int D2, int64_t R, int64_t ldT_R, int64_t ldV_R,
+                     int64_t max_chunks, int conjV);
+int64_t message_num_chunks(int64_t Q);
+// M = sum over chunks (fixed order), complex<double>.
+void launch_reduce_partials(hipStream_t st, const double2 *partials, int nchunks, int n,
+                            double2 *M);
+
+// Elementwise / reduction helpers (complex<double> unless noted).
+void launch_norm2_partials(hipStream_t st, const void *x, size_t n, int dtype, double *partials,
+                           int *nblocks);
+void launch_sum_partials(hipStream_t st, const double *partials, int n, double *out);
+void launch_scale(hipStream_t st, void *x, size_t n, int dtype, double re, double im,
+                  const double *dev_scale_inv_sqrt);
+void launch_conj(hipStream_t st, void *x, size_t n, int dtype);
+void launch_fill(hipStream_t st, void *x, size_t n, int dtype, double re, double im);
+void launch_convert(hipStream_t st, const void *x, int dtype_in, void *y, int dtype_out,
+                    size_t n);
+void launch_permute(hipStream_t st, const void *in, void *out, int elem_bytes, int order,
+                    const int32_t *in_shape, const int32_t *perm);
+void launch_axpy_c128(hipStream_t st, const double2 *x, double2 *y, size_t n, double re,
+                      double im);
+void launch_trace(hipStream_t st, const double2 *in, double2 *out, int order,
+                  const int32_t *shape, int npairs, const int32_t *pairs);
+
+// Batched one-sided Jacobi SVD of complex<double> matrices.
+//   A[b]: m x n column-major at A + offA[b]; results U[b] (m x k), s[b] (k),
+//   Vd[b] (k x n), k = min(m, n), descending.  work needs
+//   batch * (m*n + n*n) complex<double>.
+struct SvdJob {
+  int64_t a_off, u_off, s_off, vd_off, work_off;
+  int m, n;
+};
+void launch_batched_svd(hipStream_t st, const double2 *A, double2 *U, double *s, double2 *Vd,
+                        double2 *work, const SvdJob *jobs_dev, int batch, int max_n);
+
+// Small batched complex GEMM: C[b] = op(A[b]) op(B[b]) for matrices up to a few
+// hundred on a side (truncation projectors and the generic TCI contract).
+struct GemmJob {
+  int64_t a_off, b_off, c_off;
+  int m, n, k;
+  int lda, ldb, ldc;
+  int opa, opb;  // 0 = N, 1 = T, 2 = C (conjugate transpose), 3 = conj (no transpose)
+  double2 alpha;
+};
+void launch_batched_gemm(hipStream_t st, const double2 *A, const double2 *B, double2 *C,
+                         const GemmJob *jobs_dev, int batch, int max_m, int max_n);
+
+}  // namespace hbp
+
+// opaque C handles
+struct hbp_ctx : hbp::Ctx {};
+struct hbp_tensor : hbp::Tensor {};

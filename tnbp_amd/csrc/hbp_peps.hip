@@ -1,0 +1,1150 @@
+// PEPS layer: the state (site tensors V, messages E) lives in HBM and each
+// tnbp algorithm step runs as a handful of kernel launches.
+//
+//   AbsorbTPO            apply.h:14-177        k_absorb_site, k_expand_message
+//   BeliefPropagation    bp.h:65-261           mode products + message kernel
+//   BP condition         bp.h:266-355          reused from the next sweep (same messages)
+//   Truncation           truncation.h:17-222   batched Jacobi SVD, k_roots, k_rr, k_projectors
+//   Measure (site/edge)  measure.h:14-300      mode products + message kernel, k_site_ratio,
+//                                              k_edge_ratio
+//
+// Normalisation: the reference normalises T after every absorbed bond
+// (bp.h:153).  Those are positive scalings of a tensor whose final message is
+// normalised again (bp.h:166), so they change rounding only; the fused path
+// normalises each message once, in FP64.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <vector>
+
+#include "hbp_internal.h"
+
+namespace hbp {
+void set_last_error(const std::string &s);
+void svd_impl(Ctx *ctx, const Tensor *a, int nrows, Tensor *u, Tensor *s, Tensor *vd,
+              int keep_max, double s_min, double *trunc_err);
+double device_norm2(Ctx *ctx, const void *x, size_t n, int dtype, double *dev_out);
+
+namespace {
+
+__device__ inline double wsum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+__device__ inline double bsum(double v, double *sh) {
+  v = wsum(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wave] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += sh[w];
+  __syncthreads();
+  return t;
+}
+
+struct MsgJob {
+  int64_t raw_off;   // raw message (complex<double>) in the raw buffer
+  int64_t new_off;   // destination in the new message set
+  int64_t old_off;   // stored message to compare against (residual)
+  int D;
+};
+
+// normalise every raw message (FP64) and, when old != null, the Frobenius
+// distance between it and the normalised stored message (bp.h:321-350)
+__global__ __launch_bounds__(256) void k_finalize_messages(const double2 *__restrict__ raw,
+                                                            double2 *__restrict__ out,
+                                                            const double2 *__restrict__ old,
+                                                            const MsgJob *__restrict__ jobs,
+                                                            double *__restrict__ dist) {
+  __shared__ double sh[8];
+  const MsgJob j = jobs[blockIdx.x];
+  const int n = j.D * j.D;
+  double s = 0.0, so = 0.0;
+  for (int e = threadIdx.x; e < n; e += blockDim.x) {
+    const double2 v = raw[j.raw_off + e];
+    s += v.x * v.x + v.y * v.y;
+    if (old) {
+      const double2 w = old[j.old_off + e];
+      so += w.x * w.x + w.y * w.y;
+    }
+  }
+  s = bsum(s, sh);
+  if (old) so = bsum(so, sh);
+  const double inv = s > 0.0 ? 1.0 / sqrt(s) : 1.0;
+  const double invo = so > 0.0 ? 1.0 / sqrt(so) : 1.0;
+  double d = 0.0;
+  for (int e = threadIdx.x; e < n; e += blockDim.x) {
+    const double2 v = raw[j.raw_off + e];
+    const double2 m = make_double2(v.x * inv, v.y * inv);
+    out[j.new_off + e] = m;
+    if (old) {
+      const double2 w = old[j.old_off + e];
+      const double dx = m.x - w.x * invo, dy = m.y - w.y * invo;
+      d += dx * dx + dy * dy;
+    }
+  }
+  if (old) {
+    d = bsum(d, sh);
+    if (threadIdx.x == 0) dist[blockIdx.x] = sqrt(d);
+  }
+}
+
+__global__ void k_residual(const double *__restrict__ dist, int n, double vol,
+                           double *__restrict__ out) {
+  __shared__ double sh[8];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) s += dist[i] * vol;
+  s = bsum(s, sh);
+  if (threadIdx.x == 0) out[0] = s;
+}
+
+// ------------------------------------------------------------- absorb
+struct AbsorbArgs {
+  int z;
+  int64_t vstride[HBP_MAX_ORDER];  // strides of V (bonds then phys)
+  int vdim[HBP_MAX_ORDER];         // D_k of V
+  int64_t ostride[HBP_MAX_ORDER];  // strides of O (bonds, d_in, d_out)
+  int odim[HBP_MAX_ORDER];         // t_k
+  int64_t nstride[HBP_MAX_ORDER];  // strides of V' (fused bonds, d_out)
+  int ndim[HBP_MAX_ORDER];         // D_k t_k, then d_out
+  int din, dout;
+};
+
+// V'[(v o)..., p'] = sum_p O[o..., p, p'] V[v..., p]  (apply.h:52-66)
+template <typename C>
+__global__ void k_absorb_site(const C *__restrict__ V, const double2 *__restrict__ O,
+                              C *__restrict__ Vn, size_t n, AbsorbArgs a) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (size_t)gridDim.x * blockDim.x) {
+    size_t rem = e;
+    int64_t vb = 0, ob = 0;
+    for (int k = 0; k < a.z; ++k) {
+      const int f = (int)(rem % a.ndim[k]);
+      rem /= a.ndim[k];
+      const int v = f % a.vdim[k], o = f / a.vdim[k];
+      vb += v * a.vstride[k];
+      ob += o * a.ostride[k];
+    }
+    const int pp = (int)rem;
+    ob += pp * a.ostride[a.z + 1];
+    double re = 0.0, im = 0.0;
+    for (int p = 0; p < a.din; ++p) {
+      const C v = V[vb + p * a.vstride[a.z]];
+      const double2 o = O[ob + p * a.ostride[a.z]];
+      re += o.x * (double)v.x - o.y * (double)v.y;
+      im += o.x * (double)v.y + o.y * (double)v.x;
+    }
+    C out;
+    out.x = re;
+    out.y = im;
+    Vn[e] = out;
+  }
+}
+
+// E'[(e0 i0), (e1 i1)] = E[e0, e1] delta(i0, i1)  (apply.h:110-128)
+__global__ void k_expand_message(const double2 *__restrict__ E, double2 *__restrict__ En, int D,
+                                 int t) {
+  const int Dn = D * t;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < Dn * Dn; e += gridDim.x * blockDim.x) {
+    const int r = e % Dn, c = e / Dn;
+    const int e0 = r % D, i0 = r / D, e1 = c % D, i1 = c / D;
+    En[e] = (i0 == i1) ? E[e0 + D * e1] : make_double2(0.0, 0.0);
+  }
+}
+
+// ---------------------------------------------------------- truncation
+struct RootJob {
+  int64_t u_off, s_off, vd_off;  // SVD of the message
+  int64_t r_off, si_off;         // outputs R, S (D x D)
+  int D;
+};
+
+// SquareRootAndInverse (root.h:65-148): R = U diag(d) Vd, S = Vd^H diag(f) U^H
+// with d = sqrt(e) if e > sv_min else 0, f = d / (d d) if d > sqrt(sv_min) else 0.
+__global__ __launch_bounds__(256) void k_roots(const double2 *__restrict__ U,
+                                                const double *__restrict__ sv,
+                                                const double2 *__restrict__ Vd,
+                                                double2 *__restrict__ out,
+                                                const RootJob *__restrict__ jobs, double sv_min) {
+  const RootJob j = jobs[blockIdx.x];
+  const int D = j.D;
+  __shared__ double dd[512], ff[512];
+  for (int c = threadIdx.x; c < D; c += blockDim.x) {
+    const double e = sv[j.s_off + c];
+    const double d = fabs(e) > sv_min ? sqrt(e) : 0.0;
+    dd[c] = d;
+    ff[c] = fabs(d) > sqrt(sv_min) ? d / (d * d) : 0.0;
+  }
+  __syncthreads();
+  const double2 *u = U + j.u_off;
+  const double2 *v = Vd + j.vd_off;
+  for (int e = threadIdx.x; e < D * D; e += blockDim.x) {
+    const int r = e % D, c = e / D;
+    double rr = 0.0, ri = 0.0, sr = 0.0, si = 0.0;
+    for (int k = 0; k < D; ++k) {
+      // R[r,c] = sum_k U[r,k] d_k Vd[k,c]
+      const double2 a = u[r + (int64_t)D * k], b = v[k + (int64_t)D * c];
+      rr += dd[k] * (a.x * b.x - a.y * b.y);
+      ri += dd[k] * (a.x * b.y + a.y * b.x);
+      // S[r,c] = sum_k conj(Vd[k,r]) f_k conj(U[c,k])
+      const double2 x = v[k + (int64_t)D * r], y = u[c + (int64_t)D * k];
+      sr += ff[k] * (x.x * y.x - x.y * y.y);
+      si += -ff[k] * (x.x * y.y + x.y * y.x);
+    }
+    out[j.r_off + e] = make_double2(rr, ri);
+    out[j.si_off + e] = make_double2(sr, si);
+  }
+}
+
+struct EdgeJob {
+  int64_t ra_off, sa_off, rb_off, sb_off;  // in the roots buffer
+  int64_t t_off;                           // T = Ra^T Rb (D x D)
+  int64_t u_off, s_off, vd_off;            // SVD of T
+  int64_t pa_off, pb_off, e_off;           // projectors and new messages
+  int D;
+};
+
+// T = Ra^T Rb, normalised (truncation.h:110-121)
+__global__ __launch_bounds__(256) void k_rr(const double2 *__restrict__ roots,
+                                             double2 *__restrict__ T,
+                                             const EdgeJob *__restrict__ jobs) {
+  __shared__ double sh[8];
+  const EdgeJob j = jobs[blockIdx.x];
+  const int D = j.D;
+  const double2 *ra = roots + j.ra_off, *rb = roots + j.rb_off;
+  double s = 0.0;
+  for (int e = threadIdx.x; e < D * D; e += blockDim.x) {
+    const int r = e % D, c = e / D;
+    double xr = 0.0, xi = 0.0;
+    for (int k = 0; k < D; ++k) {
+      const double2 a = ra[k + (int64_t)D * r], b = rb[k + (int64_t)D * c];
+      xr += a.x * b.x - a.y * b.y;
+      xi += a.x * b.y + a.y * b.x;
+    }
+    T[j.t_off + e] = make_double2(xr, xi);
+    s += xr * xr + xi * xi;
+  }
+  s = bsum(s, sh);
+  const double inv = s > 0.0 ? 1.0 / sqrt(s) : 1.0;
+  __syncthreads();
+  for (int e = threadIdx.x; e < D * D; e += blockDim.x) {
+    const double2 v = T[j.t_off + e];
+    T[j.t_off + e] = make_double2(v.x * inv, v.y * inv);
+  }
+}
+
+// trunc_svd(T, chi, s_min) selection, E = diag(L) for both directions,
+// Pa = Sa^T X sqrt(L) (D x keep), Pb = sqrt(L) Y Sb (keep x D)  (truncation.h:123-217)
+__global__ __launch_bounds__(256) void k_projectors(const double2 *__restrict__ roots,
+                                                     const double2 *__restrict__ U,
+                                                     const double *__restrict__ sv,
+                                                     const double2 *__restrict__ Vd,
+                                                     double2 *__restrict__ proj,
+                                                     const EdgeJob *__restrict__ jobs, int chi,
+                                                     double s_min, int *__restrict__ keep_out,
+                                                     double *__restrict__ err_out) {
+  const EdgeJob j = jobs[blockIdx.x];
+  const int D = j.D;
+  __shared__ int keep_sh;
+  __shared__ double sq[512];
+  if (threadIdx.x == 0) {
+    int cnt = 0;
+    double tot = 0.0, dis = 0.0;
+    for (int c = 0; c < D; ++c)
+      if (sv[j.s_off + c] > s_min) ++cnt;
+    int keep = cnt < chi ? cnt : chi;
+    if (keep < 1) keep = 1;
+    for (int c = 0; c < D; ++c) {
+      const double s = sv[j.s_off + c];
+      tot += s * s;
+      if (c >= keep) dis += s * s;
+    }
+    keep_sh = keep;
+    keep_out[blockIdx.x] = keep;
+    err_out[blockIdx.x] = tot > 0.0 ? dis / tot : 0.0;
+  }
+  __syncthreads();
+  const int keep = keep_sh;
+  for (int c = threadIdx.x; c < keep; c += blockDim.x) sq[c] = sqrt(sv[j.s_off + c]);
+  __syncthreads();
+  const double2 *sa = roots + j.sa_off, *sb = roots + j.sb_off;
+  const double2 *X = U + j.u_off, *Y = Vd + j.vd_off;  // X: D x D (first keep cols), Y: D x D
+  for (int e = threadIdx.x; e < D * keep; e += blockDim.x) {
+    const int i = e % D, c = e / D;
+    double pr = 0.0, pi = 0.0;
+    for (int x = 0; x < D; ++x) {
+      const double2 a = sa[x + (int64_t)D * i], b = X[x + (int64_t)D * c];
+      pr += a.x * b.x - a.y * b.y;
+      pi += a.x * b.y + a.y * b.x;
+    }
+    proj[j.pa_off + e] = make_double2(pr * sq[c], pi * sq[c]);
+  }
+  for (int e = threadIdx.x; e < keep * D; e += blockDim.x) {
+    const int c = e % keep, i = e / keep;
+    double pr = 0.0, pi = 0.0;
+    for (int x = 0; x < D; ++x) {
+      const double2 a = Y[c + (int64_t)D * x], b = sb[x + (int64_t)D * i];
+      pr += a.x * b.x - a.y * b.y;
+      pi += a.x * b.y + a.y * b.x;
+    }
+    proj[j.pb_off + e] = make_double2(pr * sq[c], pi * sq[c]);
+  }
+  for (int e = threadIdx.x; e < keep * keep; e += blockDim.x) {
+    const int r = e % keep, c = e / keep;
+    proj[j.e_off + e] = make_double2(r == c ? sv[j.s_off + r] : 0.0, 0.0);
+  }
+}
+
+// ----------------------------------------------------------- measure
+// <O> = sum rho[p,p'] O[p,p'] / sum rho[p,p]  (measure.h:72-89)
+__global__ void k_site_ratio(const double2 *__restrict__ rho, const double2 *__restrict__ ops,
+                             const int *__restrict__ rho_idx, int d, int n,
+                             double2 *__restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double2 *r = rho + (int64_t)rho_idx[t] * d * d;
+  const double2 *o = ops + (int64_t)t * d * d;
+  double nr = 0.0, ni = 0.0, dr = 0.0, di = 0.0;
+  for (int p = 0; p < d; ++p)
+    for (int q = 0; q < d; ++q) {
+      const double2 a = r[p + d * q], b = o[p + d * q];
+      nr += a.x * b.x - a.y * b.y;
+      ni += a.x * b.y + a.y * b.x;
+      if (p == q) {
+        dr += a.x;
+        di += a.y;
+      }
+    }
+  const double den = dr * dr + di * di;
+  out[t] = make_double2((nr * dr + ni * di) / den, (ni * dr - nr * di) / den);
+}
+
+// R = sum AA[p,p',i,j] O[p,x,p',y] BB[x,y,i,j], S = sum (tr AA)(tr BB)  (measure.h:255-294)
+// AA is stored as d*d blocks M_{p p'}(i, j), block index p + d p'.
+__global__ __launch_bounds__(256) void k_edge_ratio(const double2 *__restrict__ AA,
+                                                     const double2 *__restrict__ BB,
+                                                     const double2 *__restrict__ ops,
+                                                     const int64_t *__restrict__ env_off,
+                                                     const int *__restrict__ Dm, int d,
+                                                     double2 *__restrict__ out) {
+  __shared__ double sh[8];
+  const int t = blockIdx.x;
+  const int D = Dm[t];
+  const int64_t DD = (int64_t)D * D;
+  const double2 *A = AA + env_off[t], *B = BB + env_off[t];
+  const double2 *O = ops + (int64_t)t * d * d * d * d;
+  double rr = 0.0, ri = 0.0, sr = 0.0, si = 0.0;
+  for (int64_t e = threadIdx.x; e < DD; e += blockDim.x) {
+    double tar = 0.0, tai = 0.0, tbr = 0.0, tbi = 0.0;
+    for (int p = 0; p < d; ++p) {
+      const double2 a = A[(p + d * p) * DD + e], b = B[(p + d * p) * DD + e];
+      tar += a.x; tai += a.y; tbr += b.x; tbi += b.y;
+    }
+    sr += tar * tbr - tai * tbi;
+    si += tar * tbi + tai * tbr;
+    for (int p = 0; p < d; ++p)
+      for (int pp = 0; pp < d; ++pp) {
+        const double2 a = A[(p + d * pp) * DD + e];
+        for (int x = 0; x < d; ++x)
+          for (int y = 0; y < d; ++y) {
+            const double2 o = O[p + d * (x + d * (pp + d * y))];
+            const double2 b = B[(x + d * y) * DD + e];
+            const double ar = a.x * o.x - a.y * o.y, ai = a.x * o.y + a.y * o.x;
+            rr += ar * b.x - ai * b.y;
+            ri += ar * b.y + ai * b.x;
+          }
+      }
+  }
+  rr = bsum(rr, sh);
+  ri = bsum(ri, sh);
+  sr = bsum(sr, sh);
+  si = bsum(si, sh);
+  if (threadIdx.x == 0) {
+    const double den = sr * sr + si * si;
+    out[t] = make_double2((rr * sr + ri * si) / den, (ri * sr - rr * si) / den);
+  }
+}
+
+inline int gridn(size_t n) {
+  size_t g = (n + 255) / 256;
+  return (int)std::max<size_t>(1, std::min<size_t>(g, 8192));
+}
+
+}  // namespace
+
+// ================================================================ state
+struct Peps {
+  int precision = HBP_PREC_FP64;
+  int n_sites = 0, n_edges = 0;
+  std::vector<int> ea, eb;                 // edge endpoints (site addresses)
+  std::vector<int> site_label;             // address -> label
+  std::vector<std::vector<int>> bonds;     // address -> incident edge ids ascending
+  std::vector<int> phys;                   // address -> d
+  std::vector<int> bond_dim;               // edge -> D
+  std::vector<void *> V;                   // address -> device tensor (S precision)
+  std::vector<size_t> V_bytes;
+  double2 *E = nullptr;                    // all 2 n_e messages
+  size_t E_bytes = 0;
+  std::vector<int64_t> E_off;              // message -> offset (complex<double>)
+  // work buffers
+  void *buf[2] = {nullptr, nullptr};
+  size_t buf_bytes = 0;
+
+  int elem_bytes() const { return precision == HBP_PREC_FP64 ? 16 : 8; }
+  int site_dtype() const { return precision == HBP_PREC_FP64 ? HBP_C128 : HBP_C64; }
+  int z(int s) const { return (int)bonds[s].size(); }
+  int in_msg(int s, int k) const {  // message entering site s on its bond k
+    const int m = bonds[s][k];
+    return ea[m] == s ? m + n_edges : m;
+  }
+  int out_msg(int s, int k) const {  // message leaving site s along bond k
+    const int m = bonds[s][k];
+    return ea[m] == s ? m : m + n_edges;
+  }
+  void shape(int s, int32_t *sh) const {
+    for (int k = 0; k < z(s); ++k) sh[k] = bond_dim[bonds[s][k]];
+    sh[z(s)] = phys[s];
+  }
+  size_t site_size(int s) const {
+    size_t n = phys[s];
+    for (int m : bonds[s]) n *= (size_t)bond_dim[m];
+    return n;
+  }
+  int msg_dim(int m) const { return bond_dim[m % n_edges]; }
+};
+
+static void layout_messages(Ctx *ctx, Peps *p, double2 **newE, size_t *bytes,
+                            std::vector<int64_t> *off) {
+  off->resize(2 * p->n_edges);
+  int64_t o = 0;
+  for (int m = 0; m < 2 * p->n_edges; ++m) {
+    (*off)[m] = o;
+    const int64_t D = p->msg_dim(m);
+    o += D * D;
+  }
+  *bytes = (size_t)std::max<int64_t>(o, 1) * 16;
+  *newE = (double2 *)ctx->alloc(*bytes);
+}
+
+static void ensure_work(Ctx *ctx, Peps *p, size_t bytes) {
+  if (bytes <= p->buf_bytes) return;
+  for (int i = 0; i < 2; ++i)
+    if (p->buf[i]) ctx->release(p->buf[i], p->buf_bytes);
+  for (int i = 0; i < 2; ++i) p->buf[i] = ctx->alloc(bytes);
+  p->buf_bytes = bytes;
+}
+
+template <typename S>
+static void mode_prod(Ctx *ctx, const void *T, const double2 *E, void *Out, const int32_t *sh,
+                      int order, int k, int Dout, int e_trans) {
+  int64_t L = 1, R = 1;
+  for (int d = 0; d < k; ++d) L *= sh[d];
+  for (int d = k + 1; d < order; ++d) R *= sh[d];
+  launch_mode_product<S>(ctx->stream, T, E, Out, L, sh[k], Dout, R, e_trans, 1.0, nullptr,
+                         nullptr);
+}
+
+// raw M[i, j] = sum T[.. i ..] conj V[.. j ..] over all axes but k (both same shape)
+template <typename S>
+static void message_raw(Ctx *ctx, const void *T, const void *V, const int32_t *sh, int order,
+                        int k, double2 *M) {
+  int64_t L = 1, R = 1;
+  for (int d = 0; d < k; ++d) L *= sh[d];
+  for (int d = k + 1; d < order; ++d) R *= sh[d];
+  const int D = sh[k];
+  const int64_t nch = message_num_chunks(L * R);
+  double2 *parts = (double2 *)ctx->get_scratch((size_t)nch * D * D * 16 + 64);
+  const int n = message_partials<S>(ctx->stream, T, V, parts, L, D, D, R, L * D, L * D, nch, 1);
+  launch_reduce_partials(ctx->stream, parts, n, D * D, M);
+}
+
+// T = V_s with every incoming message absorbed except on bond `skip` (-1: none),
+// returns the buffer holding it (V_s itself when nothing is absorbed).
+template <typename S>
+static const void *absorb_env(Ctx *ctx, Peps *p, int s, int skip, int32_t *sh) {
+  const int order = p->z(s) + 1;
+  p->shape(s, sh);
+  const void *cur = p->V[s];
+  int which = 0;
+  for (int k = 0; k < p->z(s); ++k) {
+    if (k == skip) continue;
+    const int mi = p->in_msg(s, k);
+    void *dst = p->buf[which];
+    mode_prod<S>(ctx, cur, p->E + p->E_off[mi], dst, sh, order, k, sh[k], 0);
+    cur = dst;
+    which ^= 1;
+  }
+  return cur;
+}
+
+template <typename S>
+static void sweep_raw(Ctx *ctx, Peps *p, double2 *raw, const std::vector<int64_t> &raw_off) {
+  size_t maxb = 16;
+  for (int s = 0; s < p->n_sites; ++s) maxb = std::max(maxb, p->site_size(s) * p->elem_bytes());
+  ensure_work(ctx, p, maxb);
+  int32_t sh[HBP_MAX_ORDER];
+  for (int s = 0; s < p->n_sites; ++s) {
+    for (int k = 0; k < p->z(s); ++k) {
+      const void *T = absorb_env<S>(ctx, p, s, k, sh);
+      message_raw<S>(ctx, T, p->V[s], sh, p->z(s) + 1, k, raw + raw_off[p->out_msg(s, k)]);
+    }
+  }
+}
+
+// one BP sweep: newE = normalise(BP(E)); returns residual on device if dist != null
+static void sweep(Ctx *ctx, Peps *p, double2 *newE, bool with_residual, double *dev_res) {
+  const int nm = 2 * p->n_edges;
+  // job table first: its upload waits on the stream, which is then still short
+  std::vector<MsgJob> jobs(nm);
+  for (int m = 0; m < nm; ++m) jobs[m] = MsgJob{p->E_off[m], p->E_off[m], p->E_off[m], p->msg_dim(m)};
+  MsgJob *dj = (MsgJob *)ctx->alloc(nm * sizeof(MsgJob));
+  ctx->upload(dj, jobs.data(), nm * sizeof(MsgJob));
+  double2 *raw = (double2 *)ctx->alloc(std::max<size_t>(p->E_bytes, 16));
+  if (p->precision == HBP_PREC_FP64)
+    sweep_raw<double>(ctx, p, raw, p->E_off);
+  else
+    sweep_raw<float>(ctx, p, raw, p->E_off);
+  double *dist = (double *)ctx->alloc(nm * sizeof(double));
+  hipLaunchKernelGGL(k_finalize_messages, dim3(nm), dim3(256), 0, ctx->stream, raw, newE,
+                     with_residual ? p->E : nullptr, dj, dist);
+  HBP_HIP(hipGetLastError());
+  if (with_residual) {
+    hipLaunchKernelGGL(k_residual, dim3(1), dim3(256), 0, ctx->stream, dist, nm,
+                       1.0 / (2.0 * p->n_edges), dev_res);
+    HBP_HIP(hipGetLastError());
+  }
+  ctx->release(dist, nm * sizeof(double));
+  ctx->release(dj, nm * sizeof(MsgJob));
+  ctx->release(raw, std::max<size_t>(p->E_bytes, 16));
+}
+
+// ----------------------------------------------------------- truncation
+template <typename S>
+static void apply_projectors(Ctx *ctx, Peps *p, const double2 *proj,
+                             const std::vector<EdgeJob> &ej, const std::vector<int> &keep) {
+  // new bond dims after all projectors
+  std::vector<int> old_dim = p->bond_dim;
+  for (int s = 0; s < p->n_sites; ++s) {
+    const int z = p->z(s);
+    int32_t sh[HBP_MAX_ORDER];
+    for (int k = 0; k < z; ++k) sh[k] = old_dim[p->bonds[s][k]];
+    sh[z] = p->phys[s];
+    size_t maxn = 1;
+    {
+      size_t n = p->phys[s];
+      for (int k = 0; k < z; ++k) n *= (size_t)sh[k];
+      maxn = n;
+    }
+    ensure_work(ctx, p, std::max(p->buf_bytes, maxn * p->elem_bytes()));
+    const void *cur = p->V[s];
+    int which = 0;
+    for (int k = 0; k < z; ++k) {
+      const int m = p->bonds[s][k];
+      const int D = old_dim[m], kp = keep[m];
+      void *dst = p->buf[which];
+      if (p->ea[m] == s)
+        mode_prod<S>(ctx, cur, proj + ej[m].pa_off, dst, sh, z + 1, k, kp, 0);
+      else
+        mode_prod<S>(ctx, cur, proj + ej[m].pb_off, dst, sh, z + 1, k, kp, 1);
+      (void)D;
+      sh[k] = kp;
+      cur = dst;
+      which ^= 1;
+    }
+    size_t n = p->phys[s];
+    for (int k = 0; k < z; ++k) n *= (size_t)sh[k];
+    const size_t bytes = n * p->elem_bytes();
+    void *nv = ctx->alloc(bytes);
+    HBP_HIP(hipMemcpyAsync(nv, cur, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    // normalise the site (truncation.h:185, :216)
+    double *dv = (double *)ctx->alloc(sizeof(double));
+    device_norm2(ctx, nv, n, p->site_dtype(), dv);
+    launch_scale(ctx->stream, nv, n, p->site_dtype(), 1.0, 0.0, dv);
+    ctx->release(dv, sizeof(double));
+    ctx->release(p->V[s], p->V_bytes[s]);
+    p->V[s] = nv;
+    p->V_bytes[s] = bytes;
+  }
+}
+
+static void truncate(Ctx *ctx, Peps *p, int chi, double sv_min, int32_t *dims_out,
+                     double *errs_out) {
+  const int ne = p->n_edges, nm = 2 * ne;
+  // 1. SVD of every message
+  int64_t tot = 0;
+  for (int m = 0; m < nm; ++m) tot += (int64_t)p->msg_dim(m) * p->msg_dim(m);
+  int maxD = 1;
+  for (int m = 0; m < ne; ++m) maxD = std::max(maxD, p->bond_dim[m]);
+  HBP_REQUIRE(maxD <= 512, HBP_ERR_UNSUPPORTED, "truncation: bond > 512");
+  double2 *U = (double2 *)ctx->alloc(tot * 16);
+  double2 *Vd = (double2 *)ctx->alloc(tot * 16);
+  double2 *W = (double2 *)ctx->alloc(2 * tot * 16);
+  double *sv = (double *)ctx->alloc((size_t)nm * maxD * 8);
+  std::vector<SvdJob> sj(nm);
+  for (int m = 0; m < nm; ++m) {
+    const int D = p->msg_dim(m);
+    sj[m] = SvdJob{p->E_off[m], p->E_off[m], (int64_t)m * maxD, p->E_off[m], 2 * p->E_off[m], D, D};
+  }
+  SvdJob *dsj = (SvdJob *)ctx->alloc(nm * sizeof(SvdJob));
+  ctx->upload(dsj, sj.data(), nm * sizeof(SvdJob));
+  launch_batched_svd(ctx->stream, p->E, U, sv, Vd, W, dsj, nm, maxD);
+  // 2. square roots and pseudo-inverse roots
+  double2 *roots = (double2 *)ctx->alloc(2 * tot * 16);
+  std::vector<RootJob> rj(nm);
+  for (int m = 0; m < nm; ++m)
+    rj[m] = RootJob{p->E_off[m], (int64_t)m * maxD, p->E_off[m], 2 * p->E_off[m],
+                    2 * p->E_off[m] + (int64_t)p->msg_dim(m) * p->msg_dim(m), p->msg_dim(m)};
+  RootJob *drj = (RootJob *)ctx->alloc(nm * sizeof(RootJob));
+  ctx->upload(drj, rj.data(), nm * sizeof(RootJob));
+  hipLaunchKernelGGL(k_roots, dim3(nm), dim3(256), 0, ctx->stream, U, sv, Vd, roots, drj, sv_min);
+  HBP_HIP(hipGetLastError());
+  // 3. T = Ra^T Rb per edge, normalised; 4. SVD of T
+  int64_t tt = 0;
+  std::vector<EdgeJob> ej(ne);
+  int64_t poff = 0;
+  for (int m = 0; m < ne; ++m) {
+    const int D = p->bond_dim[m];
+    EdgeJob j{};
+    j.D = D;
+    j.ra_off = rj[m].r_off;           // M_{a->b} = E[m]
+    j.sa_off = rj[m].si_off;
+    j.rb_off = rj[m + ne].r_off;      // M_{b->a} = E[m + ne]
+    j.sb_off = rj[m + ne].si_off;
+    j.t_off = tt;
+    j.u_off = tt;
+    j.vd_off = tt;
+    j.s_off = (int64_t)m * maxD;
+    j.pa_off = poff;
+    j.pb_off = poff + (int64_t)D * D;
+    j.e_off = poff + 2 * (int64_t)D * D;
+    poff += 3 * (int64_t)D * D;
+    tt += (int64_t)D * D;
+    ej[m] = j;
+  }
+  double2 *T = (double2 *)ctx->alloc(std::max<int64_t>(tt, 1) * 16);
+  EdgeJob *dej = (EdgeJob *)ctx->alloc(ne * sizeof(EdgeJob));
+  ctx->upload(dej, ej.data(), ne * sizeof(EdgeJob));
+  hipLaunchKernelGGL(k_rr, dim3(ne), dim3(256), 0, ctx->stream, roots, T, dej);
+  HBP_HIP(hipGetLastError());
+  std::vector<SvdJob> tj(ne);
+  for (int m = 0; m < ne; ++m) {
+    const int D = p->bond_dim[m];
+    tj[m] = SvdJob{ej[m].t_off, ej[m].u_off, ej[m].s_off, ej[m].vd_off, 2 * ej[m].t_off, D, D};
+  }
+  ctx->upload(dsj, tj.data(), ne * sizeof(SvdJob));
+  launch_batched_svd(ctx->stream, T, U, sv, Vd, W, dsj, ne, maxD);
+  // 5. selection and projectors
+  double2 *proj = (double2 *)ctx->alloc(std::max<int64_t>(poff, 1) * 16);
+  int *dkeep = (int *)ctx->alloc(ne * sizeof(int));
+  double *derr = (double *)ctx->alloc(ne * sizeof(double));
+  hipLaunchKernelGGL(k_projectors, dim3(ne), dim3(256), 0, ctx->stream, roots, U, sv, Vd, proj,
+                     dej, chi, sv_min, dkeep, derr);
+  HBP_HIP(hipGetLastError());
+  std::vector<int> keep(ne);
+  std::vector<double> err(ne);
+  HBP_HIP(hipMemcpyAsync(keep.data(), dkeep, ne * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  HBP_HIP(hipMemcpyAsync(err.data(), derr, ne * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  ctx->sync();
+  // 6. projectors into the sites
+  if (p->precision == HBP_PREC_FP64)
+    apply_projectors<double>(ctx, p, proj, ej, keep);
+  else
+    apply_projectors<float>(ctx, p, proj, ej, keep);
+  // 7. messages = diag(L) in both directions
+  for (int m = 0; m < ne; ++m) p->bond_dim[m] = keep[m];
+  double2 *nE;
+  size_t nbytes;
+  std::vector<int64_t> noff;
+  layout_messages(ctx, p, &nE, &nbytes, &noff);
+  for (int m = 0; m < ne; ++m) {
+    const size_t b = (size_t)keep[m] * keep[m] * 16;
+    HBP_HIP(hipMemcpyAsync(nE + noff[m], proj + ej[m].e_off, b, hipMemcpyDeviceToDevice, ctx->stream));
+    HBP_HIP(hipMemcpyAsync(nE + noff[m + ne], proj + ej[m].e_off, b, hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  ctx->release(p->E, p->E_bytes);
+  p->E = nE;
+  p->E_bytes = nbytes;
+  p->E_off = noff;
+  if (dims_out)
+    for (int m = 0; m < ne; ++m) dims_out[m] = keep[m];
+  if (errs_out)
+    for (int m = 0; m < ne; ++m) errs_out[m] = err[m];
+  ctx->release(dkeep, ne * sizeof(int));
+  ctx->release(derr, ne * sizeof(double));
+  ctx->release(proj, std::max<int64_t>(poff, 1) * 16);
+  ctx->release(dej, ne * sizeof(EdgeJob));
+  ctx->release(T, std::max<int64_t>(tt, 1) * 16);
+  ctx->release(drj, nm * sizeof(RootJob));
+  ctx->release(roots, 2 * tot * 16);
+  ctx->release(dsj, nm * sizeof(SvdJob));
+  ctx->release(sv, (size_t)nm * maxD * 8);
+  ctx->release(W, 2 * tot * 16);
+  ctx->release(Vd, tot * 16);
+  ctx->release(U, tot * 16);
+}
+
+// ------------------------------------------------------------- measure
+template <typename S>
+static void measure_sites_impl(Ctx *ctx, Peps *p, int n, const int *addr, const double *ops,
+                               double *out) {
+  // reduced matrices rho_s[p, p'] for each distinct requested site
+  std::vector<int> uniq(addr, addr + n);
+  std::sort(uniq.begin(), uniq.end());
+  uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+  int maxd = 1;
+  for (int s : uniq) maxd = std::max(maxd, p->phys[s]);
+  size_t maxb = 16;
+  for (int s : uniq) maxb = std::max(maxb, p->site_size(s) * p->elem_bytes());
+  ensure_work(ctx, p, maxb);
+  double2 *rho = (double2 *)ctx->alloc(uniq.size() * maxd * maxd * 16);
+  int32_t sh[HBP_MAX_ORDER];
+  for (size_t u = 0; u < uniq.size(); ++u) {
+    const int s = uniq[u];
+    const void *W = absorb_env<S>(ctx, p, s, -1, sh);
+    // rho[p, p'] = sum_b W[b, p] conj V[b, p']: message over the physical axis
+    message_raw<S>(ctx, W, p->V[s], sh, p->z(s) + 1, p->z(s), rho + u * maxd * maxd);
+  }
+  std::vector<int> idx(n);
+  for (int t = 0; t < n; ++t) {
+    idx[t] = (int)(std::lower_bound(uniq.begin(), uniq.end(), addr[t]) - uniq.begin());
+    HBP_REQUIRE(p->phys[addr[t]] == maxd, HBP_ERR_UNSUPPORTED, "mixed physical dims in one call");
+  }
+  const int d = maxd;
+  double2 *dops = (double2 *)ctx->alloc((size_t)n * d * d * 16);
+  int *didx = (int *)ctx->alloc(n * sizeof(int));
+  double2 *dout = (double2 *)ctx->alloc(n * 16);
+  ctx->upload(dops, ops, (size_t)n * d * d * 16);
+  ctx->upload(didx, idx.data(), n * sizeof(int));
+  hipLaunchKernelGGL(k_site_ratio, dim3((n + 127) / 128), dim3(128), 0, ctx->stream, rho, dops,
+                     didx, d, n, dout);
+  HBP_HIP(hipGetLastError());
+  HBP_HIP(hipMemcpyAsync(out, dout, n * 16, hipMemcpyDeviceToHost, ctx->stream));
+  ctx->sync();
+  ctx->release(dout, n * 16);
+  ctx->release(didx, n * sizeof(int));
+  ctx->release(dops, (size_t)n * d * d * 16);
+  ctx->release(rho, uniq.size() * maxd * maxd * 16);
+}
+
+// AdagA blocks M_{p p'}(i, j) = sum A[.., i, .., p] conj V[.., j, .., p'] (measure.h:194-204)
+template <typename S>
+static void edge_env(Ctx *ctx, Peps *p, int s, int kt, double2 *out) {
+  int32_t sh[HBP_MAX_ORDER];
+  const void *A = absorb_env<S>(ctx, p, s, kt, sh);
+  const int z = p->z(s), d = p->phys[s];
+  int64_t L = 1, R = 1;
+  for (int k = 0; k < kt; ++k) L *= sh[k];
+  for (int k = kt + 1; k < z; ++k) R *= sh[k];
+  const int D = sh[kt];
+  const int64_t slab = L * D * R;  // elements per physical index
+  const int64_t nch = message_num_chunks(L * R);
+  double2 *parts = (double2 *)ctx->get_scratch((size_t)nch * D * D * 16 + 64);
+  const size_t es = p->elem_bytes();
+  for (int pp = 0; pp < d; ++pp)
+    for (int q = 0; q < d; ++q) {
+      const void *Ap = (const char *)A + (size_t)pp * slab * es;
+      const void *Vq = (const char *)p->V[s] + (size_t)q * slab * es;
+      const int nc = message_partials<S>(ctx->stream, Ap, Vq, parts, L, D, D, R, L * D, L * D,
+                                         nch, 1);
+      launch_reduce_partials(ctx->stream, parts, nc, D * D, out + (int64_t)(pp + d * q) * D * D);
+    }
+}
+
+template <typename S>
+static void measure_edges_impl(Ctx *ctx, Peps *p, int n, const int *pa, const int *pb,
+                               const double *ops, double *out) {
+  size_t maxb = 16;
+  for (int s = 0; s < p->n_sites; ++s) maxb = std::max(maxb, p->site_size(s) * p->elem_bytes());
+  ensure_work(ctx, p, maxb);
+  std::vector<int64_t> off(n);
+  std::vector<int> Dm(n);
+  int64_t tot = 0;
+  int d = 0;
+  std::vector<int> ka(n), kb(n);
+  for (int t = 0; t < n; ++t) {
+    const int a = pa[t], b = pb[t];
+    int m = -1;
+    for (int k = 0; k < p->z(a); ++k) {
+      const int e = p->bonds[a][k];
+      if ((p->ea[e] == a && p->eb[e] == b) || (p->ea[e] == b && p->eb[e] == a)) {
+        m = e;
+        ka[t] = k;
+      }
+    }
+    HBP_REQUIRE(m >= 0, HBP_ERR_INVALID_ARGUMENT, "measure: sites are not nearest neighbours");
+    kb[t] = (int)(std::find(p->bonds[b].begin(), p->bonds[b].end(), m) - p->bonds[b].begin());
+    HBP_REQUIRE(p->phys[a] == p->phys[b] && (d == 0 || d == p->phys[a]), HBP_ERR_UNSUPPORTED,
+                "measure: mixed physical dims");
+    d = p->phys[a];
+    Dm[t] = p->bond_dim[m];
+    off[t] = tot;
+    tot += (int64_t)d * d * Dm[t] * Dm[t];
+  }
+  double2 *AA = (double2 *)ctx->alloc(std::max<int64_t>(tot, 1) * 16);
+  double2 *BB = (double2 *)ctx->alloc(std::max<int64_t>(tot, 1) * 16);
+  for (int t = 0; t < n; ++t) {
+    edge_env<S>(ctx, p, pa[t], ka[t], AA + off[t]);
+    edge_env<S>(ctx, p, pb[t], kb[t], BB + off[t]);
+  }
+  const size_t ob = (size_t)n * d * d * d * d * 16;
+  double2 *dops = (double2 *)ctx->alloc(ob);
+  int64_t *doff = (int64_t *)ctx->alloc(n * sizeof(int64_t));
+  int *dD = (int *)ctx->alloc(n * sizeof(int));
+  double2 *dout = (double2 *)ctx->alloc(n * 16);
+  ctx->upload(dops, ops, ob);
+  ctx->upload(doff, off.data(), n * sizeof(int64_t));
+  ctx->upload(dD, Dm.data(), n * sizeof(int));
+  hipLaunchKernelGGL(k_edge_ratio, dim3(n), dim3(256), 0, ctx->stream, AA, BB, dops, doff, dD, d,
+                     dout);
+  HBP_HIP(hipGetLastError());
+  HBP_HIP(hipMemcpyAsync(out, dout, n * 16, hipMemcpyDeviceToHost, ctx->stream));
+  ctx->sync();
+  ctx->release(dout, n * 16);
+  ctx->release(dD, n * sizeof(int));
+  ctx->release(doff, n * sizeof(int64_t));
+  ctx->release(dops, ob);
+  ctx->release(BB, std::max<int64_t>(tot, 1) * 16);
+  ctx->release(AA, std::max<int64_t>(tot, 1) * 16);
+}
+
+}  // namespace hbp
+
+using namespace hbp;
+struct hbp_peps : hbp::Peps {};
+
+#define API_BEGIN try {
+#define API_END                                   \
+  }                                               \
+  catch (const ::hbp::Error &e) {                 \
+    set_last_error(e.what());                     \
+    return e.code;                                \
+  }                                               \
+  catch (const std::bad_alloc &) {                \
+    set_last_error("bad_alloc");                  \
+    return HBP_ERR_BAD_ALLOC;                     \
+  }                                               \
+  catch (const std::exception &e) {               \
+    set_last_error(e.what());                     \
+    return HBP_ERR_LOGIC;                         \
+  }                                               \
+  return HBP_OK;
+
+static int site_addr(const hbp_peps *p, int label) {
+  auto it = std::lower_bound(p->site_label.begin(), p->site_label.end(), label);
+  HBP_REQUIRE(it != p->site_label.end() && *it == label, HBP_ERR_OUT_OF_RANGE, "unknown site");
+  return (int)(it - p->site_label.begin());
+}
+
+extern "C" {
+
+hbp_status hbp_peps_create(hbp_ctx *ctx, int n_edges, const int32_t *edges,
+                           const int32_t *phys_dims, int precision, hbp_peps **out) {
+  API_BEGIN
+  HBP_REQUIRE(n_edges >= 1, HBP_ERR_INVALID_ARGUMENT, "peps: no edges");
+  HBP_REQUIRE(precision == HBP_PREC_FP64 || precision == HBP_PREC_MIXED, HBP_ERR_INVALID_ARGUMENT,
+              "peps: unknown precision");
+  hbp_peps *p = new hbp_peps();
+  p->precision = precision;
+  std::vector<int> labels;
+  for (int m = 0; m < n_edges; ++m) {
+    HBP_REQUIRE(edges[2 * m] != edges[2 * m + 1], HBP_ERR_INVALID_ARGUMENT, "peps: self loop");
+    labels.push_back(edges[2 * m]);
+    labels.push_back(edges[2 * m + 1]);
+  }
+  std::sort(labels.begin(), labels.end());
+  labels.erase(std::unique(labels.begin(), labels.end()), labels.end());
+  p->site_label = labels;
+  p->n_sites = (int)labels.size();
+  p->n_edges = n_edges;
+  p->bonds.assign(p->n_sites, {});
+  for (int m = 0; m < n_edges; ++m) {
+    const int a = site_addr(p, edges[2 * m]), b = site_addr(p, edges[2 * m + 1]);
+    p->ea.push_back(a);
+    p->eb.push_back(b);
+    p->bonds[a].push_back(m);
+    p->bonds[b].push_back(m);
+  }
+  for (int s = 0; s < p->n_sites; ++s) {
+    HBP_REQUIRE((int)p->bonds[s].size() + 1 <= HBP_MAX_ORDER, HBP_ERR_UNSUPPORTED,
+                "peps: site degree above 7");
+    p->phys.push_back(phys_dims ? phys_dims[s] : 2);
+    HBP_REQUIRE(p->phys[s] >= 1 && p->phys[s] <= 16, HBP_ERR_UNSUPPORTED, "peps: physical dim");
+  }
+  p->bond_dim.assign(n_edges, 1);
+  // |0>: bonds of dimension 1, V[0] = 1 (init.h:57-60)
+  p->V.resize(p->n_sites);
+  p->V_bytes.resize(p->n_sites);
+  for (int s = 0; s < p->n_sites; ++s) {
+    const size_t bytes = (size_t)p->phys[s] * p->elem_bytes();
+    p->V[s] = ctx->alloc(bytes);
+    p->V_bytes[s] = bytes;
+    launch_fill(ctx->stream, p->V[s], p->phys[s], p->site_dtype(), 0.0, 0.0);
+    launch_fill(ctx->stream, p->V[s], 1, p->site_dtype(), 1.0, 0.0);
+  }
+  // 1x1 messages equal to one (init.h:72-78)
+  layout_messages(ctx, p, &p->E, &p->E_bytes, &p->E_off);
+  launch_fill(ctx->stream, p->E, 2 * n_edges, HBP_C128, 1.0, 0.0);
+  ctx->sync();
+  *out = p;
+  API_END
+}
+
+hbp_status hbp_peps_destroy(hbp_ctx *ctx, hbp_peps *p) {
+  API_BEGIN
+  if (!p) return HBP_OK;
+  for (int s = 0; s < p->n_sites; ++s) ctx->release(p->V[s], p->V_bytes[s]);
+  ctx->release(p->E, p->E_bytes);
+  for (int i = 0; i < 2; ++i)
+    if (p->buf[i]) ctx->release(p->buf[i], p->buf_bytes);
+  ctx->sync();
+  delete p;
+  API_END
+}
+
+hbp_status hbp_peps_num_sites(const hbp_peps *p, int *n_sites, int *n_edges) {
+  API_BEGIN
+  if (n_sites) *n_sites = p->n_sites;
+  if (n_edges) *n_edges = p->n_edges;
+  API_END
+}
+
+hbp_status hbp_peps_site_shape(const hbp_peps *p, int site, int *order, int32_t *shape) {
+  API_BEGIN
+  HBP_REQUIRE(site >= 0 && site < p->n_sites, HBP_ERR_OUT_OF_RANGE, "site out of range");
+  *order = p->z(site) + 1;
+  p->shape(site, shape);
+  API_END
+}
+
+hbp_status hbp_peps_get_site(hbp_ctx *ctx, hbp_peps *p, int site, double *host) {
+  API_BEGIN
+  HBP_REQUIRE(site >= 0 && site < p->n_sites, HBP_ERR_OUT_OF_RANGE, "site out of range");
+  const size_t n = p->site_size(site);
+  double2 *tmp = (double2 *)ctx->alloc(n * 16);
+  launch_convert(ctx->stream, p->V[site], p->site_dtype(), tmp, HBP_C128, n);
+  HBP_HIP(hipMemcpyAsync(host, tmp, n * 16, hipMemcpyDeviceToHost, ctx->stream));
+  ctx->sync();
+  ctx->release(tmp, n * 16);
+  API_END
+}
+
+hbp_status hbp_peps_set_site(hbp_ctx *ctx, hbp_peps *p, int site, int order,
+                             const int32_t *shape, const double *host) {
+  API_BEGIN
+  HBP_REQUIRE(site >= 0 && site < p->n_sites, HBP_ERR_OUT_OF_RANGE, "site out of range");
+  HBP_REQUIRE(order == p->z(site) + 1, HBP_ERR_INVALID_ARGUMENT, "set_site: order");
+  HBP_REQUIRE(shape[order - 1] == p->phys[site], HBP_ERR_INVALID_ARGUMENT, "set_site: phys dim");
+  for (int k = 0; k < p->z(site); ++k)
+    HBP_REQUIRE(shape[k] == p->bond_dim[p->bonds[site][k]], HBP_ERR_INVALID_ARGUMENT,
+                "set_site: bond dims must match the current edge dims");
+  const size_t n = p->site_size(site);
+  double2 *tmp = (double2 *)ctx->alloc(n * 16);
+  ctx->upload(tmp, host, n * 16);
+  launch_convert(ctx->stream, tmp, HBP_C128, p->V[site], p->site_dtype(), n);
+  ctx->sync();
+  ctx->release(tmp, n * 16);
+  API_END
+}
+
+hbp_status hbp_peps_get_message(hbp_ctx *ctx, hbp_peps *p, int m, int32_t *dim, double *host) {
+  API_BEGIN
+  HBP_REQUIRE(m >= 0 && m < 2 * p->n_edges, HBP_ERR_OUT_OF_RANGE, "message out of range");
+  const int D = p->msg_dim(m);
+  if (dim) *dim = D;
+  if (host) {
+    HBP_HIP(hipMemcpyAsync(host, p->E + p->E_off[m], (size_t)D * D * 16, hipMemcpyDeviceToHost,
+                           ctx->stream));
+    ctx->sync();
+  }
+  API_END
+}
+
+hbp_status hbp_peps_set_message(hbp_ctx *ctx, hbp_peps *p, int m, int32_t dim,
+                                const double *host) {
+  API_BEGIN
+  HBP_REQUIRE(m >= 0 && m < 2 * p->n_edges, HBP_ERR_OUT_OF_RANGE, "message out of range");
+  HBP_REQUIRE(dim == p->msg_dim(m), HBP_ERR_INVALID_ARGUMENT, "set_message: dim");
+  HBP_HIP(hipMemcpyAsync(p->E + p->E_off[m], host, (size_t)dim * dim * 16, hipMemcpyHostToDevice,
+                         ctx->stream));
+  ctx->sync();
+  API_END
+}
+
+hbp_status hbp_peps_absorb_tpo(hbp_ctx *ctx, hbp_peps *p, const double *const *ops,
+                               const int32_t *tpo_shapes) {
+  API_BEGIN
+  const int ns = p->n_sites, ne = p->n_edges;
+  // TPO bond dims per edge, checked at both ends
+  std::vector<int> t(ne, 0);
+  for (int s = 0; s < ns; ++s) {
+    const int32_t *sh = tpo_shapes + (size_t)s * HBP_MAX_ORDER;
+    const int z = p->z(s);
+    HBP_REQUIRE(sh[z] == p->phys[s], HBP_ERR_INVALID_ARGUMENT, "absorb: d_in differs from phys");
+    for (int k = 0; k < z; ++k) {
+      const int m = p->bonds[s][k];
+      HBP_REQUIRE(sh[k] >= 1, HBP_ERR_INVALID_ARGUMENT, "absorb: TPO bond < 1");
+      HBP_REQUIRE(t[m] == 0 || t[m] == sh[k], HBP_ERR_INVALID_ARGUMENT,
+                  "absorb: TPO bond dims differ across an edge");
+      t[m] = sh[k];
+    }
+  }
+  for (int s = 0; s < ns; ++s) {
+    const int32_t *osh = tpo_shapes + (size_t)s * HBP_MAX_ORDER;
+    const int z = p->z(s);
+    AbsorbArgs a{};
+    a.z = z;
+    a.din = osh[z];
+    a.dout = osh[z + 1];
+    int64_t vs = 1, os = 1, ns_ = 1;
+    size_t on = 1;
+    for (int k = 0; k < z; ++k) {
+      const int D = p->bond_dim[p->bonds[s][k]];
+      a.vdim[k] = D;
+      a.vstride[k] = vs;
+      vs *= D;
+      a.odim[k] = osh[k];
+      a.ostride[k] = os;
+      os *= osh[k];
+      a.ndim[k] = D * osh[k];
+      a.nstride[k] = ns_;
+      ns_ *= D * osh[k];
+    }
+    a.vstride[z] = vs;
+    a.ostride[z] = os;
+    a.ostride[z + 1] = os * osh[z];
+    a.ndim[z] = a.dout;
+    for (int k = 0; k <= z + 1; ++k) on *= (size_t)osh[k];
+    const size_t n = (size_t)ns_ * a.dout;
+    double2 *dO = (double2 *)ctx->alloc(on * 16);
+    HBP_HIP(hipMemcpyAsync(dO, ops[s], on * 16, hipMemcpyHostToDevice, ctx->stream));
+    const size_t bytes = n * p->elem_bytes();
+    void *nv = ctx->alloc(bytes);
+    if (p->precision == HBP_PREC_FP64)
+      hipLaunchKernelGGL(k_absorb_site<double2>, dim3(gridn(n)), dim3(256), 0, ctx->stream,
+                         (const double2 *)p->V[s], dO, (double2 *)nv, n, a);
+    else
+      hipLaunchKernelGGL(k_absorb_site<float2>, dim3(gridn(n)), dim3(256), 0, ctx->stream,
+                         (const float2 *)p->V[s], dO, (float2 *)nv, n, a);
+    HBP_HIP(hipGetLastError());
+    ctx->release(dO, on * 16);
+    ctx->release(p->V[s], p->V_bytes[s]);
+    p->V[s] = nv;
+    p->V_bytes[s] = bytes;
+    p->phys[s] = a.dout;
+  }
+  std::vector<int> old = p->bond_dim;
+  std::vector<int64_t> old_off = p->E_off;
+  for (int m = 0; m < ne; ++m) p->bond_dim[m] = old[m] * t[m];
+  double2 *nE;
+  size_t nbytes;
+  std::vector<int64_t> noff;
+  layout_messages(ctx, p, &nE, &nbytes, &noff);
+  for (int m = 0; m < 2 * ne; ++m) {
+    const int D = old[m % ne], tt = t[m % ne];
+    hipLaunchKernelGGL(k_expand_message, dim3(gridn((size_t)D * tt * D * tt)), dim3(256), 0,
+                       ctx->stream, p->E + old_off[m], nE + noff[m], D, tt);
+    HBP_HIP(hipGetLastError());
+  }
+  ctx->release(p->E, p->E_bytes);
+  p->E = nE;
+  p->E_bytes = nbytes;
+  p->E_off = noff;
+  ctx->sync();  // the caller's operator arrays may be freed on return
+  API_END
+}
+
+hbp_status hbp_peps_bp_sweep(hbp_ctx *ctx, hbp_peps *p) {
+  API_BEGIN
+  double2 *nE = (double2 *)ctx->alloc(p->E_bytes);
+  sweep(ctx, p, nE, false, nullptr);
+  ctx->release(p->E, p->E_bytes);
+  p->E = nE;
+  API_END
+}
+
+hbp_status hbp_peps_bp_condition(hbp_ctx *ctx, hbp_peps *p, double *residual) {
+  API_BEGIN
+  double2 *nE = (double2 *)ctx->alloc(p->E_bytes);
+  double *dres = (double *)ctx->alloc(sizeof(double));
+  sweep(ctx, p, nE, true, dres);
+  HBP_HIP(hipMemcpyAsync(residual, dres, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  ctx->sync();
+  ctx->release(dres, sizeof(double));
+  ctx->release(nE, p->E_bytes);
+  API_END
+}
+
+hbp_status hbp_peps_bp_run(hbp_ctx *ctx, hbp_peps *p, int max_iter, double tol, int *iters,
+                           double *residuals) {
+  API_BEGIN
+  HBP_REQUIRE(max_iter >= 1, HBP_ERR_INVALID_ARGUMENT, "bp_run: max_iter < 1");
+  // first sweep: E_1 = BP(E_0)
+  double2 *nE = (double2 *)ctx->alloc(p->E_bytes);
+  sweep(ctx, p, nE, false, nullptr);
+  ctx->release(p->E, p->E_bytes);
+  p->E = nE;
+  double *dres = (double *)ctx->alloc(sizeof(double));
+  int it = 0;
+  for (it = 1; it <= max_iter; ++it) {
+    // the condition's recomputed messages are the next sweep's messages
+    double2 *G = (double2 *)ctx->alloc(p->E_bytes);
+    sweep(ctx, p, G, true, dres);
+    double r = 0.0;
+    HBP_HIP(hipMemcpyAsync(&r, dres, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+    if (residuals) residuals[it - 1] = r;
+    if (r < tol || it == max_iter) {
+      ctx->release(G, p->E_bytes);
+      break;
+    }
+    ctx->release(p->E, p->E_bytes);
+    p->E = G;
+  }
+  if (iters) *iters = std::min(it, max_iter);
+  ctx->release(dres, sizeof(double));
+  API_END
+}
+
+hbp_status hbp_peps_truncate(hbp_ctx *ctx, hbp_peps *p, int chi_max, double sv_min,
+                             int32_t *bond_dims, double *trunc_errs) {
+  API_BEGIN
+  HBP_REQUIRE(chi_max >= 1, HBP_ERR_INVALID_ARGUMENT, "truncate: chi_max < 1");
+  truncate(ctx, p, chi_max, sv_min, bond_dims, trunc_errs);
+  API_END
+}
+
+hbp_status hbp_peps_measure_sites(hbp_ctx *ctx, hbp_peps *p, int n, const int32_t *sites,
+                                  const double *ops, double *out) {
+  API_BEGIN
+  if (n <= 0) return HBP_OK;
+  std::vector<int> addr(n);
+  for (int t = 0; t < n; ++t) addr[t] = site_addr(p, sites[t]);
+  if (p->precision == HBP_PREC_FP64)
+    measure_sites_impl<double>(ctx, p, n, addr.data(), ops, out);
+  else
+    measure_sites_impl<float>(ctx, p, n, addr.data(), ops, out);
+  API_END
+}
+
+hbp_status hbp_peps_measure_edges(hbp_ctx *ctx, hbp_peps *p, int n, const int32_t *pairs,
+                                  const double *ops, double *out) {
+  API_BEGIN
+  if (n <= 0) return HBP_OK;
+  std::vector<int> a(n), b(n);
+  for (int t = 0; t < n; ++t) {
+    a[t] = site_addr(p, pairs[2 * t]);
+    b[t] = site_addr(p, pairs[2 * t + 1]);
+  }
+  if (p->precision == HBP_PREC_FP64)
+    measure_edges_impl<double>(ctx, p, n, a.data(), b.data(), ops, out);
+  else
+    measure_edges_impl<float>(ctx, p, n, a.data(), b.data(), ops, out);
+  API_END
+}
+
+}  // extern "C"
