@@ -1,0 +1,234 @@
+"""Benchmark of the BP-PEPS hot path on one MI355X.
+
+Workload: kicked-Ising Floquet evolution on a 16x16 open square PEPS, chi = 32,
+complex128, J = pi/4 (RZZ angle), h drawn from a seed in [0.1, 1.0], BP to
+residual 1e-10, 256 <Z> + 480 <ZZ> measured after every step (the driver of
+apps/kicked_ising_lattice/main.cc measures every step).  The bond dimension
+ramps 1 -> 32 over the first five steps; those and `--warmup` more steps are
+untimed.  A timed step is one steady-state Floquet step: AbsorbTPO, the BP
+loop to tolerance at bond dimension D = 64, Truncation back to 32, and the 736
+expectation values.  value = expectation values / s over the timed steps.
+
+Prints one JSON line.  Runs on one GPU: --gpus above 1 exits with a message.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "hbm": 8.0e12}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--L", type=int, default=16)
+    ap.add_argument("--chi", type=int, default=32)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--max-bp-iterations", type=int, default=200)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--ramp", type=int, default=5, help="steps before the bond dimension saturates")
+    ap.add_argument("--cpu-baseline", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    return ap.parse_args()
+
+
+def cpu_baseline(site_tensor, msgs, steady, args):
+    """Time the oracle (numpy + OpenBLAS on the host cores) on a bounded sample
+    of the same workload: the outgoing messages of one interior site of the
+    steady-state PEPS, with the same message/normalisation sequence as
+    bp.h:128-175.  Scaled to expectation values/s with the site census of the
+    lattice and the GPU run's BP iteration count, with the reference's two
+    sweeps per BP iteration (sweep + condition, main.cc:121-159)."""
+    from oracle import tci as ot
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    V = site_tensor
+    z = V.ndim - 1
+    t0 = time.perf_counter()
+    n_msgs = 0
+    while True:
+        for k in range(z):
+            T = V.copy()
+            for j in range(z):
+                if j != k:
+                    la = list(range(V.ndim))
+                    la[j] = -1
+                    T = ot.contract(T, la, msgs[j], [-1, j], list(range(V.ndim)))
+                T, _ = ot.normalize(T)
+            idx_t = list(range(-V.ndim, 0))
+            idx_a = list(range(-V.ndim, 0))
+            idx_t[k], idx_a[k] = 0, 1
+            M = ot.contract(T, idx_t, ot.cplx_conj(V), idx_a, [0, 1])
+            ot.normalize(M)
+            n_msgs += 1
+            if time.perf_counter() - t0 > args.cpu_seconds:
+                break
+        if time.perf_counter() - t0 > args.cpu_seconds:
+            break
+    dt = time.perf_counter() - t0
+    per_msg = dt / n_msgs
+    L = args.L
+    # messages per sweep weighted by their cost relative to an interior (z=4) message
+    D = V.shape[0]
+    cost4 = D ** 4 * 2 * D * 4
+    cost3 = D ** 3 * 2 * D * 3
+    cost2 = D ** 2 * 2 * D * 2
+    n4, n3, n2 = (L - 2) ** 2, 4 * (L - 2), 4
+    sweep_equiv = (n4 * 4 * cost4 + n3 * 3 * cost3 + n2 * 2 * cost2) / cost4
+    iters = max(1.0, steady["bp_iterations"])
+    step_s = 2 * iters * sweep_equiv * per_msg
+    n_vals = L * L + 2 * L * (L - 1)
+    return {
+        "value": n_vals / step_s,
+        "unit": "expectation values/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"oracle (numpy/OpenBLAS) computing {n_msgs} outgoing BP messages of one interior "
+                   f"site of the steady-state {L}x{L} PEPS (D={D}) in {dt:.1f}s; scaled by the site "
+                   f"census to one sweep ({sweep_equiv:.0f} interior-message equivalents) and by "
+                   f"2 sweeps x {iters:.0f} BP iterations per step (sweep + condition); truncation and "
+                   f"measurement not included"),
+        "seconds_per_interior_message": per_msg,
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; the multi-GPU path is a design "
+                                   "(real-space decomposition of the lattice), not measured"}))
+        sys.exit(2)
+    from tnbp_amd import circuits, driver
+    from tnbp_amd.peps import (HBP_PREC_FP64, HBP_PREC_MIXED, AbsorbTPO, InitTensorProductState,
+                               Measure, MeasureEdges, Truncation, bp_loop)
+    from tnbp_amd.tci import Context
+    from tnbp_amd._lib import check
+
+    L, chi = args.L, args.chi
+    rng = np.random.default_rng(args.seed)
+    h = float(rng.uniform(0.1, 1.0))
+    J = math.pi / 4
+    edges = circuits.square_lattice(L, L)
+    sites = sorted({s for e in edges for s in e})
+    # RZZ(J) = exp(-i J/2 ZZ) and RX(h) = exp(-i h/2 X): dt = 1 (in units of pi),
+    # Jz = J / (2 pi), hx = h / (2 pi) in the reference's CircuitTPO parametrisation
+    tpo = circuits.kicked_ising_tpo(edges, J / (2 * math.pi), 0.0, h / (2 * math.pi), math.pi)
+    prec = HBP_PREC_FP64 if args.precision == "fp64" else HBP_PREC_MIXED
+    zop = circuits.op_tensor(circuits.PAULI["Z"])
+    zz = circuits.op_tensor(np.kron(circuits.PAULI["Z"], circuits.PAULI["Z"]))
+
+    ctx = Context(0)
+    peps = InitTensorProductState(ctx, edges, [2] * len(sites), prec)
+
+    def step(record):
+        t0 = time.perf_counter()
+        AbsorbTPO(ctx, tpo, peps)
+        it, res = bp_loop(ctx, peps, args.max_bp_iterations, args.tol)
+        dims, errs = Truncation(ctx, peps, chi, 1e-8)
+        z = Measure(ctx, peps, sites, [zop] * len(sites))
+        zzv = MeasureEdges(ctx, peps, edges, [zz] * len(edges))
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        rec = {"seconds": dt, "bp_iterations": it, "residual": res[-1] if res else None,
+               "max_bond": max(dims), "max_trunc_err": max(errs), "n_values": len(z) + len(zzv)}
+        if record is not None:
+            record.append(rec)
+        print(json.dumps({"progress": rec}), file=sys.stderr, flush=True)
+        return rec
+
+    for _ in range(args.ramp + args.warmup):
+        step(None)
+    ctx.synchronize()
+    check(ctx.lib.hbp_ctx_profile(ctx.h, 1))
+    timed = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(timed)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    import ctypes as C
+    stats = {}
+    for kind, name in ((0, "mode_product"), (1, "message")):
+        n, ms, fl, by = C.c_int64(), C.c_double(), C.c_double(), C.c_double()
+        check(ctx.lib.hbp_ctx_profile_read(ctx.h, kind, C.byref(n), C.byref(ms), C.byref(fl), C.byref(by)))
+        stats[name] = {"launches": n.value, "ms": ms.value, "flops": fl.value, "bytes": by.value}
+    check(ctx.lib.hbp_ctx_profile(ctx.h, 0))
+
+    n_vals = sum(r["n_values"] for r in timed)
+    value = n_vals / wall
+    lat = sorted(r["seconds"] for r in timed)
+    mp = stats["mode_product"]
+    peak = PEAK["fp64"] if args.precision == "fp64" else PEAK["fp32"]
+    achieved = mp["flops"] / (mp["ms"] * 1e-3) if mp["ms"] > 0 else 0.0
+    roofline = {
+        "bound": "mfma",
+        "kernel": "k_mode_product (T x_k E, FP64 MFMA 16x16x4)" if args.precision == "fp64"
+                  else "k_mode_product (T x_k E, FP32 MFMA 16x16x4)",
+        "achieved": achieved / 1e12,
+        "peak": peak / 1e12,
+        "unit": "TFLOP/s",
+        "frac": achieved / peak,
+        "traffic": None,
+        "launches": mp["launches"],
+        "avg_launch_ms": mp["ms"] / max(1, mp["launches"]),
+        "algorithmic_flops_per_launch": mp["flops"] / max(1, mp["launches"]),
+        "algorithmic_bytes_per_launch": mp["bytes"] / max(1, mp["launches"]),
+        "achieved_GBps_algorithmic": mp["bytes"] / (mp["ms"] * 1e-3) / 1e9 if mp["ms"] > 0 else 0.0,
+        "message_kernel_TFLOPs": (stats["message"]["flops"] / (stats["message"]["ms"] * 1e-3) / 1e12
+                                  if stats["message"]["ms"] > 0 else 0.0),
+        "kernel_time_fraction": (mp["ms"] + stats["message"]["ms"]) / (wall * 1e3),
+    }
+    cpu = None
+    if args.cpu_baseline:
+        # a steady-state interior site and its incoming messages (bp.h:147-151)
+        s = (L // 2) * L + L // 2
+        V = peps.get_site(s)
+        inc = [m for m, e in enumerate(edges) if s in e]
+        msgs = [peps.get_message(m + len(edges) if edges[m][0] == s else m) for m in inc]
+        # absorb once more so that D = 2 chi as during BP
+        AbsorbTPO(ctx, tpo, peps)
+        V = peps.get_site(s)
+        msgs = [peps.get_message(m + len(edges) if edges[m][0] == s else m) for m in inc]
+        cpu = cpu_baseline(V, msgs, timed[-1], args)
+    peps.close()
+    ctx.close()
+    out = {
+        "metric": "expectation values/s + p99 latency, 16x16 PEPS chi=32 mixed vs FP64 path",
+        "value": value,
+        "unit": "expectation values/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32-mfma/f64",
+        "data": "synthetic (seeded kicked-Ising circuit, no external data)",
+        "config": {"workload": f"kicked-Ising {L}x{L} PEPS chi={chi}, complex128, J=pi/4, "
+                               f"h={h:.6f} (seed {args.seed}), BP tol {args.tol:g}, steady-state "
+                               f"Floquet step with 256 <Z> + 480 <ZZ>",
+                   "precision": args.precision, "L": L, "chi": chi},
+        "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                      "max": lat[-1]},
+        "bp_iterations": [r["bp_iterations"] for r in timed],
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -72,6 +72,16 @@ hbp_status hbp_ctx_synchronize(hbp_ctx *ctx);
 hbp_status hbp_ctx_stream(hbp_ctx *ctx, void **stream);
 /* Bytes currently held by the context's device pool. */
 hbp_status hbp_ctx_memory(hbp_ctx *ctx, size_t *in_use, size_t *peak);
+/* Kernel timing (no reference counterpart; the reference times nothing but
+ * one SVD, apps/tci_test/main.cc:39-48).  While enabled, the hot kernels are
+ * bracketed by HIP events on the context stream.  profile_read synchronises,
+ * returns launch count, summed duration, and the algorithmic flops and bytes
+ * of all recorded launches of one kind, and forgets them.  Kinds:
+ * HBP_PROF_MODE_PRODUCT, HBP_PROF_MESSAGE (partials + fixed-order reduce). */
+enum { HBP_PROF_MODE_PRODUCT = 0, HBP_PROF_MESSAGE = 1 };
+hbp_status hbp_ctx_profile(hbp_ctx *ctx, int enable);
+hbp_status hbp_ctx_profile_read(hbp_ctx *ctx, int kind, int64_t *count, double *total_ms,
+                                double *flops, double *bytes);
 
 /* ==================================================== tensor layer ==== */
 /* Zero tensor of the given shape (tci::zeros / gqten tensor(shape)). */

@@ -136,11 +136,13 @@ def test_site_and_message_round_trip(ctx):
 
 
 def test_residual_matches_condition(ctx):
-    """bp_run's fused residual equals BeliefPropagationCondition's."""
+    """BeliefPropagationCondition and the fused sweep from random Hermitian PSD
+    messages (far from the fixed point), against the oracle."""
     from tnbp_amd import circuits as pc
     from tnbp_amd.peps import (AbsorbTPO, BeliefPropagation, BeliefPropagationCondition,
                                InitTensorProductState)
     announce("residual vs condition")
+    rng = np.random.default_rng(17)
     edges = pc.square_lattice(3, 3)
     tpo = pc.kicked_ising_tpo(edges, 0.125, 0.0, 0.05, math.pi)
     p = InitTensorProductState(ctx, edges)
@@ -150,7 +152,16 @@ def test_residual_matches_condition(ctx):
     otpo = oc.circuit_tpo(edges, 0.125, 0.0, 0.05, math.pi, "trunc")
     ot.absorb_tpo(st, otpo)
     ot.absorb_tpo(st, otpo)
+    for m in range(2 * len(edges)):
+        d = st.E[m].shape[0]
+        a = rng.standard_normal((d, d)) + 1j * rng.standard_normal((d, d))
+        e = a @ a.conj().T
+        st.E[m] = e
+        p.set_message(m, e)
     layers = ot.greedy_edge_layering(edges)
+    r0_gpu = BeliefPropagationCondition(ctx, p)
+    r0_ref = ot.belief_propagation_condition(st)
+    assert r0_ref > 1e-3 and abs(r0_gpu - r0_ref) <= 1e-10 * r0_ref
     for _ in range(3):
         BeliefPropagation(ctx, p)
         F = [None] * (2 * len(edges))
@@ -159,5 +170,7 @@ def test_residual_matches_condition(ctx):
         st.E = F
         r_gpu = BeliefPropagationCondition(ctx, p)
         r_ref = ot.belief_propagation_condition(st)
-        assert abs(r_gpu - r_ref) <= 1e-8 * r_ref
+        assert abs(r_gpu - r_ref) <= 1e-9 * r_ref + 1e-15
+        for m in (0, 5, len(edges) + 3):
+            np.testing.assert_allclose(p.get_message(m), st.E[m], atol=1e-12)
     p.close()

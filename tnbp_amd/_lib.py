@@ -35,6 +35,8 @@ SIGNATURES = {
     "hbp_ctx_synchronize": [_vp],
     "hbp_ctx_stream": [_vp, C.POINTER(_vp)],
     "hbp_ctx_memory": [_vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)],
+    "hbp_ctx_profile": [_vp, C.c_int],
+    "hbp_ctx_profile_read": [_vp, C.c_int, C.POINTER(C.c_int64), _dp, _dp, _dp],
     "hbp_tensor_create": [_vp, C.c_int, C.c_int, _i32p, C.POINTER(_vp)],
     "hbp_tensor_destroy": [_vp, _vp],
     "hbp_tensor_info": [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int), _i32p, C.POINTER(C.c_size_t)],

@@ -56,6 +56,20 @@ struct Ctx {
   void release(void *p, size_t bytes);
   void *get_scratch(size_t bytes);
   void sync();
+
+  // kernel timing with HIP events on this context's stream, per kernel class
+  struct ProfRec {
+    hipEvent_t a, b;
+    int kind;
+    double flops, bytes;
+  };
+  bool prof = false;
+  std::vector<ProfRec> prof_recs;
+  std::vector<hipEvent_t> prof_free;
+  hipEvent_t prof_event();
+  // returns the index of the open record, or -1 when profiling is off
+  int prof_begin(int kind, double flops, double bytes);
+  void prof_end(int idx);
   // host -> device copy of a small host buffer that may go out of scope on
   // return: enqueued on the context stream and waited for
   void upload(void *dst, const void *src, size_t bytes);

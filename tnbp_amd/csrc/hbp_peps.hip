@@ -445,8 +445,12 @@ static void mode_prod(Ctx *ctx, const void *T, const double2 *E, void *Out, cons
   int64_t L = 1, R = 1;
   for (int d = 0; d < k; ++d) L *= sh[d];
   for (int d = k + 1; d < order; ++d) R *= sh[d];
+  const double es = sizeof(S) * 2.0;
+  const int pr = ctx->prof_begin(HBP_PROF_MODE_PRODUCT, 8.0 * L * R * sh[k] * Dout,
+                                 es * L * R * (sh[k] + Dout) + 16.0 * sh[k] * Dout);
   launch_mode_product<S>(ctx->stream, T, E, Out, L, sh[k], Dout, R, e_trans, 1.0, nullptr,
                          nullptr);
+  ctx->prof_end(pr);
 }
 
 // raw M[i, j] = sum T[.. i ..] conj V[.. j ..] over all axes but k (both same shape)
@@ -459,8 +463,11 @@ static void message_raw(Ctx *ctx, const void *T, const void *V, const int32_t *s
   const int D = sh[k];
   const int64_t nch = message_num_chunks(L * R);
   double2 *parts = (double2 *)ctx->get_scratch((size_t)nch * D * D * 16 + 64);
+  const int pr = ctx->prof_begin(HBP_PROF_MESSAGE, 8.0 * L * R * D * D,
+                                 2.0 * sizeof(S) * 2.0 * L * R * D);
   const int n = message_partials<S>(ctx->stream, T, V, parts, L, D, D, R, L * D, L * D, nch, 1);
   launch_reduce_partials(ctx->stream, parts, n, D * D, M);
+  ctx->prof_end(pr);
 }
 
 // T = V_s with every incoming message absorbed except on bond `skip` (-1: none),

@@ -83,6 +83,30 @@ void *Ctx::get_scratch(size_t bytes) {
 
 void Ctx::sync() { HBP_HIP(hipStreamSynchronize(stream)); }
 
+hipEvent_t Ctx::prof_event() {
+  if (!prof_free.empty()) {
+    hipEvent_t e = prof_free.back();
+    prof_free.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  HBP_HIP(hipEventCreate(&e));
+  return e;
+}
+
+int Ctx::prof_begin(int kind, double flops, double bytes) {
+  if (!prof) return -1;
+  ProfRec r{prof_event(), prof_event(), kind, flops, bytes};
+  HBP_HIP(hipEventRecord(r.a, stream));
+  prof_recs.push_back(r);
+  return (int)prof_recs.size() - 1;
+}
+
+void Ctx::prof_end(int idx) {
+  if (idx < 0) return;
+  HBP_HIP(hipEventRecord(prof_recs[idx].b, stream));
+}
+
 void Ctx::upload(void *dst, const void *src, size_t bytes) {
   HBP_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
   HBP_HIP(hipStreamSynchronize(stream));
@@ -371,6 +395,36 @@ void svd_impl(Ctx *ctx, const Tensor *a, int nrows, Tensor *u, Tensor *s, Tensor
   int64_t m = 1, n = 1;
   for (int d = 0; d < nrows; ++d) m *= a->shape[d];
   for (int d = nrows; d < a->order; ++d) n *= a->shape[d];
+  if (m < n) {
+    // one-sided Jacobi orthogonalises columns: run it on A^H (n x m) and swap
+    // the factors, A = (A^H)^H = Vd'^H S U'^H
+    Tensor ah;
+    const int32_t sh2[2] = {(int32_t)m, (int32_t)n}, pr[2] = {1, 0};
+    const int32_t shh[2] = {(int32_t)n, (int32_t)m};
+    tensor_resize(ctx, &ah, HBP_C128, 2, shh);
+    launch_permute(ctx->stream, a->data, ah.data, 16, 2, sh2, pr);
+    launch_conj(ctx->stream, ah.data, ah.size, HBP_C128);
+    Tensor u2, vd2;
+    svd_impl(ctx, &ah, 1, &u2, s, &vd2, keep_max, s_min, trunc_err);
+    const int keep = s->shape[0];
+    // u = vd2^H (m x keep), vd = u2^H (keep x n)
+    int32_t ush[HBP_MAX_ORDER], vsh[HBP_MAX_ORDER];
+    for (int d = 0; d < nrows; ++d) ush[d] = a->shape[d];
+    ush[nrows] = keep;
+    tensor_resize(ctx, u, HBP_C128, nrows + 1, ush);
+    const int32_t s_vd2[2] = {keep, (int32_t)m}, s_u2[2] = {(int32_t)n, keep};
+    launch_permute(ctx->stream, vd2.data, u->data, 16, 2, s_vd2, pr);
+    launch_conj(ctx->stream, u->data, u->size, HBP_C128);
+    vsh[0] = keep;
+    for (int d = nrows; d < a->order; ++d) vsh[d - nrows + 1] = a->shape[d];
+    tensor_resize(ctx, vd, HBP_C128, a->order - nrows + 1, vsh);
+    launch_permute(ctx->stream, u2.data, vd->data, 16, 2, s_u2, pr);
+    launch_conj(ctx->stream, vd->data, vd->size, HBP_C128);
+    ctx->release(ah.data, ah.bytes);
+    ctx->release(u2.data, u2.bytes);
+    ctx->release(vd2.data, vd2.bytes);
+    return;
+  }
   HBP_REQUIRE(n <= 512 && m <= (1 << 20), HBP_ERR_UNSUPPORTED, "svd: matrix too large");
   const int k = (int)std::min(m, n);
   double2 *U = (double2 *)ctx->alloc((size_t)m * k * 16);
@@ -481,6 +535,41 @@ hbp_status hbp_ctx_memory(hbp_ctx *ctx, size_t *in_use, size_t *peak) {
   HBP_API_BEGIN
   if (in_use) *in_use = ctx->in_use;
   if (peak) *peak = ctx->peak;
+  HBP_API_END
+}
+
+hbp_status hbp_ctx_profile(hbp_ctx *ctx, int enable) {
+  HBP_API_BEGIN
+  ctx->prof = enable != 0;
+  HBP_API_END
+}
+
+hbp_status hbp_ctx_profile_read(hbp_ctx *ctx, int kind, int64_t *count, double *total_ms,
+                                double *flops, double *bytes) {
+  HBP_API_BEGIN
+  ctx->sync();
+  int64_t n = 0;
+  double ms = 0.0, fl = 0.0, by = 0.0;
+  std::vector<Ctx::ProfRec> keep;
+  for (auto &r : ctx->prof_recs) {
+    if (r.kind != kind) {
+      keep.push_back(r);
+      continue;
+    }
+    float t = 0.0f;
+    HBP_HIP(hipEventElapsedTime(&t, r.a, r.b));
+    ++n;
+    ms += t;
+    fl += r.flops;
+    by += r.bytes;
+    ctx->prof_free.push_back(r.a);
+    ctx->prof_free.push_back(r.b);
+  }
+  ctx->prof_recs.swap(keep);
+  if (count) *count = n;
+  if (total_ms) *total_ms = ms;
+  if (flops) *flops = fl;
+  if (bytes) *bytes = by;
   HBP_API_END
 }
 

@@ -42,9 +42,10 @@ class Peps:
         self.precision = precision
 
     def close(self):
-        if self.h:
+        # a closed context has already released the device pool holding the state
+        if self.h and self.ctx.h:
             check(self.ctx.lib.hbp_peps_destroy(self.ctx.h, self.h))
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
