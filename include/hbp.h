@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define HBP_MAX_ORDER 8
+#define HBP_MAX_ORDER 16
 
 typedef int hbp_status;
 enum {
@@ -151,6 +151,18 @@ hbp_status hbp_trunc_svd(hbp_ctx *ctx, const hbp_tensor *a, int nrows, int chi_m
 hbp_status hbp_peps_create(hbp_ctx *ctx, int n_edges, const int32_t *edges,
                            const int32_t *phys_dims, int precision, hbp_peps **out);
 hbp_status hbp_peps_destroy(hbp_ctx *ctx, hbp_peps *p);
+/* Execution strategy (no reference counterpart; both give the reference's
+ * results).  FUSED: each message is its mode products plus one message
+ * contraction, normalised once; the BP loop reuses the condition's messages
+ * as the next sweep.  PLAIN: the reference's op sequence one TCI call at a
+ * time - a copy of V per message, a norm and a scale pass after every bond,
+ * a conjugated copy of V, and a separate condition sweep per BP iteration
+ * (bp.h:128-175, :290-350, main.cc:121-159). */
+enum { HBP_MODE_FUSED = 0, HBP_MODE_PLAIN = 1 };
+hbp_status hbp_peps_set_mode(hbp_peps *p, int mode);
+/* Mixed precision only: the BP loop also stops once the residual is below
+ * this floor (default 1e-6), since FP32 messages cannot reach FP64 tolerances. */
+hbp_status hbp_peps_set_residual_floor(hbp_peps *p, double floor);
 hbp_status hbp_peps_num_sites(const hbp_peps *p, int *n_sites, int *n_edges);
 /* Shape of site tensor s (order = degree + 1) and of message m (2 x D). */
 hbp_status hbp_peps_site_shape(const hbp_peps *p, int site, int *order, int32_t *shape);

@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libhbp.so")
 
-HBP_MAX_ORDER = 8
+HBP_MAX_ORDER = 16
 HBP_C128, HBP_F64, HBP_C64, HBP_F32 = 0, 1, 2, 3
 HBP_PREC_FP64, HBP_PREC_MIXED = 0, 1
 
@@ -59,6 +59,8 @@ SIGNATURES = {
     "hbp_trunc_svd": [_vp, _vp, C.c_int, C.c_int, C.c_double, _vp, _vp, _vp, _dp],
     "hbp_peps_create": [_vp, C.c_int, _i32p, _i32p, C.c_int, C.POINTER(_vp)],
     "hbp_peps_destroy": [_vp, _vp],
+    "hbp_peps_set_mode": [_vp, C.c_int],
+    "hbp_peps_set_residual_floor": [_vp, C.c_double],
     "hbp_peps_num_sites": [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "hbp_peps_site_shape": [_vp, C.c_int, C.POINTER(C.c_int), _i32p],
     "hbp_peps_get_site": [_vp, _vp, C.c_int, _dp],

@@ -394,6 +394,12 @@ struct Peps {
   // work buffers
   void *buf[2] = {nullptr, nullptr};
   size_t buf_bytes = 0;
+  void *buf3 = nullptr;  // plain mode: the conjugated copy of V
+  size_t buf3_bytes = 0;
+  int mode = HBP_MODE_FUSED;
+  // mixed precision: BP stops once the residual is below this (FP32 messages
+  // cannot reach the FP64 tolerances), or below the caller's tolerance
+  double mixed_floor = 1.0e-6;
 
   int elem_bytes() const { return precision == HBP_PREC_FP64 ? 16 : 8; }
   int site_dtype() const { return precision == HBP_PREC_FP64 ? HBP_C128 : HBP_C64; }
@@ -503,6 +509,60 @@ static void sweep_raw(Ctx *ctx, Peps *p, double2 *raw, const std::vector<int64_t
   }
 }
 
+// The reference's op sequence for every message, one TCI call at a time
+// (bp.h:128-175): T = copy(V); for every bond k: (k != target) T = T x_k E_in,
+// then normalize(T) as a norm pass and a scale pass; C = conj(copy(V));
+// M = <T, C> over all bonds but the target; normalize(M).
+template <typename S>
+static void sweep_raw_plain(Ctx *ctx, Peps *p, double2 *raw, const std::vector<int64_t> &raw_off) {
+  size_t maxb = 16;
+  for (int s = 0; s < p->n_sites; ++s) maxb = std::max(maxb, p->site_size(s) * p->elem_bytes());
+  ensure_work(ctx, p, maxb);
+  if (p->buf3_bytes < maxb) {
+    if (p->buf3) ctx->release(p->buf3, p->buf3_bytes);
+    p->buf3 = ctx->alloc(maxb);
+    p->buf3_bytes = maxb;
+  }
+  double *dv = (double *)ctx->alloc(sizeof(double));
+  int32_t sh[HBP_MAX_ORDER];
+  for (int s = 0; s < p->n_sites; ++s) {
+    const int z = p->z(s);
+    p->shape(s, sh);
+    const size_t n = p->site_size(s), bytes = n * p->elem_bytes();
+    for (int k = 0; k < z; ++k) {
+      HBP_HIP(hipMemcpyAsync(p->buf[0], p->V[s], bytes, hipMemcpyDeviceToDevice, ctx->stream));
+      int cur = 0;
+      for (int j = 0; j < z; ++j) {
+        if (j != k) {
+          mode_prod<S>(ctx, p->buf[cur], p->E + p->E_off[p->in_msg(s, j)], p->buf[cur ^ 1], sh,
+                       z + 1, j, sh[j], 0);
+          cur ^= 1;
+        }
+        device_norm2(ctx, p->buf[cur], n, p->site_dtype(), dv);
+        launch_scale(ctx->stream, p->buf[cur], n, p->site_dtype(), 1.0, 0.0, dv);
+      }
+      HBP_HIP(hipMemcpyAsync(p->buf3, p->V[s], bytes, hipMemcpyDeviceToDevice, ctx->stream));
+      launch_conj(ctx->stream, p->buf3, n, p->site_dtype());
+      int64_t L = 1, R = 1;
+      for (int d = 0; d < k; ++d) L *= sh[d];
+      for (int d = k + 1; d <= z; ++d) R *= sh[d];
+      const int D = sh[k];
+      const int64_t nch = message_num_chunks(L * R);
+      double2 *parts = (double2 *)ctx->get_scratch((size_t)nch * D * D * 16 + 64);
+      const int pr = ctx->prof_begin(HBP_PROF_MESSAGE, 8.0 * L * R * D * D,
+                                     2.0 * sizeof(S) * 2.0 * L * R * D);
+      const int nc = message_partials<S>(ctx->stream, p->buf[cur], p->buf3, parts, L, D, D, R,
+                                         L * D, L * D, nch, 0);
+      double2 *M = raw + raw_off[p->out_msg(s, k)];
+      launch_reduce_partials(ctx->stream, parts, nc, D * D, M);
+      ctx->prof_end(pr);
+      device_norm2(ctx, M, (size_t)D * D, HBP_C128, dv);
+      launch_scale(ctx->stream, M, (size_t)D * D, HBP_C128, 1.0, 0.0, dv);
+    }
+  }
+  ctx->release(dv, sizeof(double));
+}
+
 // one BP sweep: newE = normalise(BP(E)); returns residual on device if dist != null
 static void sweep(Ctx *ctx, Peps *p, double2 *newE, bool with_residual, double *dev_res) {
   const int nm = 2 * p->n_edges;
@@ -512,10 +572,16 @@ static void sweep(Ctx *ctx, Peps *p, double2 *newE, bool with_residual, double *
   MsgJob *dj = (MsgJob *)ctx->alloc(nm * sizeof(MsgJob));
   ctx->upload(dj, jobs.data(), nm * sizeof(MsgJob));
   double2 *raw = (double2 *)ctx->alloc(std::max<size_t>(p->E_bytes, 16));
-  if (p->precision == HBP_PREC_FP64)
+  if (p->mode == HBP_MODE_PLAIN) {
+    if (p->precision == HBP_PREC_FP64)
+      sweep_raw_plain<double>(ctx, p, raw, p->E_off);
+    else
+      sweep_raw_plain<float>(ctx, p, raw, p->E_off);
+  } else if (p->precision == HBP_PREC_FP64) {
     sweep_raw<double>(ctx, p, raw, p->E_off);
-  else
+  } else {
     sweep_raw<float>(ctx, p, raw, p->E_off);
+  }
   double *dist = (double *)ctx->alloc(nm * sizeof(double));
   hipLaunchKernelGGL(k_finalize_messages, dim3(nm), dim3(256), 0, ctx->stream, raw, newE,
                      with_residual ? p->E : nullptr, dj, dist);
@@ -878,7 +944,7 @@ hbp_status hbp_peps_create(hbp_ctx *ctx, int n_edges, const int32_t *edges,
   }
   for (int s = 0; s < p->n_sites; ++s) {
     HBP_REQUIRE((int)p->bonds[s].size() + 1 <= HBP_MAX_ORDER, HBP_ERR_UNSUPPORTED,
-                "peps: site degree above 7");
+                "peps: site degree above 15");
     p->phys.push_back(phys_dims ? phys_dims[s] : 2);
     HBP_REQUIRE(p->phys[s] >= 1 && p->phys[s] <= 16, HBP_ERR_UNSUPPORTED, "peps: physical dim");
   }
@@ -908,6 +974,7 @@ hbp_status hbp_peps_destroy(hbp_ctx *ctx, hbp_peps *p) {
   ctx->release(p->E, p->E_bytes);
   for (int i = 0; i < 2; ++i)
     if (p->buf[i]) ctx->release(p->buf[i], p->buf_bytes);
+  if (p->buf3) ctx->release(p->buf3, p->buf3_bytes);
   ctx->sync();
   delete p;
   API_END
@@ -1065,6 +1132,21 @@ hbp_status hbp_peps_absorb_tpo(hbp_ctx *ctx, hbp_peps *p, const double *const *o
   API_END
 }
 
+hbp_status hbp_peps_set_mode(hbp_peps *p, int mode) {
+  API_BEGIN
+  HBP_REQUIRE(mode == HBP_MODE_FUSED || mode == HBP_MODE_PLAIN, HBP_ERR_INVALID_ARGUMENT,
+              "peps: unknown mode");
+  p->mode = mode;
+  API_END
+}
+
+hbp_status hbp_peps_set_residual_floor(hbp_peps *p, double floor) {
+  API_BEGIN
+  HBP_REQUIRE(floor >= 0.0, HBP_ERR_INVALID_ARGUMENT, "peps: negative residual floor");
+  p->mixed_floor = floor;
+  API_END
+}
+
 hbp_status hbp_peps_bp_sweep(hbp_ctx *ctx, hbp_peps *p) {
   API_BEGIN
   double2 *nE = (double2 *)ctx->alloc(p->E_bytes);
@@ -1090,6 +1172,30 @@ hbp_status hbp_peps_bp_run(hbp_ctx *ctx, hbp_peps *p, int max_iter, double tol, 
                            double *residuals) {
   API_BEGIN
   HBP_REQUIRE(max_iter >= 1, HBP_ERR_INVALID_ARGUMENT, "bp_run: max_iter < 1");
+  const double stop = p->precision == HBP_PREC_MIXED ? std::max(tol, p->mixed_floor) : tol;
+  if (p->mode == HBP_MODE_PLAIN) {
+    // the driver loop as written (main.cc:121-159): every iteration is a sweep
+    // (E <- F) followed by a separate condition sweep
+    double *dres = (double *)ctx->alloc(sizeof(double));
+    int it = 0;
+    for (it = 1; it <= max_iter; ++it) {
+      double2 *nE = (double2 *)ctx->alloc(p->E_bytes);
+      sweep(ctx, p, nE, false, nullptr);
+      ctx->release(p->E, p->E_bytes);
+      p->E = nE;
+      double2 *G = (double2 *)ctx->alloc(p->E_bytes);
+      sweep(ctx, p, G, true, dres);
+      ctx->release(G, p->E_bytes);
+      double r = 0.0;
+      HBP_HIP(hipMemcpyAsync(&r, dres, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+      ctx->sync();
+      if (residuals) residuals[it - 1] = r;
+      if (r < stop) break;
+    }
+    if (iters) *iters = std::min(it, max_iter);
+    ctx->release(dres, sizeof(double));
+    return HBP_OK;
+  }
   // first sweep: E_1 = BP(E_0)
   double2 *nE = (double2 *)ctx->alloc(p->E_bytes);
   sweep(ctx, p, nE, false, nullptr);
@@ -1105,7 +1211,7 @@ hbp_status hbp_peps_bp_run(hbp_ctx *ctx, hbp_peps *p, int max_iter, double tol, 
     HBP_HIP(hipMemcpyAsync(&r, dres, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     ctx->sync();
     if (residuals) residuals[it - 1] = r;
-    if (r < tol || it == max_iter) {
+    if (r < stop || it == max_iter) {
       ctx->release(G, p->E_bytes);
       break;
     }
