@@ -4,7 +4,7 @@ Workload: kicked-Ising Floquet evolution on a 16x16 open square PEPS, chi = 32,
 complex128, J = pi/4 (RZZ angle), h drawn from a seed in [0.1, 1.0], BP to
 residual 1e-10, 256 <Z> + 480 <ZZ> measured after every step (the driver of
 apps/kicked_ising_lattice/main.cc measures every step).  The bond dimension
-ramps 1 -> 32 over the first five steps; those and `--warmup` more steps are
+ramps 1 -> 32 over the first six steps; those and `--warmup` more steps are
 untimed.  A timed step is one steady-state Floquet step: AbsorbTPO, the BP
 loop to tolerance at bond dimension D = 64, Truncation back to 32, and the 736
 expectation values.  value = expectation values / s over the timed steps.
@@ -36,10 +36,13 @@ def parse():
     ap.add_argument("--L", type=int, default=16)
     ap.add_argument("--chi", type=int, default=32)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--mode", choices=["fused", "plain"], default="fused",
+                    help="plain: the reference's op sequence one TCI call at a time")
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--max-bp-iterations", type=int, default=200)
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--ramp", type=int, default=5, help="steps before the bond dimension saturates")
+    ap.add_argument("--max-ramp", type=int, default=12,
+                    help="untimed steps allowed for the bond dimension to reach chi")
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     return ap.parse_args()
@@ -131,6 +134,7 @@ def main():
 
     ctx = Context(0)
     peps = InitTensorProductState(ctx, edges, [2] * len(sites), prec)
+    check(ctx.lib.hbp_peps_set_mode(peps.h, 0 if args.mode == "fused" else 1))
 
     def step(record):
         t0 = time.perf_counter()
@@ -148,7 +152,12 @@ def main():
         print(json.dumps({"progress": rec}), file=sys.stderr, flush=True)
         return rec
 
-    for _ in range(args.ramp + args.warmup):
+    # ramp: untimed steps until the bond dimension has saturated at chi, so that
+    # every timed step absorbs to D = 2 chi and truncates back
+    for _ in range(args.max_ramp):
+        if step(None)["max_bond"] >= chi:
+            break
+    for _ in range(args.warmup):
         step(None)
     ctx.synchronize()
     check(ctx.lib.hbp_ctx_profile(ctx.h, 1))
@@ -220,7 +229,7 @@ def main():
         "config": {"workload": f"kicked-Ising {L}x{L} PEPS chi={chi}, complex128, J=pi/4, "
                                f"h={h:.6f} (seed {args.seed}), BP tol {args.tol:g}, steady-state "
                                f"Floquet step with 256 <Z> + 480 <ZZ>",
-                   "precision": args.precision, "L": L, "chi": chi},
+                   "precision": args.precision, "mode": args.mode, "L": L, "chi": chi},
         "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                       "max": lat[-1]},
         "bp_iterations": [r["bp_iterations"] for r in timed],

@@ -101,10 +101,18 @@ void launch_mode_product(hipStream_t st, const void *T, const double2 *E, void *
 // T[q, i] V[q, j] (conjV = 0) over row chunks; T is a (L, D1, R) view and V a
 // (L, D2, R) view of scalar S, with r strides ldT_R, ldV_R (normally L*D).
 // Returns the number of chunks written (each D1*D2 complex<double>).
+// wL (L entries) and wR (R entries), when given, weight row (l, r) of T by
+// wL[l] wR[r]: a mode product with a diagonal matrix folded into the load.
 template <typename S>
 int message_partials(hipStream_t st, const void *T, const void *V, double2 *partials,
                      int64_t L, int D1, int D2, int64_t R, int64_t ldT_R, int64_t ldV_R,
-                     int64_t max_chunks, int conjV);
+                     int64_t max_chunks, int conjV, const double2 *wL = nullptr,
+                     const double2 *wR = nullptr, int Da = 0, int64_t Sb = 0);
+// (Da, Sb): column c of a row at L (c % Da) + Sb (c / Da); Da = 0 is the plain view.
+// w[i] = prod_b diag(E_b)[c_b] over the column-major split of i into dims[0..nb);
+// n may exceed prod(dims) by a trailing unit-weight factor.
+void launch_build_weights(hipStream_t st, const double2 *E, double2 *w, int64_t n, int nb,
+                          const int64_t *dims, const int64_t *diag_off);
 int64_t message_num_chunks(int64_t Q);
 // M = sum over chunks (fixed order), complex<double>.
 void launch_reduce_partials(hipStream_t st, const double2 *partials, int nchunks, int n,

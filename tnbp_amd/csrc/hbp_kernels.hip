@@ -208,11 +208,18 @@ template <typename S, int NCB>
 __global__ __launch_bounds__(256) void k_message_partials(
     const typename Cplx<S>::type *__restrict__ T, const typename Cplx<S>::type *__restrict__ V,
     double2 *__restrict__ partials, int64_t L, int D1, int D2, int64_t Q, int64_t chunk,
-    int64_t ldT_R, int64_t ldV_R, int conjV) {
+    int64_t ldT_R, int64_t ldV_R, int conjV, const double2 *__restrict__ wL,
+    const double2 *__restrict__ wR, int Da, int64_t Sb) {
   using C = typename Cplx<S>::type;
   using M = Mfma<S>;
   using v4 = typename M::v4;
   constexpr int NC = 16 * NCB;
+  // column c of a row sits at L (c % Da) + Sb (c / Da) from the row's base:
+  // Da = D1 gives the plain (L, D, R) view; Da = D, Sb = slab stacks the
+  // physical slices of a tensor as further columns (edge environments)
+  auto coff = [&](int c) -> int64_t { return L * (int64_t)(c % Da) + Sb * (int64_t)(c / Da); };
+  constexpr int NT = MSG_QS * MSG_IB / 256;  // T elements per thread per stage
+  constexpr int NV = MSG_QS * NC / 256;      // V elements per thread per stage
   __shared__ S tre[MSG_QS][MSG_IB + 1], tim[MSG_QS][MSG_IB + 1];
   __shared__ S vre[MSG_QS][NC + 1], vim[MSG_QS][NC + 1];
 
@@ -222,6 +229,7 @@ __global__ __launch_bounds__(256) void k_message_partials(
   const int64_t qbeg = (int64_t)blockIdx.x * chunk;
   const int64_t qend = min(Q, qbeg + chunk);
   const bool kcontig = (L == 1);
+  const int tid = threadIdx.x;
 
   v4 acc_re[NCB], acc_im[NCB];
 #pragma unroll
@@ -230,49 +238,113 @@ __global__ __launch_bounds__(256) void k_message_partials(
     acc_im[cb] = v4{0, 0, 0, 0};
   }
 
-  for (int64_t q0 = qbeg; q0 < qend; q0 += MSG_QS) {
-    __syncthreads();
+  // Each thread owns NT elements of the T tile and NV of the V tile.  For
+  // L == 1 a row holds its columns contiguously and threads run along columns;
+  // otherwise threads run along rows (contiguous l).  Out-of-range elements
+  // load from a clamped address and are zeroed.
+  C tr[NT], vr[NV];
+  auto weight = [&](int64_t l, int64_t r) -> C {
+    double2 w = make_double2(1.0, 0.0);
+    if (wL) w = wL[l];
+    if (wR) {
+      const double2 b = wR[r];
+      w = make_double2(w.x * b.x - w.y * b.y, w.x * b.y + w.y * b.x);
+    }
+    C c;
+    c.x = (S)w.x;
+    c.y = (S)w.y;
+    return c;
+  };
+  auto fetch = [&](int64_t q0) {
     if (kcontig) {
-      // row q holds its columns contiguously: threads run along the columns
-      for (int idx = threadIdx.x; idx < MSG_QS * MSG_IB; idx += blockDim.x) {
+#pragma unroll
+      for (int it = 0; it < NT; ++it) {
+        const int idx = tid + 256 * it;
         const int qq = idx / MSG_IB, c = idx - qq * MSG_IB;
         const int64_t q = q0 + qq;
         const int i = i0 + c;
-        C t = C{0, 0};
-        if (q < qend && i < D1) t = T[(int64_t)i + ldT_R * q];
-        tre[qq][c] = t.x;
-        tim[qq][c] = t.y;
+        const bool ok = q < qend && i < D1;
+        C t = T[coff(ok ? i : 0) + ldT_R * (ok ? q : qbeg)];
+        if (ok && (wL || wR)) {
+          const C w = weight(0, q);
+          t = C{t.x * w.x - t.y * w.y, t.x * w.y + t.y * w.x};
+        }
+        tr[it] = ok ? t : C{0, 0};
       }
-      for (int idx = threadIdx.x; idx < MSG_QS * NC; idx += blockDim.x) {
+#pragma unroll
+      for (int it = 0; it < NV; ++it) {
+        const int idx = tid + 256 * it;
         const int qq = idx / NC, c = idx - qq * NC;
         const int64_t q = q0 + qq;
-        C v = C{0, 0};
-        if (q < qend && c < D2) v = V[(int64_t)c + ldV_R * q];
-        vre[qq][c] = v.x;
-        vim[qq][c] = v.y;
+        const bool ok = q < qend && c < D2;
+        const C v = V[coff(ok ? c : 0) + ldV_R * (ok ? q : qbeg)];
+        vr[it] = ok ? v : C{0, 0};
       }
     } else {
-      // threads run along the rows (contiguous l); one row base per thread
-      const int qq = threadIdx.x % MSG_QS;
+      const int qq = tid % MSG_QS;
       const int64_t q = q0 + qq;
-      const bool ok = q < qend;
-      const int64_t l = ok ? q % L : 0, r = ok ? q / L : 0;
+      const bool okq = q < qend;
+      const int64_t qc = okq ? q : qbeg;
+      const int64_t l = qc % L, r = qc / L;
       const int64_t bT = l + ldT_R * r, bV = l + ldV_R * r;
-      for (int c = threadIdx.x / MSG_QS; c < MSG_IB; c += blockDim.x / MSG_QS) {
-        const int i = i0 + c;
-        C t = C{0, 0};
-        if (ok && i < D1) t = T[bT + L * (int64_t)i];
-        tre[qq][c] = t.x;
-        tim[qq][c] = t.y;
+      C w = C{1, 0};
+      if (wL || wR) w = weight(l, r);
+#pragma unroll
+      for (int it = 0; it < NT; ++it) {
+        const int i = i0 + tid / MSG_QS + (256 / MSG_QS) * it;
+        const bool ok = okq && i < D1;
+        C t = T[bT + coff(ok ? i : 0)];
+        t = C{t.x * w.x - t.y * w.y, t.x * w.y + t.y * w.x};
+        tr[it] = ok ? t : C{0, 0};
       }
-      for (int c = threadIdx.x / MSG_QS; c < NC; c += blockDim.x / MSG_QS) {
-        C v = C{0, 0};
-        if (ok && c < D2) v = V[bV + L * (int64_t)c];
-        vre[qq][c] = v.x;
-        vim[qq][c] = v.y;
+#pragma unroll
+      for (int it = 0; it < NV; ++it) {
+        const int c = tid / MSG_QS + (256 / MSG_QS) * it;
+        const bool ok = okq && c < D2;
+        const C v = V[bV + coff(ok ? c : 0)];
+        vr[it] = ok ? v : C{0, 0};
       }
     }
+  };
+  auto commit = [&]() {
+    if (kcontig) {
+#pragma unroll
+      for (int it = 0; it < NT; ++it) {
+        const int idx = tid + 256 * it;
+        const int qq = idx / MSG_IB, c = idx - qq * MSG_IB;
+        tre[qq][c] = tr[it].x;
+        tim[qq][c] = tr[it].y;
+      }
+#pragma unroll
+      for (int it = 0; it < NV; ++it) {
+        const int idx = tid + 256 * it;
+        const int qq = idx / NC, c = idx - qq * NC;
+        vre[qq][c] = vr[it].x;
+        vim[qq][c] = vr[it].y;
+      }
+    } else {
+      const int qq = tid % MSG_QS;
+#pragma unroll
+      for (int it = 0; it < NT; ++it) {
+        const int c = tid / MSG_QS + (256 / MSG_QS) * it;
+        tre[qq][c] = tr[it].x;
+        tim[qq][c] = tr[it].y;
+      }
+#pragma unroll
+      for (int it = 0; it < NV; ++it) {
+        const int c = tid / MSG_QS + (256 / MSG_QS) * it;
+        vre[qq][c] = vr[it].x;
+        vim[qq][c] = vr[it].y;
+      }
+    }
+  };
+
+  if (qbeg < qend) fetch(qbeg);
+  for (int64_t q0 = qbeg; q0 < qend; q0 += MSG_QS) {
     __syncthreads();
+    commit();
+    __syncthreads();
+    if (q0 + MSG_QS < qend) fetch(q0 + MSG_QS);  // in flight during the MFMAs below
 #pragma unroll 2
     for (int k4 = 0; k4 < MSG_QS; k4 += 4) {
       const S ar = tre[k4 + lk][wave * 16 + li];
@@ -302,6 +374,32 @@ __global__ __launch_bounds__(256) void k_message_partials(
       if (j >= D2) continue;
       out[i + (int64_t)D1 * j] = make_double2((double)acc_re[cb][v], (double)acc_im[cb][v]);
     }
+  }
+}
+
+// w[i] = prod_b E_b[c_b, c_b] over the column-major decomposition of i into
+// (dims[0], ..., dims[nb-1], unit); the trailing unit factor covers indices
+// that carry no message (the physical bond).
+struct WeightArgs {
+  int nb;
+  int64_t dims[HBP_MAX_ORDER];
+  int64_t diag_off[HBP_MAX_ORDER];  // offset of the message's (0,0) element
+};
+__global__ void k_build_weights(const double2 *__restrict__ E, double2 *__restrict__ w, int64_t n,
+                                WeightArgs a) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int64_t rem = i;
+    double re = 1.0, im = 0.0;
+    for (int b = 0; b < a.nb; ++b) {
+      const int64_t c = rem % a.dims[b];
+      rem /= a.dims[b];
+      const double2 e = E[a.diag_off[b] + c * (a.dims[b] + 1)];
+      const double r2 = re * e.x - im * e.y, i2 = re * e.y + im * e.x;
+      re = r2;
+      im = i2;
+    }
+    w[i] = make_double2(re, im);
   }
 }
 
@@ -693,8 +791,9 @@ template void launch_mode_product<float>(hipStream_t, const void *, const double
                                          int *);
 
 int64_t message_num_chunks(int64_t Q) {
-  // about 1024 rows per chunk, between 1 and 1024 chunks, rows a multiple of MSG_QS
-  int64_t nch = (Q + 1023) / 1024;
+  // enough workgroups to fill 256 CUs several times over: up to 1024 chunks,
+  // at least MSG_QS rows each
+  int64_t nch = (Q + MSG_QS - 1) / MSG_QS;
   if (nch < 1) nch = 1;
   if (nch > 1024) nch = 1024;
   return nch;
@@ -703,18 +802,25 @@ int64_t message_num_chunks(int64_t Q) {
 template <typename S, int NCB>
 static void msg_launch(hipStream_t st, const void *T, const void *V, double2 *partials,
                        int64_t L, int D1, int D2, int64_t Q, int64_t nch, int64_t chunk,
-                       int64_t ldT_R, int64_t ldV_R, int conjV) {
+                       int64_t ldT_R, int64_t ldV_R, int conjV, const double2 *wL,
+                       const double2 *wR, int Da, int64_t Sb) {
   using C = typename Cplx<S>::type;
   dim3 grid((unsigned)nch, (unsigned)((D1 + MSG_IB - 1) / MSG_IB));
   hipLaunchKernelGGL((k_message_partials<S, NCB>), grid, dim3(256), 0, st, (const C *)T,
-                     (const C *)V, partials, L, D1, D2, Q, chunk, ldT_R, ldV_R, conjV);
+                     (const C *)V, partials, L, D1, D2, Q, chunk, ldT_R, ldV_R, conjV, wL, wR,
+                     Da, Sb);
   HBP_HIP(hipGetLastError());
 }
 
 template <typename S>
 int message_partials(hipStream_t st, const void *T, const void *V, double2 *partials,
                      int64_t L, int D1, int D2, int64_t R, int64_t ldT_R, int64_t ldV_R,
-                     int64_t max_chunks, int conjV) {
+                     int64_t max_chunks, int conjV, const double2 *wL, const double2 *wR,
+                     int Da, int64_t Sb) {
+  if (Da <= 0) {
+    Da = std::max(D1, D2);
+    Sb = 0;
+  }
   HBP_REQUIRE(D2 <= 128, HBP_ERR_UNSUPPORTED, "message: bond > 128");
   const int64_t Q = L * R;
   int64_t nch = message_num_chunks(Q);
@@ -724,23 +830,35 @@ int message_partials(hipStream_t st, const void *T, const void *V, double2 *part
   nch = (Q + chunk - 1) / chunk;
   const int ncb = (D2 + 15) / 16;
   switch (ncb) {
-    case 1: msg_launch<S, 1>(st, T, V, partials, L, D1, D2, Q, nch, chunk, ldT_R, ldV_R, conjV); break;
-    case 2: msg_launch<S, 2>(st, T, V, partials, L, D1, D2, Q, nch, chunk, ldT_R, ldV_R, conjV); break;
-    case 3: msg_launch<S, 3>(st, T, V, partials, L, D1, D2, Q, nch, chunk, ldT_R, ldV_R, conjV); break;
-    case 4: msg_launch<S, 4>(st, T, V, partials, L, D1, D2, Q, nch, chunk, ldT_R, ldV_R, conjV); break;
-    case 5: msg_launch<S, 5>(st, T, V, partials, L, D1, D2, Q, nch, chunk, ldT_R, ldV_R, conjV); break;
-    case 6: msg_launch<S, 6>(st, T, V, partials, L, D1, D2, Q, nch, chunk, ldT_R, ldV_R, conjV); break;
-    case 7: msg_launch<S, 7>(st, T, V, partials, L, D1, D2, Q, nch, chunk, ldT_R, ldV_R, conjV); break;
-    default: msg_launch<S, 8>(st, T, V, partials, L, D1, D2, Q, nch, chunk, ldT_R, ldV_R, conjV); break;
+    case 1: msg_launch<S, 1>(st, T, V, partials, L, D1, D2, Q, nch, chunk, ldT_R, ldV_R, conjV, wL, wR, Da, Sb); break;
+    case 2: msg_launch<S, 2>(st, T, V, partials, L, D1, D2, Q, nch, chunk, ldT_R, ldV_R, conjV, wL, wR, Da, Sb); break;
+    case 3: msg_launch<S, 3>(st, T, V, partials, L, D1, D2, Q, nch, chunk, ldT_R, ldV_R, conjV, wL, wR, Da, Sb); break;
+    case 4: msg_launch<S, 4>(st, T, V, partials, L, D1, D2, Q, nch, chunk, ldT_R, ldV_R, conjV, wL, wR, Da, Sb); break;
+    case 5: msg_launch<S, 5>(st, T, V, partials, L, D1, D2, Q, nch, chunk, ldT_R, ldV_R, conjV, wL, wR, Da, Sb); break;
+    case 6: msg_launch<S, 6>(st, T, V, partials, L, D1, D2, Q, nch, chunk, ldT_R, ldV_R, conjV, wL, wR, Da, Sb); break;
+    case 7: msg_launch<S, 7>(st, T, V, partials, L, D1, D2, Q, nch, chunk, ldT_R, ldV_R, conjV, wL, wR, Da, Sb); break;
+    default: msg_launch<S, 8>(st, T, V, partials, L, D1, D2, Q, nch, chunk, ldT_R, ldV_R, conjV, wL, wR, Da, Sb); break;
   }
   return (int)nch;
 }
 template int message_partials<double>(hipStream_t, const void *, const void *, double2 *,
                                       int64_t, int, int, int64_t, int64_t, int64_t, int64_t,
-                                      int);
+                                      int, const double2 *, const double2 *, int, int64_t);
 template int message_partials<float>(hipStream_t, const void *, const void *, double2 *,
                                      int64_t, int, int, int64_t, int64_t, int64_t, int64_t,
-                                     int);
+                                     int, const double2 *, const double2 *, int, int64_t);
+
+void launch_build_weights(hipStream_t st, const double2 *E, double2 *w, int64_t n, int nb,
+                          const int64_t *dims, const int64_t *diag_off) {
+  WeightArgs a{};
+  a.nb = nb;
+  for (int b = 0; b < nb; ++b) {
+    a.dims[b] = dims[b];
+    a.diag_off[b] = diag_off[b];
+  }
+  hipLaunchKernelGGL(k_build_weights, dim3(grid_for((size_t)n)), dim3(256), 0, st, E, w, n, a);
+  HBP_HIP(hipGetLastError());
+}
 
 void launch_reduce_partials(hipStream_t st, const double2 *partials, int nchunks, int n,
                             double2 *M) {

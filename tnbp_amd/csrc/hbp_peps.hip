@@ -339,21 +339,25 @@ __global__ __launch_bounds__(256) void k_edge_ratio(const double2 *__restrict__ 
   const double2 *A = AA + env_off[t], *B = BB + env_off[t];
   const double2 *O = ops + (int64_t)t * d * d * d * d;
   double rr = 0.0, ri = 0.0, sr = 0.0, si = 0.0;
+  const int64_t Dd = (int64_t)D * d;
+  // element (p, p', i, j) of an environment sits at (i + D p) + Dd (j + D p')
+  auto at = [&](int pp, int qq, int i, int j) { return (i + (int64_t)D * pp) + Dd * (j + (int64_t)D * qq); };
   for (int64_t e = threadIdx.x; e < DD; e += blockDim.x) {
+    const int i = (int)(e % D), j = (int)(e / D);
     double tar = 0.0, tai = 0.0, tbr = 0.0, tbi = 0.0;
     for (int p = 0; p < d; ++p) {
-      const double2 a = A[(p + d * p) * DD + e], b = B[(p + d * p) * DD + e];
+      const double2 a = A[at(p, p, i, j)], b = B[at(p, p, i, j)];
       tar += a.x; tai += a.y; tbr += b.x; tbi += b.y;
     }
     sr += tar * tbr - tai * tbi;
     si += tar * tbi + tai * tbr;
     for (int p = 0; p < d; ++p)
       for (int pp = 0; pp < d; ++pp) {
-        const double2 a = A[(p + d * pp) * DD + e];
+        const double2 a = A[at(p, pp, i, j)];
         for (int x = 0; x < d; ++x)
           for (int y = 0; y < d; ++y) {
             const double2 o = O[p + d * (x + d * (pp + d * y))];
-            const double2 b = B[(x + d * y) * DD + e];
+            const double2 b = B[at(x, y, i, j)];
             const double ar = a.x * o.x - a.y * o.y, ai = a.x * o.y + a.y * o.x;
             rr += ar * b.x - ai * b.y;
             ri += ar * b.y + ai * b.x;
@@ -391,11 +395,12 @@ struct Peps {
   double2 *E = nullptr;                    // all 2 n_e messages
   size_t E_bytes = 0;
   std::vector<int64_t> E_off;              // message -> offset (complex<double>)
-  // work buffers
-  void *buf[2] = {nullptr, nullptr};
+  // message m is exactly diagonal (after init, AbsorbTPO's E (x) I and
+  // Truncation's diag(L)); a BP sweep produces dense messages
+  std::vector<char> E_diag;
+  // work buffers, each as large as the largest site tensor
+  void *buf[3] = {nullptr, nullptr, nullptr};
   size_t buf_bytes = 0;
-  void *buf3 = nullptr;  // plain mode: the conjugated copy of V
-  size_t buf3_bytes = 0;
   int mode = HBP_MODE_FUSED;
   // mixed precision: BP stops once the residual is below this (FP32 messages
   // cannot reach the FP64 tolerances), or below the caller's tolerance
@@ -439,10 +444,66 @@ static void layout_messages(Ctx *ctx, Peps *p, double2 **newE, size_t *bytes,
 
 static void ensure_work(Ctx *ctx, Peps *p, size_t bytes) {
   if (bytes <= p->buf_bytes) return;
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < 3; ++i)
     if (p->buf[i]) ctx->release(p->buf[i], p->buf_bytes);
-  for (int i = 0; i < 2; ++i) p->buf[i] = ctx->alloc(bytes);
+  for (int i = 0; i < 3; ++i) p->buf[i] = ctx->alloc(bytes);
   p->buf_bytes = bytes;
+}
+
+// Row weights of a mode product with diagonal messages: w[i] = prod_j
+// diag(E_in(j))[c_j] over the bonds j in [j0, j1) of site s, i split
+// column-major over their dims, times a trailing unit factor of `unit`.
+static double2 *diag_weights(Ctx *ctx, Peps *p, int s, int j0, int j1, int64_t unit,
+                             int64_t *n_out) {
+  int64_t dims[HBP_MAX_ORDER], offs[HBP_MAX_ORDER];
+  int nb = 0;
+  int64_t n = unit;
+  for (int j = j0; j < j1; ++j) {
+    dims[nb] = p->bond_dim[p->bonds[s][j]];
+    offs[nb] = p->E_off[p->in_msg(s, j)];
+    n *= dims[nb];
+    ++nb;
+  }
+  *n_out = n;
+  if (nb == 0) return nullptr;
+  double2 *w = (double2 *)ctx->alloc((size_t)n * 16);
+  launch_build_weights(ctx->stream, p->E, w, n, nb, dims, offs);
+  return w;
+}
+
+// M = sum over all bonds but k of (V x_{j != k} diag E_in(j))[.. i ..] conj V[.. j ..]:
+// every incoming message on the other bonds is diagonal, so the mode products
+// are row weights folded into the message kernel.  k == z gives the reduced
+// matrix over the physical index (measure.h:54-84).
+template <typename S>
+static void weighted_message(Ctx *ctx, Peps *p, int s, int k, double2 *M) {
+  const int z = p->z(s);
+  int32_t sh[HBP_MAX_ORDER];
+  p->shape(s, sh);
+  int64_t L = 1, R = 1;
+  for (int d = 0; d < k; ++d) L *= sh[d];
+  for (int d = k + 1; d <= z; ++d) R *= sh[d];
+  const int D = sh[k];
+  int64_t nl = 0, nr = 0;
+  double2 *wL = diag_weights(ctx, p, s, 0, k, 1, &nl);
+  double2 *wR = k < z ? diag_weights(ctx, p, s, k + 1, z, p->phys[s], &nr) : nullptr;
+  const int64_t nch = message_num_chunks(L * R);
+  double2 *parts = (double2 *)ctx->get_scratch((size_t)nch * D * D * 16 + 64);
+  const int pr = ctx->prof_begin(HBP_PROF_MESSAGE, 8.0 * L * R * D * D,
+                                 2.0 * sizeof(S) * 2.0 * L * R * D);
+  const int n = message_partials<S>(ctx->stream, p->V[s], p->V[s], parts, L, D, D, R, L * D,
+                                    L * D, nch, 1, wL, wR);
+  launch_reduce_partials(ctx->stream, parts, n, D * D, M);
+  ctx->prof_end(pr);
+  if (wL) ctx->release(wL, (size_t)nl * 16);
+  if (wR) ctx->release(wR, (size_t)nr * 16);
+}
+
+// every message entering site s except on bond `skip` is diagonal
+static bool diag_env(const Peps *p, int s, int skip) {
+  for (int k = 0; k < p->z(s); ++k)
+    if (k != skip && !p->E_diag[p->in_msg(s, k)]) return false;
+  return true;
 }
 
 template <typename S>
@@ -502,9 +563,50 @@ static void sweep_raw(Ctx *ctx, Peps *p, double2 *raw, const std::vector<int64_t
   ensure_work(ctx, p, maxb);
   int32_t sh[HBP_MAX_ORDER];
   for (int s = 0; s < p->n_sites; ++s) {
-    for (int k = 0; k < p->z(s); ++k) {
-      const void *T = absorb_env<S>(ctx, p, s, k, sh);
-      message_raw<S>(ctx, T, p->V[s], sh, p->z(s) + 1, k, raw + raw_off[p->out_msg(s, k)]);
+    const int z = p->z(s), order = z + 1;
+    p->shape(s, sh);
+    if (diag_env(p, s, -1)) {
+      // first sweep after AbsorbTPO / Truncation: E = diag(L) (x) I on every bond
+      for (int k = 0; k < z; ++k) weighted_message<S>(ctx, p, s, k, raw + raw_off[p->out_msg(s, k)]);
+      continue;
+    }
+    if (z <= 2) {
+      for (int k = 0; k < z; ++k) {
+        const void *T = absorb_env<S>(ctx, p, s, k, sh);
+        message_raw<S>(ctx, T, p->V[s], sh, order, k, raw + raw_off[p->out_msg(s, k)]);
+      }
+      continue;
+    }
+    // Shared partial products: with the bonds split in halves H1 = [0, h) and
+    // H2 = [h, z), A = V x_{H1} serves every target in H2 and B = V x_{H2}
+    // every target in H1 (8 instead of 12 mode products at z = 4).  Mode
+    // products on different bonds commute; only rounding changes.
+    const int h = z / 2;
+    for (int half = 0; half < 2; ++half) {
+      const int a0 = half == 0 ? 0 : h, a1 = half == 0 ? h : z;   // absorbed first
+      const int t0 = half == 0 ? h : 0, t1 = half == 0 ? z : h;   // targets
+      // A = V x_{[a0, a1)} into buf[0] (ping-pong with buf[1] when needed)
+      const void *cur = p->V[s];
+      int nprod = a1 - a0, w = (nprod % 2 == 1) ? 0 : 1;
+      for (int j = a0; j < a1; ++j) {
+        void *dst = p->buf[w];
+        mode_prod<S>(ctx, cur, p->E + p->E_off[p->in_msg(s, j)], dst, sh, order, j, sh[j], 0);
+        cur = dst;
+        w ^= 1;
+      }
+      // cur == buf[0] (or V when nothing was absorbed); targets use buf[1], buf[2]
+      for (int k = t0; k < t1; ++k) {
+        const void *T = cur;
+        int u = 1;
+        for (int j = t0; j < t1; ++j) {
+          if (j == k) continue;
+          void *dst = p->buf[u];
+          mode_prod<S>(ctx, T, p->E + p->E_off[p->in_msg(s, j)], dst, sh, order, j, sh[j], 0);
+          T = dst;
+          u = u == 1 ? 2 : 1;
+        }
+        message_raw<S>(ctx, T, p->V[s], sh, order, k, raw + raw_off[p->out_msg(s, k)]);
+      }
     }
   }
 }
@@ -518,11 +620,6 @@ static void sweep_raw_plain(Ctx *ctx, Peps *p, double2 *raw, const std::vector<i
   size_t maxb = 16;
   for (int s = 0; s < p->n_sites; ++s) maxb = std::max(maxb, p->site_size(s) * p->elem_bytes());
   ensure_work(ctx, p, maxb);
-  if (p->buf3_bytes < maxb) {
-    if (p->buf3) ctx->release(p->buf3, p->buf3_bytes);
-    p->buf3 = ctx->alloc(maxb);
-    p->buf3_bytes = maxb;
-  }
   double *dv = (double *)ctx->alloc(sizeof(double));
   int32_t sh[HBP_MAX_ORDER];
   for (int s = 0; s < p->n_sites; ++s) {
@@ -541,8 +638,8 @@ static void sweep_raw_plain(Ctx *ctx, Peps *p, double2 *raw, const std::vector<i
         device_norm2(ctx, p->buf[cur], n, p->site_dtype(), dv);
         launch_scale(ctx->stream, p->buf[cur], n, p->site_dtype(), 1.0, 0.0, dv);
       }
-      HBP_HIP(hipMemcpyAsync(p->buf3, p->V[s], bytes, hipMemcpyDeviceToDevice, ctx->stream));
-      launch_conj(ctx->stream, p->buf3, n, p->site_dtype());
+      HBP_HIP(hipMemcpyAsync(p->buf[2], p->V[s], bytes, hipMemcpyDeviceToDevice, ctx->stream));
+      launch_conj(ctx->stream, p->buf[2], n, p->site_dtype());
       int64_t L = 1, R = 1;
       for (int d = 0; d < k; ++d) L *= sh[d];
       for (int d = k + 1; d <= z; ++d) R *= sh[d];
@@ -551,7 +648,7 @@ static void sweep_raw_plain(Ctx *ctx, Peps *p, double2 *raw, const std::vector<i
       double2 *parts = (double2 *)ctx->get_scratch((size_t)nch * D * D * 16 + 64);
       const int pr = ctx->prof_begin(HBP_PROF_MESSAGE, 8.0 * L * R * D * D,
                                      2.0 * sizeof(S) * 2.0 * L * R * D);
-      const int nc = message_partials<S>(ctx->stream, p->buf[cur], p->buf3, parts, L, D, D, R,
+      const int nc = message_partials<S>(ctx->stream, p->buf[cur], p->buf[2], parts, L, D, D, R,
                                          L * D, L * D, nch, 0);
       double2 *M = raw + raw_off[p->out_msg(s, k)];
       launch_reduce_partials(ctx->stream, parts, nc, D * D, M);
@@ -743,6 +840,7 @@ static void truncate(Ctx *ctx, Peps *p, int chi, double sv_min, int32_t *dims_ou
   p->E = nE;
   p->E_bytes = nbytes;
   p->E_off = noff;
+  p->E_diag.assign(2 * ne, 1);
   if (dims_out)
     for (int m = 0; m < ne; ++m) dims_out[m] = keep[m];
   if (errs_out)
@@ -778,6 +876,10 @@ static void measure_sites_impl(Ctx *ctx, Peps *p, int n, const int *addr, const 
   int32_t sh[HBP_MAX_ORDER];
   for (size_t u = 0; u < uniq.size(); ++u) {
     const int s = uniq[u];
+    if (diag_env(p, s, -1)) {  // after Truncation every message is diag(L)
+      weighted_message<S>(ctx, p, s, p->z(s), rho + u * maxd * maxd);
+      continue;
+    }
     const void *W = absorb_env<S>(ctx, p, s, -1, sh);
     // rho[p, p'] = sum_b W[b, p] conj V[b, p']: message over the physical axis
     message_raw<S>(ctx, W, p->V[s], sh, p->z(s) + 1, p->z(s), rho + u * maxd * maxd);
@@ -804,28 +906,36 @@ static void measure_sites_impl(Ctx *ctx, Peps *p, int n, const int *addr, const 
   ctx->release(rho, uniq.size() * maxd * maxd * 16);
 }
 
-// AdagA blocks M_{p p'}(i, j) = sum A[.., i, .., p] conj V[.., j, .., p'] (measure.h:194-204)
+// AdagA[p, p', i, j] = sum A[.., i, .., p] conj V[.., j, .., p'] (measure.h:194-204)
+// in one pass: the physical slices become further columns, so out is the
+// (D d) x (D d) matrix with row i + D p and column j + D p'.
 template <typename S>
 static void edge_env(Ctx *ctx, Peps *p, int s, int kt, double2 *out) {
   int32_t sh[HBP_MAX_ORDER];
-  const void *A = absorb_env<S>(ctx, p, s, kt, sh);
+  const bool diag = diag_env(p, s, kt);
+  const void *A = diag ? p->V[s] : absorb_env<S>(ctx, p, s, kt, sh);
+  if (diag) p->shape(s, sh);
   const int z = p->z(s), d = p->phys[s];
+  // diagonal environment: the absorbed bonds become row weights of A
+  int64_t nl = 0, nr = 0;
+  double2 *wL = diag ? diag_weights(ctx, p, s, 0, kt, 1, &nl) : nullptr;
+  double2 *wR = diag ? diag_weights(ctx, p, s, kt + 1, z, 1, &nr) : nullptr;
   int64_t L = 1, R = 1;
   for (int k = 0; k < kt; ++k) L *= sh[k];
   for (int k = kt + 1; k < z; ++k) R *= sh[k];
-  const int D = sh[kt];
+  const int D = sh[kt], Dd = D * d;
+  HBP_REQUIRE(Dd <= 128, HBP_ERR_UNSUPPORTED, "measure: bond x physical dim above 128");
   const int64_t slab = L * D * R;  // elements per physical index
   const int64_t nch = message_num_chunks(L * R);
-  double2 *parts = (double2 *)ctx->get_scratch((size_t)nch * D * D * 16 + 64);
-  const size_t es = p->elem_bytes();
-  for (int pp = 0; pp < d; ++pp)
-    for (int q = 0; q < d; ++q) {
-      const void *Ap = (const char *)A + (size_t)pp * slab * es;
-      const void *Vq = (const char *)p->V[s] + (size_t)q * slab * es;
-      const int nc = message_partials<S>(ctx->stream, Ap, Vq, parts, L, D, D, R, L * D, L * D,
-                                         nch, 1);
-      launch_reduce_partials(ctx->stream, parts, nc, D * D, out + (int64_t)(pp + d * q) * D * D);
-    }
+  double2 *parts = (double2 *)ctx->get_scratch((size_t)nch * Dd * Dd * 16 + 64);
+  const int pr = ctx->prof_begin(HBP_PROF_MESSAGE, 8.0 * L * R * Dd * Dd,
+                                 2.0 * sizeof(S) * 2.0 * L * R * Dd);
+  const int nc = message_partials<S>(ctx->stream, A, p->V[s], parts, L, Dd, Dd, R, L * D, L * D,
+                                     nch, 1, wL, wR, D, slab);
+  launch_reduce_partials(ctx->stream, parts, nc, Dd * Dd, out);
+  ctx->prof_end(pr);
+  if (wL) ctx->release(wL, (size_t)nl * 16);
+  if (wR) ctx->release(wR, (size_t)nr * 16);
 }
 
 template <typename S>
@@ -962,6 +1072,7 @@ hbp_status hbp_peps_create(hbp_ctx *ctx, int n_edges, const int32_t *edges,
   // 1x1 messages equal to one (init.h:72-78)
   layout_messages(ctx, p, &p->E, &p->E_bytes, &p->E_off);
   launch_fill(ctx->stream, p->E, 2 * n_edges, HBP_C128, 1.0, 0.0);
+  p->E_diag.assign(2 * n_edges, 1);
   ctx->sync();
   *out = p;
   API_END
@@ -972,9 +1083,8 @@ hbp_status hbp_peps_destroy(hbp_ctx *ctx, hbp_peps *p) {
   if (!p) return HBP_OK;
   for (int s = 0; s < p->n_sites; ++s) ctx->release(p->V[s], p->V_bytes[s]);
   ctx->release(p->E, p->E_bytes);
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < 3; ++i)
     if (p->buf[i]) ctx->release(p->buf[i], p->buf_bytes);
-  if (p->buf3) ctx->release(p->buf3, p->buf3_bytes);
   ctx->sync();
   delete p;
   API_END
@@ -1043,6 +1153,14 @@ hbp_status hbp_peps_set_message(hbp_ctx *ctx, hbp_peps *p, int m, int32_t dim,
   API_BEGIN
   HBP_REQUIRE(m >= 0 && m < 2 * p->n_edges, HBP_ERR_OUT_OF_RANGE, "message out of range");
   HBP_REQUIRE(dim == p->msg_dim(m), HBP_ERR_INVALID_ARGUMENT, "set_message: dim");
+  bool diag = true;
+  for (int c = 0; c < dim && diag; ++c)
+    for (int r = 0; r < dim; ++r)
+      if (r != c && (host[2 * (r + (size_t)dim * c)] != 0.0 || host[2 * (r + (size_t)dim * c) + 1] != 0.0)) {
+        diag = false;
+        break;
+      }
+  p->E_diag[m] = diag ? 1 : 0;
   HBP_HIP(hipMemcpyAsync(p->E + p->E_off[m], host, (size_t)dim * dim * 16, hipMemcpyHostToDevice,
                          ctx->stream));
   ctx->sync();
@@ -1153,6 +1271,7 @@ hbp_status hbp_peps_bp_sweep(hbp_ctx *ctx, hbp_peps *p) {
   sweep(ctx, p, nE, false, nullptr);
   ctx->release(p->E, p->E_bytes);
   p->E = nE;
+  p->E_diag.assign(2 * p->n_edges, 0);
   API_END
 }
 
@@ -1183,6 +1302,7 @@ hbp_status hbp_peps_bp_run(hbp_ctx *ctx, hbp_peps *p, int max_iter, double tol, 
       sweep(ctx, p, nE, false, nullptr);
       ctx->release(p->E, p->E_bytes);
       p->E = nE;
+      p->E_diag.assign(2 * p->n_edges, 0);
       double2 *G = (double2 *)ctx->alloc(p->E_bytes);
       sweep(ctx, p, G, true, dres);
       ctx->release(G, p->E_bytes);
@@ -1201,6 +1321,7 @@ hbp_status hbp_peps_bp_run(hbp_ctx *ctx, hbp_peps *p, int max_iter, double tol, 
   sweep(ctx, p, nE, false, nullptr);
   ctx->release(p->E, p->E_bytes);
   p->E = nE;
+  p->E_diag.assign(2 * p->n_edges, 0);
   double *dres = (double *)ctx->alloc(sizeof(double));
   int it = 0;
   for (it = 1; it <= max_iter; ++it) {
@@ -1217,6 +1338,7 @@ hbp_status hbp_peps_bp_run(hbp_ctx *ctx, hbp_peps *p, int max_iter, double tol, 
     }
     ctx->release(p->E, p->E_bytes);
     p->E = G;
+    p->E_diag.assign(2 * p->n_edges, 0);
   }
   if (iters) *iters = std::min(it, max_iter);
   ctx->release(dres, sizeof(double));
