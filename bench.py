@@ -139,7 +139,9 @@ def main():
     def step(record):
         t0 = time.perf_counter()
         AbsorbTPO(ctx, tpo, peps)
-        it, res = bp_loop(ctx, peps, args.max_bp_iterations, args.tol)
+        it, res = bp_loop(ctx, peps, args.max_bp_iterations, args.tol,
+                          progress=lambda k, r: print(json.dumps({"bp": k, "residual": r}),
+                                                      file=sys.stderr, flush=True))
         dims, errs = Truncation(ctx, peps, chi, 1e-8)
         z = Measure(ctx, peps, sites, [zop] * len(sites))
         zzv = MeasureEdges(ctx, peps, edges, [zz] * len(edges))

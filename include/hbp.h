@@ -193,6 +193,13 @@ hbp_status hbp_peps_bp_condition(hbp_ctx *ctx, hbp_peps *p, double *residual);
  * residuals (may be NULL) receives max_iter values; iters the count run. */
 hbp_status hbp_peps_bp_run(hbp_ctx *ctx, hbp_peps *p, int max_iter, double tol, int *iters,
                            double *residuals);
+/* The same loop in pieces, for callers that interleave work or report
+ * progress: up to n iterations; `first` starts a loop (main.cc:121), `last`
+ * marks the piece that ends at max_bp_iterations.  converged = 1 once the
+ * residual fell below tol.  Consecutive pieces give the results of one call
+ * of hbp_peps_bp_run. */
+hbp_status hbp_peps_bp_iterate(hbp_ctx *ctx, hbp_peps *p, int n, double tol, int first,
+                               int last, int *iters, double *residuals, int *converged);
 /* tnbp::Truncation (truncation.h:17-222 with SquareRootAndInverse root.h:65-148):
  * bond_dims / trunc_errs (may be NULL) receive one value per edge. */
 hbp_status hbp_peps_truncate(hbp_ctx *ctx, hbp_peps *p, int chi_max, double sv_min,

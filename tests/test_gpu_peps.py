@@ -174,3 +174,29 @@ def test_residual_matches_condition(ctx):
         for m in (0, 5, len(edges) + 3):
             np.testing.assert_allclose(p.get_message(m), st.E[m], atol=1e-12)
     p.close()
+
+
+def test_bp_iterate_pieces_equal_one_call(ctx):
+    """bp_loop one iteration per call gives the same iterations, residuals and
+    state as hbp_peps_bp_run (also when stopping at max_bp_iterations)."""
+    from tnbp_amd import circuits as pc
+    from tnbp_amd.peps import AbsorbTPO, InitTensorProductState, Measure, bp_loop
+    announce("bp pieces")
+    rng = np.random.default_rng(9)
+    edges = pc.square_lattice(3, 3)
+    tpo = pc.layer_tpo(edges, pc.random_layer(rng, edges, 9))
+    for max_iter, tol in ((50, 1e-10), (3, 0.0)):
+        runs = []
+        for pieces in (False, True):
+            p = InitTensorProductState(ctx, edges)
+            for _ in range(3):
+                AbsorbTPO(ctx, tpo, p)
+            prog = (lambda k, r: None) if pieces else None
+            it, res = bp_loop(ctx, p, max_iter, tol, progress=prog)
+            z = Measure(ctx, p, list(range(9)), [pc.op_tensor(pc.PAULI["Z"])] * 9)
+            runs.append((it, res, z, p.get_message(3)))
+            p.close()
+        assert runs[0][0] == runs[1][0]
+        np.testing.assert_array_equal(runs[0][1], runs[1][1])
+        np.testing.assert_array_equal(runs[0][2], runs[1][2])
+        np.testing.assert_array_equal(runs[0][3], runs[1][3])

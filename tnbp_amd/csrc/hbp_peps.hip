@@ -1287,17 +1287,22 @@ hbp_status hbp_peps_bp_condition(hbp_ctx *ctx, hbp_peps *p, double *residual) {
   API_END
 }
 
-hbp_status hbp_peps_bp_run(hbp_ctx *ctx, hbp_peps *p, int max_iter, double tol, int *iters,
-                           double *residuals) {
-  API_BEGIN
-  HBP_REQUIRE(max_iter >= 1, HBP_ERR_INVALID_ARGUMENT, "bp_run: max_iter < 1");
+// Up to n iterations of the driver's BP loop (main.cc:121-159).  `first`
+// starts a loop: the fused path then runs the loop's first sweep E_1 = BP(E_0).
+// Each fused iteration computes G = BP(E) once: G gives the condition's
+// residual against E, and becomes the next E unless the loop stops there.
+// When n iterations end without convergence and `last` is 0, G is kept so
+// that the next call continues the same sequence; with `last` the loop stops
+// on E like the reference at max_bp_iterations.
+static void bp_iterate(Ctx *ctx, Peps *p, int n, double tol, int first, int last, int *iters,
+                       double *residuals, int *converged) {
+  HBP_REQUIRE(n >= 1, HBP_ERR_INVALID_ARGUMENT, "bp: iteration count < 1");
   const double stop = p->precision == HBP_PREC_MIXED ? std::max(tol, p->mixed_floor) : tol;
+  double *dres = (double *)ctx->alloc(sizeof(double));
+  int it = 0, conv = 0;
   if (p->mode == HBP_MODE_PLAIN) {
-    // the driver loop as written (main.cc:121-159): every iteration is a sweep
-    // (E <- F) followed by a separate condition sweep
-    double *dres = (double *)ctx->alloc(sizeof(double));
-    int it = 0;
-    for (it = 1; it <= max_iter; ++it) {
+    // every iteration is a sweep (E <- F) followed by a separate condition sweep
+    for (it = 1; it <= n; ++it) {
       double2 *nE = (double2 *)ctx->alloc(p->E_bytes);
       sweep(ctx, p, nE, false, nullptr);
       ctx->release(p->E, p->E_bytes);
@@ -1310,38 +1315,52 @@ hbp_status hbp_peps_bp_run(hbp_ctx *ctx, hbp_peps *p, int max_iter, double tol, 
       HBP_HIP(hipMemcpyAsync(&r, dres, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
       ctx->sync();
       if (residuals) residuals[it - 1] = r;
-      if (r < stop) break;
+      if (r < stop) {
+        conv = 1;
+        break;
+      }
     }
-    if (iters) *iters = std::min(it, max_iter);
-    ctx->release(dres, sizeof(double));
-    return HBP_OK;
-  }
-  // first sweep: E_1 = BP(E_0)
-  double2 *nE = (double2 *)ctx->alloc(p->E_bytes);
-  sweep(ctx, p, nE, false, nullptr);
-  ctx->release(p->E, p->E_bytes);
-  p->E = nE;
-  p->E_diag.assign(2 * p->n_edges, 0);
-  double *dres = (double *)ctx->alloc(sizeof(double));
-  int it = 0;
-  for (it = 1; it <= max_iter; ++it) {
-    // the condition's recomputed messages are the next sweep's messages
-    double2 *G = (double2 *)ctx->alloc(p->E_bytes);
-    sweep(ctx, p, G, true, dres);
-    double r = 0.0;
-    HBP_HIP(hipMemcpyAsync(&r, dres, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    ctx->sync();
-    if (residuals) residuals[it - 1] = r;
-    if (r < stop || it == max_iter) {
-      ctx->release(G, p->E_bytes);
-      break;
+  } else {
+    if (first) {
+      double2 *nE = (double2 *)ctx->alloc(p->E_bytes);
+      sweep(ctx, p, nE, false, nullptr);
+      ctx->release(p->E, p->E_bytes);
+      p->E = nE;
+      p->E_diag.assign(2 * p->n_edges, 0);
     }
-    ctx->release(p->E, p->E_bytes);
-    p->E = G;
-    p->E_diag.assign(2 * p->n_edges, 0);
+    for (it = 1; it <= n; ++it) {
+      double2 *G = (double2 *)ctx->alloc(p->E_bytes);
+      sweep(ctx, p, G, true, dres);
+      double r = 0.0;
+      HBP_HIP(hipMemcpyAsync(&r, dres, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+      ctx->sync();
+      if (residuals) residuals[it - 1] = r;
+      if (r < stop || (it == n && last)) {
+        conv = r < stop ? 1 : 0;
+        ctx->release(G, p->E_bytes);
+        break;
+      }
+      ctx->release(p->E, p->E_bytes);
+      p->E = G;
+      p->E_diag.assign(2 * p->n_edges, 0);
+    }
   }
-  if (iters) *iters = std::min(it, max_iter);
+  if (iters) *iters = std::min(it, n);
+  if (converged) *converged = conv;
   ctx->release(dres, sizeof(double));
+}
+
+hbp_status hbp_peps_bp_run(hbp_ctx *ctx, hbp_peps *p, int max_iter, double tol, int *iters,
+                           double *residuals) {
+  API_BEGIN
+  bp_iterate(ctx, p, max_iter, tol, 1, 1, iters, residuals, nullptr);
+  API_END
+}
+
+hbp_status hbp_peps_bp_iterate(hbp_ctx *ctx, hbp_peps *p, int n, double tol, int first,
+                               int last, int *iters, double *residuals, int *converged) {
+  API_BEGIN
+  bp_iterate(ctx, p, n, tol, first, last, iters, residuals, converged);
   API_END
 }
 

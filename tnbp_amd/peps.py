@@ -122,13 +122,34 @@ def BeliefPropagationCondition(ctx, peps: Peps) -> float:
     return r.value
 
 
-def bp_loop(ctx, peps: Peps, max_iter: int, tol: float):
-    """The drivers' BP loop (main.cc:121-159) -> (iterations, residuals)."""
-    it = C.c_int()
-    res = np.zeros(max_iter, dtype=np.float64)
-    check(ctx.lib.hbp_peps_bp_run(ctx.h, peps.h, max_iter, tol, C.byref(it),
-                                  res.ctypes.data_as(C.POINTER(C.c_double))))
-    return it.value, list(res[: it.value])
+def bp_loop(ctx, peps: Peps, max_iter: int, tol: float, progress=None):
+    """The drivers' BP loop (main.cc:121-159) -> (iterations, residuals).
+
+    With ``progress`` the loop runs one iteration per call into the library
+    and calls ``progress(iteration, residual)`` after each."""
+    if progress is None:
+        it = C.c_int()
+        res = np.zeros(max_iter, dtype=np.float64)
+        check(ctx.lib.hbp_peps_bp_run(ctx.h, peps.h, max_iter, tol, C.byref(it),
+                                      res.ctypes.data_as(C.POINTER(C.c_double))))
+        return it.value, list(res[: it.value])
+    residuals = []
+    for k in range(max_iter):
+        it, conv, r = BPIterate(ctx, peps, 1, tol, first=(k == 0), last=(k == max_iter - 1))
+        residuals += r
+        progress(len(residuals), r[-1])
+        if conv:
+            break
+    return len(residuals), residuals
+
+
+def BPIterate(ctx, peps: Peps, n: int, tol: float, first: bool, last: bool):
+    """Up to n iterations of the BP loop -> (iterations, converged, residuals)."""
+    it, conv = C.c_int(), C.c_int()
+    res = np.zeros(n, dtype=np.float64)
+    check(ctx.lib.hbp_peps_bp_iterate(ctx.h, peps.h, n, tol, int(first), int(last), C.byref(it),
+                                      res.ctypes.data_as(C.POINTER(C.c_double)), C.byref(conv)))
+    return it.value, bool(conv.value), list(res[: it.value])
 
 
 def Truncation(ctx, peps: Peps, max_dim: int, eps: float):
