@@ -32,14 +32,15 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
-    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--L", type=int, default=16)
     ap.add_argument("--chi", type=int, default=32)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--mode", choices=["fused", "plain"], default="fused",
                     help="plain: the reference's op sequence one TCI call at a time")
     ap.add_argument("--tol", type=float, default=1e-10)
-    ap.add_argument("--max-bp-iterations", type=int, default=200)
+    ap.add_argument("--max-bp-iterations", type=int, default=50,
+                    help="the kicked driver's default (apps/kicked_ising_lattice/option.h:19)")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--max-ramp", type=int, default=12,
                     help="untimed steps allowed for the bond dimension to reach chi")
@@ -140,8 +141,9 @@ def main():
         t0 = time.perf_counter()
         AbsorbTPO(ctx, tpo, peps)
         it, res = bp_loop(ctx, peps, args.max_bp_iterations, args.tol,
-                          progress=lambda k, r: print(json.dumps({"bp": k, "residual": r}),
-                                                      file=sys.stderr, flush=True))
+                          progress=lambda k, r: print(json.dumps(
+                              {"bp": k, "residual": r, "t": time.perf_counter() - t0}),
+                              file=sys.stderr, flush=True))
         dims, errs = Truncation(ctx, peps, chi, 1e-8)
         z = Measure(ctx, peps, sites, [zop] * len(sites))
         zzv = MeasureEdges(ctx, peps, edges, [zz] * len(edges))

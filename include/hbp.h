@@ -72,6 +72,11 @@ hbp_status hbp_ctx_synchronize(hbp_ctx *ctx);
 hbp_status hbp_ctx_stream(hbp_ctx *ctx, void **stream);
 /* Bytes currently held by the context's device pool. */
 hbp_status hbp_ctx_memory(hbp_ctx *ctx, size_t *in_use, size_t *peak);
+/* Return the pool's unused device memory to the device (after a job ends) and
+ * reset the peak to the bytes still in use.  No reference counterpart. */
+hbp_status hbp_ctx_trim(hbp_ctx *ctx);
+/* Free and total device memory of a device (hipMemGetInfo). */
+hbp_status hbp_device_memory(int device, size_t *free_bytes, size_t *total_bytes);
 /* Kernel timing (no reference counterpart; the reference times nothing but
  * one SVD, apps/tci_test/main.cc:39-48).  While enabled, the hot kernels are
  * bracketed by HIP events on the context stream.  profile_read synchronises,
@@ -163,6 +168,14 @@ hbp_status hbp_peps_set_mode(hbp_peps *p, int mode);
 /* Mixed precision only: the BP loop also stops once the residual is below
  * this floor (default 1e-6), since FP32 messages cannot reach FP64 tolerances. */
 hbp_status hbp_peps_set_residual_floor(hbp_peps *p, double floor);
+/* FUSED mode only (default on): fix each message's free scalar phase, trace
+ * real and positive, and compare messages modulo that phase in the residual.
+ * bp.h:321-350 compares phases too; rounding seeds them and each sweep
+ * multiplies them along non-backtracking walks (growth ~ z - 1 per sweep), so
+ * on large lattices that residual stops falling near 1e-10 and then rises to
+ * ~4/pi.  Expectation values do not depend on the phases.  No reference
+ * counterpart; PLAIN mode keeps the reference's residual. */
+hbp_status hbp_peps_set_phase_gauge(hbp_peps *p, int on);
 hbp_status hbp_peps_num_sites(const hbp_peps *p, int *n_sites, int *n_edges);
 /* Shape of site tensor s (order = degree + 1) and of message m (2 x D). */
 hbp_status hbp_peps_site_shape(const hbp_peps *p, int site, int *order, int32_t *shape);
@@ -212,6 +225,16 @@ hbp_status hbp_peps_measure_sites(hbp_ctx *ctx, hbp_peps *p, int n, const int32_
  * complex (d,d,d,d) operators indexed (a_in, b_in, a_out, b_out). */
 hbp_status hbp_peps_measure_edges(hbp_ctx *ctx, hbp_peps *p, int n, const int32_t *pairs,
                                   const double *ops, double *out);
+/* Product operators O_1 ... O_k on simple paths of neighbouring sites (k up to
+ * 63), under the BP environment of the path: an extension of tnbp::Measure
+ * (measure.h:14-300 stops at two sites; paulitotpo.h:39-153 drops weight >= 3).
+ * Path t is sites[offsets[t] .. offsets[t+1]) (site labels, consecutive sites
+ * adjacent, none repeated); ops holds one complex (d,d) operator (in, out) per
+ * path site in the same order.  out: n complex values <O_1 ... O_k>.
+ * k = 1 equals hbp_peps_measure_sites; k = 2 equals hbp_peps_measure_edges
+ * with the product operator. */
+hbp_status hbp_peps_measure_paths(hbp_ctx *ctx, hbp_peps *p, int n, const int32_t *offsets,
+                                  const int32_t *sites, const double *ops, double *out);
 
 #ifdef __cplusplus
 }

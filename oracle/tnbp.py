@@ -457,3 +457,56 @@ def measure_edges(st, edge_list, ops):
         S = tci.get_elem(tci.contract(AIA, [-2, -1], BIB, [-2, -1], []), [])
         out.append(R / S)
     return out
+
+
+def measure_paths(st, paths, ops):
+    """Product operators O_1 ... O_k on simple paths of neighbouring sites
+    under the BP environment - the extension of ``Measure`` (measure.h:14-300,
+    which stops at two sites) that hbp_peps_measure_paths implements; no
+    reference counterpart beyond weight 2.
+
+    Stated as one network: every path site's ket layer W_s (V_s with the
+    incoming message absorbed on each bond that leaves the path, as in
+    measure.h:54-70) with O_s on its physical index, the conjugate bra layer,
+    path bonds open between neighbours in both layers; <O> = N / S with S the
+    same network without operators.  Contracted in one ``np.einsum`` call,
+    independent of the transfer-matrix order the library uses.
+    """
+    I, E, edge_idx = st.I, st.E, st.EdgeIdx
+    ne = len(edge_idx)
+    out = []
+    for path, o in zip(paths, ops):
+        on_path = set()
+        for a, b in zip(path, path[1:]):
+            on_path.add(_locate(I, surrounding_bond_index(a, I), a, b)[1])
+        letters = iter("abcdefghijklmnopqrstuvwxyzABCDEFGHIJKLMNOPQRSTUVWXYZ")
+        ket_lab, bra_lab = {}, {}
+        terms_n, terms_s, ops_n, ops_s = [], [], [], []
+        for s, op in zip(path, o):
+            v = st.V[st.SiteIdx.index(s)]
+            bonds = surrounding_bond_index(s, I)
+            W = v.copy()
+            lk, lb = [], []
+            for k, m in enumerate(bonds):
+                if m in on_path:
+                    if m not in ket_lab:
+                        ket_lab[m], bra_lab[m] = next(letters), next(letters)
+                    lk.append(ket_lab[m])
+                    lb.append(bra_lab[m])
+                else:
+                    ea = edge_idx.index(m) + (ne if I[m][0] == s else 0)
+                    la = list(range(W.ndim))
+                    la[k] = -1
+                    W = tci.contract(W, la, E[ea], [-1, k], list(range(W.ndim)))
+                    c = next(letters)
+                    lk.append(c)
+                    lb.append(c)
+            p, q = next(letters), next(letters)
+            terms_n += ["".join(lk) + p, p + q, "".join(lb) + q]
+            ops_n += [W, np.asarray(op, dtype=CPLX), np.conj(v)]
+            terms_s += ["".join(lk) + p, "".join(lb) + p]
+            ops_s += [W, np.conj(v)]
+        N = np.einsum(",".join(terms_n) + "->", *ops_n, optimize="greedy")
+        S = np.einsum(",".join(terms_s) + "->", *ops_s, optimize="greedy")
+        out.append(complex(N / S))
+    return out

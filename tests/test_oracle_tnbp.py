@@ -148,3 +148,55 @@ def test_residual_is_zero_at_fixed_point_and_positive_otherwise():
     assert tnbp.belief_propagation_condition(st) < 1e-13
     st.E[0] = st.E[0] + 0.1 * np.ones_like(st.E[0])
     assert tnbp.belief_propagation_condition(st) > 1e-3
+
+
+def _sv_string(sv, ops):
+    """<psi| prod_q U_q |psi> for {qubit: U} by applying each factor."""
+    phi = StateVector(sv.n)
+    phi.psi = sv.psi.copy()
+    for q, u in ops.items():
+        phi.apply1(u, q)
+    return complex(np.vdot(sv.psi, phi.psi))
+
+
+def test_measure_paths_reduces_to_site_and_edge():
+    """Weight 1 equals Measure on a site, weight 2 equals Measure on an edge
+    with the product operator (both on a loopy 3x3 lattice, where they are
+    computed by different contractions)."""
+    edges = tnbp.bond_square_lattice(3, 3)
+    r = circuits.run_kicked(3, 3, 0.125, 0.05, 0.07, 1.0, 500, 1e-13, 8, 1e-12)
+    st = r["state"]
+    ops1 = [circuits.gate_tensor(u) for u in (X, Y, Z)]
+    for s in (0, 4, 7):
+        got = tnbp.measure_paths(st, [[s]] * 3, [[o] for o in ops1])
+        want = tnbp.measure_sites(st, [s] * 3, ops1)
+        np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-14)
+    for a, b in ((0, 1), (4, 7), (5, 4)):
+        ua, ub = Y, X
+        got = tnbp.measure_paths(st, [[a, b]], [[circuits.gate_tensor(ua), circuits.gate_tensor(ub)]])
+        want = tnbp.measure_edges(st, [(a, b)], [circuits.gate_tensor(np.kron(ub, ua))])
+        np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-14)
+    got = tnbp.measure_paths(st, [[0, 1, 2, 5, 4]], [[circuits.gate_tensor(np.eye(2))] * 5])
+    np.testing.assert_allclose(got, [1.0], atol=1e-13)
+
+
+def test_measure_paths_exact_on_trees():
+    """On trees BP is exact: strings of weight 3-6 equal the statevector."""
+    rng = np.random.default_rng(31)
+    for edges, n, paths in (
+        (tnbp.bond_oned_lattice(8), 8, [[1, 2, 3], [0, 1, 2, 3, 4, 5], [2, 3, 4, 5, 6, 7], [5, 6, 7]]),
+        (_tree_edges(), 6, [[2, 1, 3, 4, 5], [0, 1, 3], [4, 3, 1, 2]]),
+    ):
+        sv = StateVector(n)
+        tpos = []
+        for _ in range(3):
+            gates = circuits.random_layer_gates(rng, edges, n)
+            tpos.append(circuits.layer_tpo(edges, gates))
+            for u, q in gates:
+                sv.apply1(u, q[0]) if len(q) == 1 else sv.apply2(u, q[0], q[1])
+        st, _ = circuits.run_tpo_layers(edges, tpos, 200, 1e-14, 64, 1e-12)
+        for path in paths:
+            us = [(X, Y, Z)[rng.integers(3)] for _ in path]
+            got = tnbp.measure_paths(st, [path], [[circuits.gate_tensor(u) for u in us]])[0]
+            want = _sv_string(sv, dict(zip(path, us)))
+            assert abs(got - want) < 1e-11, (path, got, want)

@@ -1,0 +1,112 @@
+"""Serving scheduler logic on the host (no GPU): admission by footprint,
+first-come-first-served with bounded bypass, continuous admission while
+other jobs run, failures isolated to their job."""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+from tnbp_amd import serving
+from tnbp_amd._lib import HBP_PREC_FP64, HBP_PREC_MIXED
+
+
+class FakeCtx:
+    def trim(self):
+        pass
+
+    def close(self):
+        pass
+
+
+def _job(name, fp, dur, fail=False):
+    j = serving.EstimatorJob(edges=[(0, 1)], tpos=[], chi=1, terms=[], name=name)
+    j._fp, j._dur, j._fail = fp, dur, fail
+    return j
+
+
+def _server(budget, slots, log, max_bypass=8):
+    lock = threading.Lock()
+    live = [0]
+
+    def runner(ctx, job):
+        with lock:
+            live[0] += 1
+            log.append(("start", job.name, live[0]))
+        time.sleep(job._dur)
+        with lock:
+            live[0] -= 1
+            log.append(("end", job.name, live[0]))
+        if job._fail:
+            raise RuntimeError("job failed")
+        return {"value": job.name, "n_values": 1}
+
+    srv = serving.Server(slots=slots, memory_budget=budget, max_bypass=max_bypass, runner=runner,
+                         make_context=FakeCtx)
+    return srv
+
+
+@pytest.fixture(autouse=True)
+def fake_footprint(monkeypatch):
+    monkeypatch.setattr(serving, "footprint_bytes", lambda job: job._fp)
+
+
+def test_jobs_overlap_and_all_finish():
+    log = []
+    with _server(100, 4, log) as srv:
+        futs = [srv.submit(_job(f"j{i}", 10, 0.05)) for i in range(6)]
+        res = [f.result(timeout=10) for f in futs]
+    assert [r["value"] for r in res] == [f"j{i}" for i in range(6)]
+    assert max(n for kind, _, n in log if kind == "start") >= 2
+    st = srv.stats()
+    assert st["jobs"] == 6 and st["p99_s"] >= st["p50_s"] > 0 and st["max_concurrent"] >= 2
+
+
+def test_admission_respects_memory_budget():
+    log = []
+    with _server(100, 4, log) as srv:
+        futs = [srv.submit(_job(f"big{i}", 60, 0.03)) for i in range(3)]
+        for f in futs:
+            f.result(timeout=10)
+    # two 60-byte jobs never fit a 100-byte budget together
+    assert max(n for kind, _, n in log if kind == "start") == 1
+    with _server(100, 2, log) as srv:
+        with pytest.raises(MemoryError):
+            srv.submit(_job("huge", 101, 0.0))
+
+
+def test_small_jobs_pass_a_blocked_head_but_not_forever():
+    log = []
+    with _server(100, 4, log, max_bypass=2) as srv:
+        first = srv.submit(_job("running", 70, 0.3))
+        time.sleep(0.05)
+        head = srv.submit(_job("head", 60, 0.01))       # waits for "running"
+        smalls = [srv.submit(_job(f"s{i}", 10, 0.01)) for i in range(5)]
+        for f in [first, head] + smalls:
+            f.result(timeout=10)
+    starts = [name for kind, name, _ in log if kind == "start"]
+    # two small jobs pass the head, then the head runs before the others
+    assert starts.index("s0") < starts.index("head")
+    assert starts.index("head") < starts.index("s4")
+
+
+def test_failure_is_isolated():
+    log = []
+    with _server(100, 2, log) as srv:
+        bad = srv.submit(_job("bad", 10, 0.01, fail=True))
+        good = srv.submit(_job("good", 10, 0.01))
+        with pytest.raises(RuntimeError):
+            bad.result(timeout=10)
+        assert good.result(timeout=10)["value"] == "good"
+
+
+def test_footprint_model_scales(monkeypatch):
+    monkeypatch.undo()
+    rng = np.random.default_rng(0)
+    small = serving.random_circuit_job(rng, 4, 2, 8, 4)
+    big = serving.random_circuit_job(rng, 8, 2, 32, 4)
+    fp64 = serving.random_circuit_job(rng, 8, 2, 32, 4, precision=HBP_PREC_FP64)
+    assert serving.footprint_bytes(small) < serving.footprint_bytes(big) < serving.footprint_bytes(fp64)
+    # 16x16 chi=32 mixed kicked-Ising job: resident state ~4.3 GB plus D=64 work buffers
+    k = serving.kicked_job(16, 32, 1, 0.7, precision=HBP_PREC_MIXED)
+    assert 4e9 < serving.footprint_bytes(k) < 10e9

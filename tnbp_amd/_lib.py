@@ -35,6 +35,8 @@ SIGNATURES = {
     "hbp_ctx_synchronize": [_vp],
     "hbp_ctx_stream": [_vp, C.POINTER(_vp)],
     "hbp_ctx_memory": [_vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)],
+    "hbp_ctx_trim": [_vp],
+    "hbp_device_memory": [C.c_int, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)],
     "hbp_ctx_profile": [_vp, C.c_int],
     "hbp_ctx_profile_read": [_vp, C.c_int, C.POINTER(C.c_int64), _dp, _dp, _dp],
     "hbp_tensor_create": [_vp, C.c_int, C.c_int, _i32p, C.POINTER(_vp)],
@@ -61,6 +63,7 @@ SIGNATURES = {
     "hbp_peps_destroy": [_vp, _vp],
     "hbp_peps_set_mode": [_vp, C.c_int],
     "hbp_peps_set_residual_floor": [_vp, C.c_double],
+    "hbp_peps_set_phase_gauge": [_vp, C.c_int],
     "hbp_peps_num_sites": [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "hbp_peps_site_shape": [_vp, C.c_int, C.POINTER(C.c_int), _i32p],
     "hbp_peps_get_site": [_vp, _vp, C.c_int, _dp],
@@ -76,6 +79,7 @@ SIGNATURES = {
     "hbp_peps_truncate": [_vp, _vp, C.c_int, C.c_double, _i32p, _dp],
     "hbp_peps_measure_sites": [_vp, _vp, C.c_int, _i32p, _dp, _dp],
     "hbp_peps_measure_edges": [_vp, _vp, C.c_int, _i32p, _dp, _dp],
+    "hbp_peps_measure_paths": [_vp, _vp, C.c_int, _i32p, _i32p, _dp, _dp],
     "hbp_last_error": [],
     "hbp_version": [],
 }

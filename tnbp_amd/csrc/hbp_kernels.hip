@@ -74,7 +74,7 @@ constexpr int MP_KC = 64;
 // k-step's loads issued before the current step's MFMAs; addresses are
 // clamped in bounds and out-of-range rows/k are zeroed after the load.
 template <typename S, int NCB, int RB>
-__global__ __launch_bounds__(256) void k_mode_product(
+__global__ __launch_bounds__(256, 2) void k_mode_product(
     const typename Cplx<S>::type *__restrict__ T, const double2 *__restrict__ E,
     typename Cplx<S>::type *__restrict__ Out, int64_t L, int Din, int Dout, int64_t Q,
     int e_trans, S scale, double *__restrict__ norm_partials) {
@@ -205,7 +205,7 @@ constexpr int MSG_QS = 32;   // rows per LDS stage
 constexpr int MSG_IB = 64;   // output rows (i) per block (4 waves x 16)
 
 template <typename S, int NCB>
-__global__ __launch_bounds__(256) void k_message_partials(
+__global__ __launch_bounds__(256, 2) void k_message_partials(
     const typename Cplx<S>::type *__restrict__ T, const typename Cplx<S>::type *__restrict__ V,
     double2 *__restrict__ partials, int64_t L, int D1, int D2, int64_t Q, int64_t chunk,
     int64_t ldT_R, int64_t ldV_R, int conjV, const double2 *__restrict__ wL,

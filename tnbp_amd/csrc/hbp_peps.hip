@@ -54,13 +54,22 @@ struct MsgJob {
   int D;
 };
 
+// unit complex c with c * tr = |tr| (1 when the trace vanishes)
+__device__ inline double2 phase_of_trace(double re, double im) {
+  const double a = sqrt(re * re + im * im);
+  return a > 0.0 ? make_double2(re / a, -im / a) : make_double2(1.0, 0.0);
+}
+
 // normalise every raw message (FP64) and, when old != null, the Frobenius
-// distance between it and the normalised stored message (bp.h:321-350)
+// distance between it and the normalised stored message (bp.h:321-350).
+// phase_gauge != 0 also fixes each message's free scalar phase (trace real
+// and positive) on both sides of the distance.
 __global__ __launch_bounds__(256) void k_finalize_messages(const double2 *__restrict__ raw,
                                                             double2 *__restrict__ out,
                                                             const double2 *__restrict__ old,
                                                             const MsgJob *__restrict__ jobs,
-                                                            double *__restrict__ dist) {
+                                                            double *__restrict__ dist,
+                                                            int phase_gauge) {
   __shared__ double sh[8];
   const MsgJob j = jobs[blockIdx.x];
   const int n = j.D * j.D;
@@ -75,16 +84,40 @@ __global__ __launch_bounds__(256) void k_finalize_messages(const double2 *__rest
   }
   s = bsum(s, sh);
   if (old) so = bsum(so, sh);
+  double2 c = make_double2(1.0, 0.0), co = make_double2(1.0, 0.0);
+  if (phase_gauge) {
+    double tr = 0.0, ti = 0.0, tor = 0.0, toi = 0.0;
+    for (int i = threadIdx.x; i < j.D; i += blockDim.x) {
+      const double2 v = raw[j.raw_off + (int64_t)i * (j.D + 1)];
+      tr += v.x;
+      ti += v.y;
+      if (old) {
+        const double2 w = old[j.old_off + (int64_t)i * (j.D + 1)];
+        tor += w.x;
+        toi += w.y;
+      }
+    }
+    tr = bsum(tr, sh);
+    ti = bsum(ti, sh);
+    c = phase_of_trace(tr, ti);
+    if (old) {
+      tor = bsum(tor, sh);
+      toi = bsum(toi, sh);
+      co = phase_of_trace(tor, toi);
+    }
+  }
   const double inv = s > 0.0 ? 1.0 / sqrt(s) : 1.0;
   const double invo = so > 0.0 ? 1.0 / sqrt(so) : 1.0;
+  c = make_double2(c.x * inv, c.y * inv);
+  co = make_double2(co.x * invo, co.y * invo);
   double d = 0.0;
   for (int e = threadIdx.x; e < n; e += blockDim.x) {
     const double2 v = raw[j.raw_off + e];
-    const double2 m = make_double2(v.x * inv, v.y * inv);
+    const double2 m = make_double2(v.x * c.x - v.y * c.y, v.x * c.y + v.y * c.x);
     out[j.new_off + e] = m;
     if (old) {
       const double2 w = old[j.old_off + e];
-      const double dx = m.x - w.x * invo, dy = m.y - w.y * invo;
+      const double dx = m.x - (w.x * co.x - w.y * co.y), dy = m.y - (w.x * co.y + w.y * co.x);
       d += dx * dx + dy * dy;
     }
   }
@@ -374,6 +407,57 @@ __global__ __launch_bounds__(256) void k_edge_ratio(const double2 *__restrict__ 
   }
 }
 
+// Path transfer matrix X (n complex): X /= ||X||_F, lg = (init ? 0 : lg) + log ||X||_F.
+__global__ __launch_bounds__(256) void k_chain_norm(double2 *__restrict__ X, int n,
+                                                     double *__restrict__ lg, int init) {
+  __shared__ double sh[8];
+  double s = 0.0;
+  for (int e = threadIdx.x; e < n; e += blockDim.x) s += X[e].x * X[e].x + X[e].y * X[e].y;
+  s = bsum(s, sh);
+  const double nrm = sqrt(s);
+  const double inv = nrm > 0.0 ? 1.0 / nrm : 1.0;
+  for (int e = threadIdx.x; e < n; e += blockDim.x) X[e] = make_double2(X[e].x * inv, X[e].y * inv);
+  if (threadIdx.x == 0) *lg = (init ? 0.0 : *lg) + (nrm > 0.0 ? log(nrm) : 0.0);
+}
+
+// Closing a path: val = sum_ij X[i, j] M[i, j] (X = null: trace of the d x d M)
+__global__ __launch_bounds__(256) void k_chain_close(const double2 *__restrict__ X,
+                                                      const double2 *__restrict__ M, int n, int d,
+                                                      double2 *__restrict__ val,
+                                                      double *__restrict__ lg, int init) {
+  __shared__ double sh[8];
+  double re = 0.0, im = 0.0;
+  if (X) {
+    for (int e = threadIdx.x; e < n; e += blockDim.x) {
+      const double2 a = X[e], b = M[e];
+      re += a.x * b.x - a.y * b.y;
+      im += a.x * b.y + a.y * b.x;
+    }
+  } else {
+    for (int q = threadIdx.x; q < d; q += blockDim.x) {
+      re += M[q + d * q].x;
+      im += M[q + d * q].y;
+    }
+  }
+  re = bsum(re, sh);
+  im = bsum(im, sh);
+  if (threadIdx.x == 0) {
+    *val = make_double2(re, im);
+    if (init) *lg = 0.0;
+  }
+}
+
+// out[t] = (val[t] / val[den[t]]) exp(lg[t] - lg[den[t]])
+__global__ void k_chain_ratio(const double2 *__restrict__ val, const double *__restrict__ lg,
+                              const int *__restrict__ den, int n, double2 *__restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double2 a = val[t], b = val[den[t]];
+  const double sc = exp(lg[t] - lg[den[t]]);
+  const double dd = b.x * b.x + b.y * b.y;
+  out[t] = make_double2(sc * (a.x * b.x + a.y * b.y) / dd, sc * (a.y * b.x - a.x * b.y) / dd);
+}
+
 inline int gridn(size_t n) {
   size_t g = (n + 255) / 256;
   return (int)std::max<size_t>(1, std::min<size_t>(g, 8192));
@@ -398,6 +482,17 @@ struct Peps {
   // message m is exactly diagonal (after init, AbsorbTPO's E (x) I and
   // Truncation's diag(L)); a BP sweep produces dense messages
   std::vector<char> E_diag;
+  // Lazy absorption: AbsorbTPO keeps V_s and its operator O_s; the absorbed
+  // tensor is materialised into abs_buf when a kernel needs it, so only the
+  // chi-sized state stays resident (the absorbed lattice is ~D^4 d per site).
+  bool lazy = true;
+  std::vector<char> pending;
+  std::vector<void *> pend_O;
+  std::vector<size_t> pend_O_bytes;
+  std::vector<AbsorbArgs> pend_args;
+  void *abs_buf = nullptr;
+  size_t abs_bytes = 0;
+  int abs_site = -1;
   // work buffers, each as large as the largest site tensor
   void *buf[3] = {nullptr, nullptr, nullptr};
   size_t buf_bytes = 0;
@@ -405,6 +500,13 @@ struct Peps {
   // mixed precision: BP stops once the residual is below this (FP32 messages
   // cannot reach the FP64 tolerances), or below the caller's tolerance
   double mixed_floor = 1.0e-6;
+  // Fused mode fixes the free scalar phase of every message (trace real and
+  // positive).  The residual of bp.h:321-350 compares normalised messages
+  // including that phase, which rounding seeds and every sweep multiplies along
+  // non-backtracking walks (growth ~ z - 1 per sweep on a square lattice), so
+  // without the fix the residual stalls near 4/pi once it is below ~1e-10 on
+  // large lattices.  Expectation values and projectors do not depend on it.
+  bool phase_gauge = true;
 
   int elem_bytes() const { return precision == HBP_PREC_FP64 ? 16 : 8; }
   int site_dtype() const { return precision == HBP_PREC_FP64 ? HBP_C128 : HBP_C64; }
@@ -450,6 +552,65 @@ static void ensure_work(Ctx *ctx, Peps *p, size_t bytes) {
   p->buf_bytes = bytes;
 }
 
+// V'_s = O_s applied to V_s (apply.h:52-66) into dst
+static void materialize(Ctx *ctx, Peps *p, int s, void *dst) {
+  const size_t n = p->site_size(s);
+  if (p->precision == HBP_PREC_FP64)
+    hipLaunchKernelGGL(k_absorb_site<double2>, dim3(gridn(n)), dim3(256), 0, ctx->stream,
+                       (const double2 *)p->V[s], (const double2 *)p->pend_O[s], (double2 *)dst, n,
+                       p->pend_args[s]);
+  else
+    hipLaunchKernelGGL(k_absorb_site<float2>, dim3(gridn(n)), dim3(256), 0, ctx->stream,
+                       (const float2 *)p->V[s], (const double2 *)p->pend_O[s], (float2 *)dst, n,
+                       p->pend_args[s]);
+  HBP_HIP(hipGetLastError());
+}
+
+// The site tensor as the algorithms see it: V_s itself, or its absorbed form
+// materialised into abs_buf (reused while the same site is asked for again).
+static const void *site_data(Ctx *ctx, Peps *p, int s) {
+  if (!p->pending[s]) return p->V[s];
+  if (p->abs_site == s) return p->abs_buf;
+  const size_t bytes = p->site_size(s) * p->elem_bytes();
+  if (bytes > p->abs_bytes) {
+    if (p->abs_buf) ctx->release(p->abs_buf, p->abs_bytes);
+    p->abs_buf = ctx->alloc(bytes);
+    p->abs_bytes = bytes;
+  }
+  materialize(ctx, p, s, p->abs_buf);
+  p->abs_site = s;
+  return p->abs_buf;
+}
+
+// Store the absorbed form of site s as its tensor and drop the operator.
+static void make_resident(Ctx *ctx, Peps *p, int s) {
+  if (!p->pending[s]) return;
+  const size_t bytes = p->site_size(s) * p->elem_bytes();
+  void *nv = ctx->alloc(bytes);
+  materialize(ctx, p, s, nv);
+  ctx->release(p->V[s], p->V_bytes[s]);
+  p->V[s] = nv;
+  p->V_bytes[s] = bytes;
+  ctx->release(p->pend_O[s], p->pend_O_bytes[s]);
+  p->pend_O[s] = nullptr;
+  p->pending[s] = 0;
+  if (p->abs_site == s) p->abs_site = -1;
+}
+
+// Replace site s's tensor (already allocated at the current shape) and drop
+// any pending operator.
+static void set_resident(Ctx *ctx, Peps *p, int s, void *nv, size_t bytes) {
+  ctx->release(p->V[s], p->V_bytes[s]);
+  p->V[s] = nv;
+  p->V_bytes[s] = bytes;
+  if (p->pending[s]) {
+    ctx->release(p->pend_O[s], p->pend_O_bytes[s]);
+    p->pend_O[s] = nullptr;
+    p->pending[s] = 0;
+  }
+  if (p->abs_site == s) p->abs_site = -1;
+}
+
 // Row weights of a mode product with diagonal messages: w[i] = prod_j
 // diag(E_in(j))[c_j] over the bonds j in [j0, j1) of site s, i split
 // column-major over their dims, times a trailing unit factor of `unit`.
@@ -491,7 +652,8 @@ static void weighted_message(Ctx *ctx, Peps *p, int s, int k, double2 *M) {
   double2 *parts = (double2 *)ctx->get_scratch((size_t)nch * D * D * 16 + 64);
   const int pr = ctx->prof_begin(HBP_PROF_MESSAGE, 8.0 * L * R * D * D,
                                  2.0 * sizeof(S) * 2.0 * L * R * D);
-  const int n = message_partials<S>(ctx->stream, p->V[s], p->V[s], parts, L, D, D, R, L * D,
+  const void *Vs = site_data(ctx, p, s);
+  const int n = message_partials<S>(ctx->stream, Vs, Vs, parts, L, D, D, R, L * D,
                                     L * D, nch, 1, wL, wR);
   launch_reduce_partials(ctx->stream, parts, n, D * D, M);
   ctx->prof_end(pr);
@@ -543,7 +705,7 @@ template <typename S>
 static const void *absorb_env(Ctx *ctx, Peps *p, int s, int skip, int32_t *sh) {
   const int order = p->z(s) + 1;
   p->shape(s, sh);
-  const void *cur = p->V[s];
+  const void *cur = site_data(ctx, p, s);
   int which = 0;
   for (int k = 0; k < p->z(s); ++k) {
     if (k == skip) continue;
@@ -573,7 +735,7 @@ static void sweep_raw(Ctx *ctx, Peps *p, double2 *raw, const std::vector<int64_t
     if (z <= 2) {
       for (int k = 0; k < z; ++k) {
         const void *T = absorb_env<S>(ctx, p, s, k, sh);
-        message_raw<S>(ctx, T, p->V[s], sh, order, k, raw + raw_off[p->out_msg(s, k)]);
+        message_raw<S>(ctx, T, site_data(ctx, p, s), sh, order, k, raw + raw_off[p->out_msg(s, k)]);
       }
       continue;
     }
@@ -586,7 +748,8 @@ static void sweep_raw(Ctx *ctx, Peps *p, double2 *raw, const std::vector<int64_t
       const int a0 = half == 0 ? 0 : h, a1 = half == 0 ? h : z;   // absorbed first
       const int t0 = half == 0 ? h : 0, t1 = half == 0 ? z : h;   // targets
       // A = V x_{[a0, a1)} into buf[0] (ping-pong with buf[1] when needed)
-      const void *cur = p->V[s];
+      const void *Vs = site_data(ctx, p, s);
+      const void *cur = Vs;
       int nprod = a1 - a0, w = (nprod % 2 == 1) ? 0 : 1;
       for (int j = a0; j < a1; ++j) {
         void *dst = p->buf[w];
@@ -605,7 +768,7 @@ static void sweep_raw(Ctx *ctx, Peps *p, double2 *raw, const std::vector<int64_t
           T = dst;
           u = u == 1 ? 2 : 1;
         }
-        message_raw<S>(ctx, T, p->V[s], sh, order, k, raw + raw_off[p->out_msg(s, k)]);
+        message_raw<S>(ctx, T, Vs, sh, order, k, raw + raw_off[p->out_msg(s, k)]);
       }
     }
   }
@@ -627,7 +790,8 @@ static void sweep_raw_plain(Ctx *ctx, Peps *p, double2 *raw, const std::vector<i
     p->shape(s, sh);
     const size_t n = p->site_size(s), bytes = n * p->elem_bytes();
     for (int k = 0; k < z; ++k) {
-      HBP_HIP(hipMemcpyAsync(p->buf[0], p->V[s], bytes, hipMemcpyDeviceToDevice, ctx->stream));
+      HBP_HIP(hipMemcpyAsync(p->buf[0], site_data(ctx, p, s), bytes, hipMemcpyDeviceToDevice,
+                             ctx->stream));
       int cur = 0;
       for (int j = 0; j < z; ++j) {
         if (j != k) {
@@ -638,7 +802,8 @@ static void sweep_raw_plain(Ctx *ctx, Peps *p, double2 *raw, const std::vector<i
         device_norm2(ctx, p->buf[cur], n, p->site_dtype(), dv);
         launch_scale(ctx->stream, p->buf[cur], n, p->site_dtype(), 1.0, 0.0, dv);
       }
-      HBP_HIP(hipMemcpyAsync(p->buf[2], p->V[s], bytes, hipMemcpyDeviceToDevice, ctx->stream));
+      HBP_HIP(hipMemcpyAsync(p->buf[2], site_data(ctx, p, s), bytes, hipMemcpyDeviceToDevice,
+                             ctx->stream));
       launch_conj(ctx->stream, p->buf[2], n, p->site_dtype());
       int64_t L = 1, R = 1;
       for (int d = 0; d < k; ++d) L *= sh[d];
@@ -681,7 +846,8 @@ static void sweep(Ctx *ctx, Peps *p, double2 *newE, bool with_residual, double *
   }
   double *dist = (double *)ctx->alloc(nm * sizeof(double));
   hipLaunchKernelGGL(k_finalize_messages, dim3(nm), dim3(256), 0, ctx->stream, raw, newE,
-                     with_residual ? p->E : nullptr, dj, dist);
+                     with_residual ? p->E : nullptr, dj, dist,
+                     p->mode == HBP_MODE_FUSED && p->phase_gauge ? 1 : 0);
   HBP_HIP(hipGetLastError());
   if (with_residual) {
     hipLaunchKernelGGL(k_residual, dim3(1), dim3(256), 0, ctx->stream, dist, nm,
@@ -711,7 +877,7 @@ static void apply_projectors(Ctx *ctx, Peps *p, const double2 *proj,
       maxn = n;
     }
     ensure_work(ctx, p, std::max(p->buf_bytes, maxn * p->elem_bytes()));
-    const void *cur = p->V[s];
+    const void *cur = site_data(ctx, p, s);
     int which = 0;
     for (int k = 0; k < z; ++k) {
       const int m = p->bonds[s][k];
@@ -736,9 +902,7 @@ static void apply_projectors(Ctx *ctx, Peps *p, const double2 *proj,
     device_norm2(ctx, nv, n, p->site_dtype(), dv);
     launch_scale(ctx->stream, nv, n, p->site_dtype(), 1.0, 0.0, dv);
     ctx->release(dv, sizeof(double));
-    ctx->release(p->V[s], p->V_bytes[s]);
-    p->V[s] = nv;
-    p->V_bytes[s] = bytes;
+    set_resident(ctx, p, s, nv, bytes);
   }
 }
 
@@ -882,7 +1046,7 @@ static void measure_sites_impl(Ctx *ctx, Peps *p, int n, const int *addr, const 
     }
     const void *W = absorb_env<S>(ctx, p, s, -1, sh);
     // rho[p, p'] = sum_b W[b, p] conj V[b, p']: message over the physical axis
-    message_raw<S>(ctx, W, p->V[s], sh, p->z(s) + 1, p->z(s), rho + u * maxd * maxd);
+    message_raw<S>(ctx, W, site_data(ctx, p, s), sh, p->z(s) + 1, p->z(s), rho + u * maxd * maxd);
   }
   std::vector<int> idx(n);
   for (int t = 0; t < n; ++t) {
@@ -913,7 +1077,8 @@ template <typename S>
 static void edge_env(Ctx *ctx, Peps *p, int s, int kt, double2 *out) {
   int32_t sh[HBP_MAX_ORDER];
   const bool diag = diag_env(p, s, kt);
-  const void *A = diag ? p->V[s] : absorb_env<S>(ctx, p, s, kt, sh);
+  const void *Vs = site_data(ctx, p, s);
+  const void *A = diag ? Vs : absorb_env<S>(ctx, p, s, kt, sh);
   if (diag) p->shape(s, sh);
   const int z = p->z(s), d = p->phys[s];
   // diagonal environment: the absorbed bonds become row weights of A
@@ -930,7 +1095,7 @@ static void edge_env(Ctx *ctx, Peps *p, int s, int kt, double2 *out) {
   double2 *parts = (double2 *)ctx->get_scratch((size_t)nch * Dd * Dd * 16 + 64);
   const int pr = ctx->prof_begin(HBP_PROF_MESSAGE, 8.0 * L * R * Dd * Dd,
                                  2.0 * sizeof(S) * 2.0 * L * R * Dd);
-  const int nc = message_partials<S>(ctx->stream, A, p->V[s], parts, L, Dd, Dd, R, L * D, L * D,
+  const int nc = message_partials<S>(ctx->stream, A, Vs, parts, L, Dd, Dd, R, L * D, L * D,
                                      nch, 1, wL, wR, D, slab);
   launch_reduce_partials(ctx->stream, parts, nc, Dd * Dd, out);
   ctx->prof_end(pr);
@@ -993,6 +1158,140 @@ static void measure_edges_impl(Ctx *ctx, Peps *p, int n, const int *pa, const in
   ctx->release(dops, ob);
   ctx->release(BB, std::max<int64_t>(tot, 1) * 16);
   ctx->release(AA, std::max<int64_t>(tot, 1) * 16);
+}
+
+// Product operator O_1 ... O_k on a simple path s_1 - ... - s_k (an extension of
+// measure.h, whose operators stop at two neighbouring sites).  The BP
+// environment of the path is the incoming message on every bond that leaves
+// it; along the path a transfer matrix X[b, b'] (ket, bra) carries the
+// contraction from site to site:
+//   X_1 = M_{b_1}(V_1 env O_1),  X_i = M_{b_i}((V_i env) x_{a_i} X_{i-1} x_p O_i)
+//   value = sum X_{k-1} o M_{a_k}((V_k env) x_p O_k)
+// where M_b(T)[i, j] = sum over all other axes of T[.. i ..] conj V[.. j ..].
+// X is normalised after every site, the logarithm of the scale kept beside it.
+// ops == null runs the same path with identities (the normalisation S).
+template <typename S>
+static void chain_value(Ctx *ctx, Peps *p, const std::vector<int> &path,
+                        const std::vector<int> &kin, const std::vector<int> &kout,
+                        const double2 *dops, double2 *X, double2 *M, double2 *val, double *lg) {
+  const int k = (int)path.size();
+  int32_t sh[HBP_MAX_ORDER];
+  for (int i = 0; i < k; ++i) {
+    const int s = path[i], z = p->z(s), order = z + 1, d = p->phys[s];
+    p->shape(s, sh);
+    const void *Vs = site_data(ctx, p, s);
+    const void *cur = Vs;
+    int which = 0;
+    for (int b = 0; b < z; ++b) {
+      if (b == kin[i] || b == kout[i]) continue;
+      mode_prod<S>(ctx, cur, p->E + p->E_off[p->in_msg(s, b)], p->buf[which], sh, order, b,
+                   sh[b], 0);
+      cur = p->buf[which];
+      which ^= 1;
+    }
+    if (kin[i] >= 0 && kout[i] >= 0) {  // the last site meets X in k_chain_close
+      mode_prod<S>(ctx, cur, X, p->buf[which], sh, order, kin[i], sh[kin[i]], 0);
+      cur = p->buf[which];
+      which ^= 1;
+    }
+    if (dops) {
+      mode_prod<S>(ctx, cur, dops + (size_t)i * d * d, p->buf[which], sh, order, z, d, 0);
+      cur = p->buf[which];
+      which ^= 1;
+    }
+    const int t = kout[i] >= 0 ? kout[i] : (kin[i] >= 0 ? kin[i] : z);
+    message_raw<S>(ctx, cur, Vs, sh, order, t, M);
+    const int n = sh[t] * sh[t];
+    if (kout[i] >= 0) {
+      HBP_HIP(hipMemcpyAsync(X, M, (size_t)n * 16, hipMemcpyDeviceToDevice, ctx->stream));
+      hipLaunchKernelGGL(k_chain_norm, dim3(1), dim3(256), 0, ctx->stream, X, n, lg, i == 0);
+    } else {
+      hipLaunchKernelGGL(k_chain_close, dim3(1), dim3(256), 0, ctx->stream,
+                         kin[i] >= 0 ? (const double2 *)X : nullptr, (const double2 *)M, n,
+                         sh[t], val, lg, i == 0);
+    }
+    HBP_HIP(hipGetLastError());
+  }
+}
+
+// bond index at site a of the edge {a, b}, -1 when they are not neighbours
+static int bond_to(const Peps *p, int a, int b) {
+  for (int k = 0; k < p->z(a); ++k) {
+    const int e = p->bonds[a][k];
+    if ((p->ea[e] == a && p->eb[e] == b) || (p->ea[e] == b && p->eb[e] == a)) return k;
+  }
+  return -1;
+}
+
+template <typename S>
+static void measure_paths_impl(Ctx *ctx, Peps *p, int n, const int *offsets, const int *addr,
+                               const double *ops, double *out) {
+  std::vector<std::vector<int>> paths(n), kins(n), kouts(n);
+  std::map<std::vector<int>, int> den_slot;
+  std::vector<int> den(n);
+  int d = 0, maxD = 1;
+  size_t maxb = 16;
+  for (int t = 0; t < n; ++t) {
+    const int k = offsets[t + 1] - offsets[t];
+    HBP_REQUIRE(k >= 1, HBP_ERR_INVALID_ARGUMENT, "measure: empty operator path");
+    std::vector<int> &pt = paths[t];
+    pt.assign(addr + offsets[t], addr + offsets[t + 1]);
+    std::vector<int> sorted = pt;
+    std::sort(sorted.begin(), sorted.end());
+    HBP_REQUIRE(std::adjacent_find(sorted.begin(), sorted.end()) == sorted.end(),
+                HBP_ERR_INVALID_ARGUMENT, "measure: a path visits a site twice");
+    kins[t].assign(k, -1);
+    kouts[t].assign(k, -1);
+    for (int i = 0; i < k; ++i) {
+      const int s = pt[i];
+      HBP_REQUIRE(d == 0 || p->phys[s] == d, HBP_ERR_UNSUPPORTED, "measure: mixed physical dims");
+      d = p->phys[s];
+      maxb = std::max(maxb, p->site_size(s) * p->elem_bytes());
+      for (int b = 0; b < p->z(s); ++b) maxD = std::max(maxD, p->bond_dim[p->bonds[s][b]]);
+      if (i + 1 < k) {
+        kouts[t][i] = bond_to(p, s, pt[i + 1]);
+        HBP_REQUIRE(kouts[t][i] >= 0, HBP_ERR_INVALID_ARGUMENT,
+                    "measure: consecutive path sites are not neighbours");
+      }
+      if (i > 0) kins[t][i] = bond_to(p, s, pt[i - 1]);
+    }
+    auto it = den_slot.find(pt);
+    if (it == den_slot.end()) it = den_slot.emplace(pt, n + (int)den_slot.size()).first;
+    den[t] = it->second;
+  }
+  maxD = std::max(maxD, d);
+  ensure_work(ctx, p, maxb);
+  const int nslots = n + (int)den_slot.size();
+  size_t nop = (size_t)(offsets[n] - offsets[0]) * d * d;
+  double2 *dops = (double2 *)ctx->alloc(std::max<size_t>(nop, 1) * 16);
+  ctx->upload(dops, ops, nop * 16);
+  double2 *X = (double2 *)ctx->alloc((size_t)maxD * maxD * 16);
+  double2 *M = (double2 *)ctx->alloc((size_t)maxD * maxD * 16);
+  double2 *val = (double2 *)ctx->alloc((size_t)nslots * 16);
+  double *lg = (double *)ctx->alloc((size_t)nslots * 8);
+  for (const auto &kv : den_slot) {
+    const int t = (int)(std::find(paths.begin(), paths.end(), kv.first) - paths.begin());
+    chain_value<S>(ctx, p, kv.first, kins[t], kouts[t], nullptr, X, M, val + kv.second,
+                   lg + kv.second);
+  }
+  for (int t = 0; t < n; ++t)
+    chain_value<S>(ctx, p, paths[t], kins[t], kouts[t],
+                   dops + (size_t)(offsets[t] - offsets[0]) * d * d, X, M, val + t, lg + t);
+  int *dden = (int *)ctx->alloc(n * sizeof(int));
+  double2 *dout = (double2 *)ctx->alloc((size_t)n * 16);
+  ctx->upload(dden, den.data(), n * sizeof(int));
+  hipLaunchKernelGGL(k_chain_ratio, dim3((n + 127) / 128), dim3(128), 0, ctx->stream, val, lg,
+                     dden, n, dout);
+  HBP_HIP(hipGetLastError());
+  HBP_HIP(hipMemcpyAsync(out, dout, (size_t)n * 16, hipMemcpyDeviceToHost, ctx->stream));
+  ctx->sync();
+  ctx->release(dout, (size_t)n * 16);
+  ctx->release(dden, n * sizeof(int));
+  ctx->release(lg, (size_t)nslots * 8);
+  ctx->release(val, (size_t)nslots * 16);
+  ctx->release(M, (size_t)maxD * maxD * 16);
+  ctx->release(X, (size_t)maxD * maxD * 16);
+  ctx->release(dops, std::max<size_t>(nop, 1) * 16);
 }
 
 }  // namespace hbp
@@ -1062,6 +1361,10 @@ hbp_status hbp_peps_create(hbp_ctx *ctx, int n_edges, const int32_t *edges,
   // |0>: bonds of dimension 1, V[0] = 1 (init.h:57-60)
   p->V.resize(p->n_sites);
   p->V_bytes.resize(p->n_sites);
+  p->pending.assign(p->n_sites, 0);
+  p->pend_O.assign(p->n_sites, nullptr);
+  p->pend_O_bytes.assign(p->n_sites, 0);
+  p->pend_args.resize(p->n_sites);
   for (int s = 0; s < p->n_sites; ++s) {
     const size_t bytes = (size_t)p->phys[s] * p->elem_bytes();
     p->V[s] = ctx->alloc(bytes);
@@ -1081,7 +1384,11 @@ hbp_status hbp_peps_create(hbp_ctx *ctx, int n_edges, const int32_t *edges,
 hbp_status hbp_peps_destroy(hbp_ctx *ctx, hbp_peps *p) {
   API_BEGIN
   if (!p) return HBP_OK;
-  for (int s = 0; s < p->n_sites; ++s) ctx->release(p->V[s], p->V_bytes[s]);
+  for (int s = 0; s < p->n_sites; ++s) {
+    ctx->release(p->V[s], p->V_bytes[s]);
+    if (p->pend_O[s]) ctx->release(p->pend_O[s], p->pend_O_bytes[s]);
+  }
+  if (p->abs_buf) ctx->release(p->abs_buf, p->abs_bytes);
   ctx->release(p->E, p->E_bytes);
   for (int i = 0; i < 3; ++i)
     if (p->buf[i]) ctx->release(p->buf[i], p->buf_bytes);
@@ -1110,7 +1417,7 @@ hbp_status hbp_peps_get_site(hbp_ctx *ctx, hbp_peps *p, int site, double *host) 
   HBP_REQUIRE(site >= 0 && site < p->n_sites, HBP_ERR_OUT_OF_RANGE, "site out of range");
   const size_t n = p->site_size(site);
   double2 *tmp = (double2 *)ctx->alloc(n * 16);
-  launch_convert(ctx->stream, p->V[site], p->site_dtype(), tmp, HBP_C128, n);
+  launch_convert(ctx->stream, site_data(ctx, p, site), p->site_dtype(), tmp, HBP_C128, n);
   HBP_HIP(hipMemcpyAsync(host, tmp, n * 16, hipMemcpyDeviceToHost, ctx->stream));
   ctx->sync();
   ctx->release(tmp, n * 16);
@@ -1129,7 +1436,10 @@ hbp_status hbp_peps_set_site(hbp_ctx *ctx, hbp_peps *p, int site, int order,
   const size_t n = p->site_size(site);
   double2 *tmp = (double2 *)ctx->alloc(n * 16);
   ctx->upload(tmp, host, n * 16);
-  launch_convert(ctx->stream, tmp, HBP_C128, p->V[site], p->site_dtype(), n);
+  const size_t bytes = n * p->elem_bytes();
+  void *nv = ctx->alloc(bytes);
+  launch_convert(ctx->stream, tmp, HBP_C128, nv, p->site_dtype(), n);
+  set_resident(ctx, p, site, nv, bytes);
   ctx->sync();
   ctx->release(tmp, n * 16);
   API_END
@@ -1185,6 +1495,8 @@ hbp_status hbp_peps_absorb_tpo(hbp_ctx *ctx, hbp_peps *p, const double *const *o
       t[m] = sh[k];
     }
   }
+  // a second absorption before BP composes with the first: store the first
+  for (int s = 0; s < ns; ++s) make_resident(ctx, p, s);
   for (int s = 0; s < ns; ++s) {
     const int32_t *osh = tpo_shapes + (size_t)s * HBP_MAX_ORDER;
     const int z = p->z(s);
@@ -1214,6 +1526,15 @@ hbp_status hbp_peps_absorb_tpo(hbp_ctx *ctx, hbp_peps *p, const double *const *o
     const size_t n = (size_t)ns_ * a.dout;
     double2 *dO = (double2 *)ctx->alloc(on * 16);
     HBP_HIP(hipMemcpyAsync(dO, ops[s], on * 16, hipMemcpyHostToDevice, ctx->stream));
+    p->phys[s] = a.dout;
+    if (p->lazy && p->mode == HBP_MODE_FUSED) {
+      p->pending[s] = 1;
+      p->pend_O[s] = dO;
+      p->pend_O_bytes[s] = on * 16;
+      p->pend_args[s] = a;
+      if (p->abs_site == s) p->abs_site = -1;
+      continue;
+    }
     const size_t bytes = n * p->elem_bytes();
     void *nv = ctx->alloc(bytes);
     if (p->precision == HBP_PREC_FP64)
@@ -1227,7 +1548,6 @@ hbp_status hbp_peps_absorb_tpo(hbp_ctx *ctx, hbp_peps *p, const double *const *o
     ctx->release(p->V[s], p->V_bytes[s]);
     p->V[s] = nv;
     p->V_bytes[s] = bytes;
-    p->phys[s] = a.dout;
   }
   std::vector<int> old = p->bond_dim;
   std::vector<int64_t> old_off = p->E_off;
@@ -1255,6 +1575,12 @@ hbp_status hbp_peps_set_mode(hbp_peps *p, int mode) {
   HBP_REQUIRE(mode == HBP_MODE_FUSED || mode == HBP_MODE_PLAIN, HBP_ERR_INVALID_ARGUMENT,
               "peps: unknown mode");
   p->mode = mode;
+  API_END
+}
+
+hbp_status hbp_peps_set_phase_gauge(hbp_peps *p, int on) {
+  API_BEGIN
+  p->phase_gauge = on != 0;
   API_END
 }
 
@@ -1398,6 +1724,23 @@ hbp_status hbp_peps_measure_edges(hbp_ctx *ctx, hbp_peps *p, int n, const int32_
     measure_edges_impl<double>(ctx, p, n, a.data(), b.data(), ops, out);
   else
     measure_edges_impl<float>(ctx, p, n, a.data(), b.data(), ops, out);
+  API_END
+}
+
+hbp_status hbp_peps_measure_paths(hbp_ctx *ctx, hbp_peps *p, int n, const int32_t *offsets,
+                                  const int32_t *sites, const double *ops, double *out) {
+  API_BEGIN
+  if (n <= 0) return HBP_OK;
+  HBP_REQUIRE(offsets[0] == 0, HBP_ERR_INVALID_ARGUMENT, "measure_paths: offsets[0] != 0");
+  for (int t = 0; t < n; ++t)
+    HBP_REQUIRE(offsets[t + 1] > offsets[t] && offsets[t + 1] - offsets[t] < 64,
+                HBP_ERR_INVALID_ARGUMENT, "measure_paths: path length outside 1..63");
+  std::vector<int> addr(offsets[n]);
+  for (int i = 0; i < offsets[n]; ++i) addr[i] = site_addr(p, sites[i]);
+  if (p->precision == HBP_PREC_FP64)
+    measure_paths_impl<double>(ctx, p, n, offsets, addr.data(), ops, out);
+  else
+    measure_paths_impl<float>(ctx, p, n, offsets, addr.data(), ops, out);
   API_END
 }
 

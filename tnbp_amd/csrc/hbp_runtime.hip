@@ -538,6 +538,33 @@ hbp_status hbp_ctx_memory(hbp_ctx *ctx, size_t *in_use, size_t *peak) {
   HBP_API_END
 }
 
+hbp_status hbp_ctx_trim(hbp_ctx *ctx) {
+  HBP_API_BEGIN
+  if (ctx->scratch) {
+    ctx->release(ctx->scratch, ctx->scratch_bytes);
+    ctx->scratch = nullptr;
+    ctx->scratch_bytes = 0;
+  }
+  ctx->sync();
+  HBP_HIP(hipMemPoolTrimTo(ctx->pool, 0));
+  ctx->peak = ctx->in_use;
+  HBP_API_END
+}
+
+hbp_status hbp_device_memory(int device, size_t *free_bytes, size_t *total_bytes) {
+  HBP_API_BEGIN
+  int cur = 0;
+  HBP_HIP(hipGetDevice(&cur));
+  HBP_HIP(hipSetDevice(device));
+  size_t f = 0, t = 0;
+  const hipError_t e = hipMemGetInfo(&f, &t);
+  HBP_HIP(hipSetDevice(cur));
+  HBP_HIP(e);
+  if (free_bytes) *free_bytes = f;
+  if (total_bytes) *total_bytes = t;
+  HBP_API_END
+}
+
 hbp_status hbp_ctx_profile(hbp_ctx *ctx, int enable) {
   HBP_API_BEGIN
   ctx->prof = enable != 0;

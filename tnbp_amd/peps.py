@@ -189,3 +189,28 @@ def MeasureEdges(ctx, peps: Peps, pairs, ops):
                                          buf.ctypes.data_as(C.POINTER(C.c_double)),
                                          out.ctypes.data_as(C.POINTER(C.c_double))))
     return list(out)
+
+
+def MeasurePaths(ctx, peps: Peps, paths, ops):
+    """<O_1 ... O_k> for product operators on simple paths of neighbouring
+    sites (extends measure.h:14-300 beyond two sites) -> list of complex values.
+
+    paths[t] is a list of site labels (consecutive ones adjacent); ops[t] the
+    list of (d, d) operators (in, out) on those sites, in the same order."""
+    n = len(paths)
+    if n == 0:
+        return []
+    offsets = [0]
+    flat_sites, flat_ops = [], []
+    for path, o in zip(paths, ops):
+        if len(path) != len(o):
+            raise ValueError("MeasurePaths: one operator per path site")
+        flat_sites += list(path)
+        flat_ops += [np.asarray(x, dtype=np.complex128).reshape(-1, order="F") for x in o]
+        offsets.append(len(flat_sites))
+    buf = np.ascontiguousarray(np.stack(flat_ops))
+    out = np.zeros(n, dtype=np.complex128)
+    check(ctx.lib.hbp_peps_measure_paths(ctx.h, peps.h, n, i32(offsets), i32(flat_sites),
+                                         buf.ctypes.data_as(C.POINTER(C.c_double)),
+                                         out.ctypes.data_as(C.POINTER(C.c_double))))
+    return list(out)

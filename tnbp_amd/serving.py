@@ -1,0 +1,287 @@
+"""Estimator serving on one GPU: a request loop with continuous batching.
+
+No reference counterpart: the reference runs one job per process
+(apps/estimator/main.cc:42-239: QASM circuit -> TPO slices -> AbsorbTPO, BP,
+Truncation per slice -> sparse-Pauli measurement).  Here a ``Server`` accepts
+such jobs as they arrive and runs several at once:
+
+* every job's PEPS and messages stay resident in HBM from admission to result;
+* admission is by memory footprint (``footprint_bytes``, the resident state
+  plus the work buffers of the largest absorbed site) against a budget of the
+  device's free memory, first come first served, with later jobs allowed past
+  a head job that does not fit yet (at most ``max_bypass`` times, so the head
+  cannot starve);
+* each of ``slots`` worker threads owns an hbp context (its own HIP stream and
+  memory pool); the library calls release the interpreter lock, so admitted
+  jobs' kernels overlap on the device, and a job admitted while others run
+  starts at once (continuous batching, not a queue of batches);
+* latency is measured from arrival to result; ``stats`` gives p50/p99 and
+  expectation values/s.
+
+Every job runs on its own stream with deterministic kernels (fixed-order
+reductions, no atomics), so its result does not depend on what else is
+resident.
+"""
+from __future__ import annotations
+
+import math
+import threading
+import time
+from collections import deque
+from concurrent.futures import Future
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import circuits, pauli
+from ._lib import HBP_PREC_FP64, HBP_PREC_MIXED
+from .peps import AbsorbTPO, InitTensorProductState, Truncation, bp_loop
+from .tci import Context, device_memory
+
+
+@dataclass
+class EstimatorJob:
+    """One estimator request: a circuit as TPO slices on a lattice and a sparse
+    Pauli observable [(coefficient, label)] (qubit 0 = last character)."""
+    edges: list
+    tpos: list
+    chi: int
+    terms: list
+    precision: int = HBP_PREC_MIXED
+    max_bp_iterations: int = 50
+    bp_tolerance: float = 1.0e-8      # apps/estimator/option.h:17
+    sv_min: float = 1.0e-8
+    name: str = ""
+    arrival: float = field(default=0.0, repr=False)
+    bypassed: int = field(default=0, repr=False)
+
+
+def _degrees(edges):
+    deg = {}
+    for a, b in edges:
+        deg[a] = deg.get(a, 0) + 1
+        deg[b] = deg.get(b, 0) + 1
+    return deg
+
+
+def footprint_bytes(job: EstimatorJob) -> int:
+    """Device bytes a job holds at its peak: resident chi-sized sites, the
+    messages (three sets during a sweep), three work buffers and the absorbed
+    site of the largest site at D = chi * t, split-K partials, and slack."""
+    elem = 16 if job.precision == HBP_PREC_FP64 else 8
+    deg = _degrees(job.edges)
+    t = 1
+    for tpo in job.tpos:
+        for O in tpo:
+            if O.ndim > 2:
+                t = max(t, max(O.shape[:-2]))
+    chi, d = job.chi, 2
+    D = chi * t
+    resident = sum(chi ** z * d * elem for z in deg.values())
+    zmax = max(deg.values())
+    work = 4 * D ** zmax * d * elem
+    msgs = 3 * 2 * len(job.edges) * D * D * 16
+    partials = 1024 * min(D, 128) ** 2 * 16 * 2
+    return int(1.15 * (resident + work + msgs + partials)) + (64 << 20)
+
+
+def run_job(ctx: Context, job: EstimatorJob):
+    """The estimator loop (apps/estimator/main.cc:119-178) and its measurement
+    (:181-240, extended to every term and the coefficient-weighted sum)."""
+    sites = sorted({s for e in job.edges for s in e})
+    peps = InitTensorProductState(ctx, job.edges, [2] * len(sites), job.precision)
+    iters = []
+    try:
+        for tpo in job.tpos:
+            AbsorbTPO(ctx, tpo, peps)
+            it, _ = bp_loop(ctx, peps, job.max_bp_iterations, job.bp_tolerance)
+            iters.append(it)
+            Truncation(ctx, peps, job.chi, job.sv_min)
+        total, vals = pauli.estimate(ctx, peps, job.edges, job.terms)
+    finally:
+        peps.close()
+    return {"value": total, "values": vals, "bp_iterations": iters, "n_values": len(vals)}
+
+
+class Server:
+    """Continuous-batching estimator server on one device (see module doc)."""
+
+    def __init__(self, device=0, slots=4, memory_budget=None, max_bypass=8, runner=None,
+                 make_context=None):
+        self.device = device
+        self._runner = runner or run_job
+        self._make_context = make_context or (lambda: Context(device))
+        self.slots = slots
+        if memory_budget is None:
+            free, _ = device_memory(device)
+            memory_budget = int(0.9 * free)
+        self.budget = memory_budget
+        self.max_bypass = max_bypass
+        self._pending = deque()
+        self._cv = threading.Condition()
+        self._in_use = 0
+        self._running = 0
+        self._closing = False
+        self.records = []
+        self._t0 = time.perf_counter()
+        self._threads = [threading.Thread(target=self._worker, name=f"hbp-slot{i}", daemon=True)
+                         for i in range(slots)]
+        for th in self._threads:
+            th.start()
+
+    # ---------------------------------------------------------------- API
+    def submit(self, job: EstimatorJob) -> Future:
+        fp = footprint_bytes(job)
+        if fp > self.budget:
+            raise MemoryError(f"job {job.name!r} needs {fp / 2**30:.1f} GiB, budget "
+                              f"{self.budget / 2**30:.1f} GiB")
+        fut = Future()
+        job.arrival = time.perf_counter()
+        with self._cv:
+            if self._closing:
+                raise RuntimeError("server is closed")
+            self._pending.append((job, fp, fut))
+            self._cv.notify_all()
+        return fut
+
+    def close(self):
+        with self._cv:
+            self._closing = True
+            self._cv.notify_all()
+        for th in self._threads:
+            th.join()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def stats(self):
+        """p50/p99 latency (arrival -> result), mean queue wait, expectation
+        values/s over the span from the first arrival to the last result."""
+        recs = list(self.records)
+        if not recs:
+            return {}
+        lat = np.array([r["latency"] for r in recs])
+        span = max(r["done"] for r in recs) - min(r["arrival"] for r in recs)
+        nv = sum(r["n_values"] for r in recs)
+        return {"jobs": len(recs), "p50_s": float(np.percentile(lat, 50)),
+                "p99_s": float(np.percentile(lat, 99)), "max_s": float(lat.max()),
+                "mean_wait_s": float(np.mean([r["wait"] for r in recs])),
+                "values_per_s": nv / span if span > 0 else float("inf"),
+                "span_s": span, "max_concurrent": max(r["concurrent"] for r in recs)}
+
+    # ------------------------------------------------------------ workers
+    def _take(self):
+        """Next admissible job (FIFO; later jobs may pass a head that does not
+        fit, up to max_bypass times), or None when closing and drained."""
+        with self._cv:
+            while True:
+                free = self.budget - self._in_use
+                for i, (job, fp, fut) in enumerate(self._pending):
+                    if fp <= free:
+                        del self._pending[i]
+                        for j in range(i):
+                            self._pending[j][0].bypassed += 1
+                        self._in_use += fp
+                        self._running += 1
+                        return job, fp, fut, self._running
+                    if job.bypassed >= self.max_bypass:
+                        break  # the head has waited long enough: nobody passes it
+                if self._closing and not self._pending:
+                    return None
+                self._cv.wait()
+
+    def _release(self, fp):
+        with self._cv:
+            self._in_use -= fp
+            self._running -= 1
+            self._cv.notify_all()
+
+    def _worker(self):
+        ctx = self._make_context()
+        try:
+            while True:
+                item = self._take()
+                if item is None:
+                    return
+                job, fp, fut, conc = item
+                start = time.perf_counter()
+                try:
+                    res = self._runner(ctx, job)
+                except BaseException as e:  # the job fails, the server goes on
+                    self._release(fp)
+                    fut.set_exception(e)
+                    continue
+                ctx.trim()
+                done = time.perf_counter()
+                self._release(fp)
+                rec = {"name": job.name, "arrival": job.arrival, "start": start, "done": done,
+                       "latency": done - job.arrival, "wait": start - job.arrival,
+                       "n_values": res["n_values"], "concurrent": conc, "footprint": fp}
+                res.update(rec)
+                self.records.append(rec)
+                fut.set_result(res)
+        finally:
+            ctx.close()
+
+
+# ------------------------------------------------------------- workloads
+def random_terms(rng, L, n_terms, w_min, w_max):
+    """Pauli strings of weight w ~ U{w_min..w_max} on a contiguous row run of
+    an L x L lattice (site x + L y), coefficients ~ N(0, 1)."""
+    terms = []
+    n = L * L
+    for _ in range(n_terms):
+        w = int(rng.integers(w_min, min(w_max, L) + 1))
+        y = int(rng.integers(0, L))
+        x0 = int(rng.integers(0, L - w + 1))
+        label = ["I"] * n
+        for x in range(x0, x0 + w):
+            label[n - 1 - (x + L * y)] = "XYZ"[rng.integers(3)]
+        terms.append((complex(rng.standard_normal()), "".join(label)))
+    return terms
+
+
+def random_circuit_job(rng, L, depth, chi, n_terms, w_max=4, precision=HBP_PREC_MIXED,
+                       name=""):
+    """A ragged-batch job: depth layers of RZZ(theta ~ U[0, pi]) on a random
+    matching then U3 (angles ~ U[0, 2 pi]) on every site, one TPO slice per
+    layer, and n_terms random Pauli strings of weight 1..w_max."""
+    edges = circuits.square_lattice(L, L)
+    tpos = [circuits.layer_tpo(edges, circuits.random_layer(rng, edges, L * L))
+            for _ in range(depth)]
+    return EstimatorJob(edges=edges, tpos=tpos, chi=chi, terms=random_terms(rng, L, n_terms, 1, w_max),
+                        precision=precision, name=name or f"L{L}-d{depth}-chi{chi}")
+
+
+def kicked_job(L, chi, steps, h, J=math.pi / 4, precision=HBP_PREC_MIXED, tol=1e-10, name=""):
+    """A kicked-Ising job: `steps` Floquet steps RZZ(J) on every bond then
+    RX(h) on every site, observable = all <Z_i> and nearest-neighbour <Z_i Z_j>
+    with unit coefficients."""
+    edges = circuits.square_lattice(L, L)
+    tpo = circuits.kicked_ising_tpo(edges, J / (2 * math.pi), 0.0, h / (2 * math.pi), math.pi)
+    n = L * L
+    terms = []
+    for s in range(n):
+        lab = ["I"] * n
+        lab[n - 1 - s] = "Z"
+        terms.append((1.0 + 0j, "".join(lab)))
+    for a, b in edges:
+        lab = ["I"] * n
+        lab[n - 1 - a] = "Z"
+        lab[n - 1 - b] = "Z"
+        terms.append((1.0 + 0j, "".join(lab)))
+    return EstimatorJob(edges=edges, tpos=[tpo] * steps, chi=chi, terms=terms, precision=precision,
+                        bp_tolerance=tol, name=name or f"kicked-L{L}-chi{chi}")
+
+
+def poisson_submit(server, jobs, rate, rng):
+    """Submit jobs with exponential inter-arrival times of mean 1/rate (s);
+    returns their futures."""
+    futs = []
+    for job in jobs:
+        futs.append(server.submit(job))
+        time.sleep(float(rng.exponential(1.0 / rate)))
+    return futs

@@ -38,6 +38,17 @@ class Context:
         check(self.lib.hbp_ctx_memory(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value
 
+    def trim(self):
+        """Give the pool's unused memory back to the device."""
+        check(self.lib.hbp_ctx_trim(self.h))
+
+
+def device_memory(device: int = 0):
+    """(free, total) bytes of a device."""
+    a, b = C.c_size_t(), C.c_size_t()
+    check(_lib.load().hbp_device_memory(device, C.byref(a), C.byref(b)))
+    return a.value, b.value
+
     def __enter__(self):
         return self
 
