@@ -329,3 +329,57 @@ def random_layer_gates(rng, edges, n_sites):
     for s in range(n_sites):
         gates.append((gate_matrix("u3", tuple(rng.uniform(0, 2 * math.pi, 3))), (s,)))
     return gates
+
+
+def opt_tpo_by_svd(edges, O, eps):
+    """``OptTPObySVD`` (include/tnbp/ptns/function/optsvdtpo.h:87-214): for
+    every edge in order, contract the two site operators over their shared
+    bond, trunc_svd with rows = the first site's other indices (chi_max = the
+    bond's dimension, s_min = eps), and put sqrt(S) on both sides, the new
+    bond back at its position.  Returns the new list (order of
+    ``site_index_from_bond``)."""
+    sites = tnbp.site_index_from_bond(edges)
+    O = [o.copy() for o in O]
+    for m, (a, b) in enumerate(edges):
+        ia, ib = sites.index(a), sites.index(b)
+        ka = tnbp.surrounding_bond_index(a, edges).index(m)
+        kb = tnbp.surrounding_bond_index(b, edges).index(m)
+        na, nb = O[ia].ndim, O[ib].ndim
+        la, lb, lc = [0] * na, [0] * nb, []
+        u = 0
+        for k in range(na):
+            if k == ka:
+                la[k] = -1
+            else:
+                la[k] = u
+                lc.append(u)
+                u += 1
+        for k in range(nb):
+            if k == kb:
+                lb[k] = -1
+            else:
+                lb[k] = u
+                lc.append(u)
+                u += 1
+        C = tci.contract(O[ia], la, O[ib], lb, lc)
+        U, S, V, _ = tci.trunc_svd(C, na - 1, O[ia].shape[ka], eps)
+        D = np.diag(np.sqrt(S)).astype(np.complex128)
+        lu = [0] * na
+        for k in range(na):
+            if k == ka:
+                lu[na - 1] = -1
+            elif k < ka:
+                lu[k] = k
+            else:
+                lu[k - 1] = k
+        O[ia] = tci.contract(U, lu, D, [-1, ka], list(range(na)))
+        lv = [0] * nb
+        for k in range(nb):
+            if k == kb:
+                lv[0] = -1
+            elif k < kb:
+                lv[k + 1] = k
+            else:
+                lv[k] = k
+        O[ib] = tci.contract(D, [kb, -1], V, lv, list(range(nb)))
+    return O

@@ -152,3 +152,29 @@ def random_layer(rng, edges, n_sites):
     for s in range(n_sites):
         gates.append((u3(*rng.uniform(0, 2 * math.pi, 3)), (s,)))
     return gates
+
+
+def opt_tpo_by_svd(edges, O, eps=1.0e-12):
+    """Compress every TPO bond by SVD (``OptTPObySVD``, optsvdtpo.h:87-214):
+    per edge in order, the two site operators contracted over the shared bond
+    are refactored as U sqrt(S) | sqrt(S) V keeping singular values > eps (at
+    most the old dimension, at least one).  E.g. cx . u1 . cx on an edge
+    (t = 4 when stacked gate by gate) compresses to t = 2."""
+    sites = sorted({s for e in edges for s in e})
+    pos = {s: i for i, s in enumerate(sites)}
+    inc = {s: incident(edges, s) for s in sites}
+    O = [np.asarray(o, dtype=complex) for o in O]
+    for m, (a, b) in enumerate(edges):
+        ia, ib = pos[a], pos[b]
+        ka, kb = inc[a].index(m), inc[b].index(m)
+        A = np.moveaxis(O[ia], ka, -1)                     # (others_a..., t)
+        B = np.moveaxis(O[ib], kb, 0)                      # (t, others_b...)
+        t = A.shape[-1]
+        sa, sb = A.shape[:-1], B.shape[1:]
+        C = A.reshape(-1, t, order="F") @ B.reshape(t, -1, order="F")
+        u, s, vh = np.linalg.svd(C, full_matrices=False)
+        k = max(1, min(t, int(np.count_nonzero(s > eps))))
+        r = np.sqrt(s[:k])
+        O[ia] = np.moveaxis((u[:, :k] * r).reshape(sa + (k,), order="F"), -1, ka)
+        O[ib] = np.moveaxis((r[:, None] * vh[:k]).reshape((k,) + sb, order="F"), 0, kb)
+    return O

@@ -403,17 +403,35 @@ __global__ void k_build_weights(const double2 *__restrict__ E, double2 *__restri
   }
 }
 
-__global__ void k_reduce_partials(const double2 *__restrict__ p, int nchunks, int n,
-                                  double2 *__restrict__ M) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n) return;
+// M[e] = sum_c p[c n + e].  A block owns 16 entries; its 16 thread groups
+// each sum the chunks g, g + 16, ... in order and the 16 group sums are added
+// in a fixed order, so the result is bitwise reproducible.
+constexpr int RED_E = 16, RED_G = 16;
+__global__ __launch_bounds__(256) void k_reduce_partials(const double2 *__restrict__ p,
+                                                          int nchunks, int n,
+                                                          double2 *__restrict__ M) {
+  __shared__ double sre[RED_G][RED_E + 1], sim[RED_G][RED_E + 1];
+  const int el = threadIdx.x % RED_E, g = threadIdx.x / RED_E;
+  const int e = blockIdx.x * RED_E + el;
   double re = 0.0, im = 0.0;
-  for (int c = 0; c < nchunks; ++c) {
-    const double2 v = p[(int64_t)c * n + e];
-    re += v.x;
-    im += v.y;
+  if (e < n) {
+    for (int c = g; c < nchunks; c += RED_G) {
+      const double2 v = p[(int64_t)c * n + e];
+      re += v.x;
+      im += v.y;
+    }
   }
-  M[e] = make_double2(re, im);
+  sre[g][el] = re;
+  sim[g][el] = im;
+  __syncthreads();
+  if (g == 0 && e < n) {
+    double tr = 0.0, ti = 0.0;
+    for (int k = 0; k < RED_G; ++k) {
+      tr += sre[k][el];
+      ti += sim[k][el];
+    }
+    M[e] = make_double2(tr, ti);
+  }
 }
 
 // ------------------------------------------------------------ elementwise
@@ -862,8 +880,8 @@ void launch_build_weights(hipStream_t st, const double2 *E, double2 *w, int64_t 
 
 void launch_reduce_partials(hipStream_t st, const double2 *partials, int nchunks, int n,
                             double2 *M) {
-  hipLaunchKernelGGL(k_reduce_partials, dim3((n + 255) / 256), dim3(256), 0, st, partials,
-                     nchunks, n, M);
+  hipLaunchKernelGGL(k_reduce_partials, dim3((n + RED_E - 1) / RED_E), dim3(256), 0, st,
+                     partials, nchunks, n, M);
   HBP_HIP(hipGetLastError());
 }
 
