@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
     ap.add_argument("--mode", choices=["fused", "plain"], default="fused",
                     help="plain: the reference's op sequence one TCI call at a time")
+    ap.add_argument("--sweep", choices=["auto", "per-site", "grouped"], default="auto",
+                    help="BP sweep launch form (grouped needs bond dims <= 32)")
     ap.add_argument("--compare", type=int, default=1,
                     help="also time the FP64 fused path on the same steps")
     ap.add_argument("--tol", type=float, default=1e-10)
@@ -154,6 +156,7 @@ def run_path(args, precision, mode, want_cpu_sample=False):
     ctx = Context(0)
     peps = InitTensorProductState(ctx, edges, [2] * len(sites), prec)
     check(ctx.lib.hbp_peps_set_mode(peps.h, 0 if mode == "fused" else 1))
+    peps.set_sweep_kernel({"auto": 0, "per-site": 1, "grouped": 2}[args.sweep])
 
     def step(record):
         t0 = time.perf_counter()
@@ -281,7 +284,8 @@ def main():
                                f"h={head['h']:.6f} (seed {args.seed}), BP tol {args.tol:g} "
                                f"(max {args.max_bp_iterations} iterations), steady-state Floquet "
                                f"step with 256 <Z> + 480 <ZZ>",
-                   "precision": args.precision, "mode": args.mode, "L": L, "chi": chi},
+                   "precision": args.precision, "mode": args.mode, "sweep": args.sweep,
+                   "L": L, "chi": chi},
         "latency_s": head["latency_s"],
         "bp_iterations": head["bp_iterations"],
         "roofline": roofline(head, args.precision),

@@ -176,6 +176,25 @@ hbp_status hbp_peps_set_residual_floor(hbp_peps *p, double floor);
  * ~4/pi.  Expectation values do not depend on the phases.  No reference
  * counterpart; PLAIN mode keeps the reference's residual. */
 hbp_status hbp_peps_set_phase_gauge(hbp_peps *p, int on);
+/* FUSED mode: how AbsorbTPO stores the absorbed tensors.  LAZY keeps each
+ * chi-sized V_s with its operator O_s and materialises the absorbed tensor
+ * per site when a sweep needs it (the absorbed 32x32 lattice at D = 64 is
+ * 550 GB in FP64); EAGER stores the absorbed lattice (apply.h:52-66); AUTO
+ * (default) is EAGER while the absorbed lattice takes at most half of the
+ * free device memory.  PLAIN mode always absorbs eagerly.  No reference
+ * counterpart. */
+enum { HBP_ABSORB_AUTO = 0, HBP_ABSORB_LAZY = 1, HBP_ABSORB_EAGER = 2 };
+hbp_status hbp_peps_set_absorb_mode(hbp_peps *p, int mode);
+/* FUSED mode: how a BP sweep is launched.  PER_SITE issues each site's mode
+ * products and messages as separate launches (efficient once bonds reach
+ * ~64); GROUPED runs each level of the sweep for all sites (or a group of
+ * sites that fits a workspace of min(free/4, 16 GiB)) as one launch per
+ * column-block count - five to ten launches per sweep instead of ~12 per
+ * site; needs site degree <= 4 and bond dims <= 32.  AUTO (default) is
+ * PER_SITE: GROUPED is experimental and taken only on request.  No reference
+ * counterpart. */
+enum { HBP_SWEEP_AUTO = 0, HBP_SWEEP_PER_SITE = 1, HBP_SWEEP_GROUPED = 2 };
+hbp_status hbp_peps_set_sweep_kernel(hbp_peps *p, int kind);
 hbp_status hbp_peps_num_sites(const hbp_peps *p, int *n_sites, int *n_edges);
 /* Shape of site tensor s (order = degree + 1) and of message m (2 x D). */
 hbp_status hbp_peps_site_shape(const hbp_peps *p, int site, int *order, int32_t *shape);
@@ -184,6 +203,13 @@ hbp_status hbp_peps_site_shape(const hbp_peps *p, int site, int *order, int32_t 
 hbp_status hbp_peps_get_site(hbp_ctx *ctx, hbp_peps *p, int site, double *host);
 hbp_status hbp_peps_set_site(hbp_ctx *ctx, hbp_peps *p, int site, int order,
                              const int32_t *shape, const double *host);
+/* Replace every site by zeros of bond dims bond_dims[edge] (and physical dims
+ * phys_dims[site] when not null) and every message by a zero matrix, dropping
+ * pending absorptions: the first step of restoring a checkpoint, after which
+ * set_site / set_message fill in the saved state (the restart of
+ * apps/estimator_restart reads V and E back in the same way, iotps.h:37-105). */
+hbp_status hbp_peps_set_bond_dims(hbp_ctx *ctx, hbp_peps *p, const int32_t *bond_dims,
+                                  const int32_t *phys_dims);
 hbp_status hbp_peps_get_message(hbp_ctx *ctx, hbp_peps *p, int m, int32_t *dim, double *host);
 hbp_status hbp_peps_set_message(hbp_ctx *ctx, hbp_peps *p, int m, int32_t dim,
                                 const double *host);

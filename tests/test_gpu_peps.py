@@ -200,3 +200,100 @@ def test_bp_iterate_pieces_equal_one_call(ctx):
         np.testing.assert_array_equal(runs[0][1], runs[1][1])
         np.testing.assert_array_equal(runs[0][2], runs[1][2])
         np.testing.assert_array_equal(runs[0][3], runs[1][3])
+
+
+def test_lazy_and_eager_absorption_agree(ctx):
+    """Lazy absorption materialises the same absorbed tensors the eager path
+    stores, so every result is bitwise the same."""
+    from tnbp_amd import circuits as pc
+    from tnbp_amd.peps import (HBP_ABSORB_EAGER, HBP_ABSORB_LAZY, AbsorbTPO, InitTensorProductState,
+                               Measure, MeasureEdges, Truncation, bp_loop)
+    announce("lazy vs eager absorption")
+    rng = np.random.default_rng(4)
+    edges = pc.square_lattice(3, 4)
+    tpos = [pc.layer_tpo(edges, pc.random_layer(rng, edges, 12)) for _ in range(3)]
+    out = []
+    for mode in (HBP_ABSORB_LAZY, HBP_ABSORB_EAGER):
+        p = InitTensorProductState(ctx, edges)
+        p.set_absorb_mode(mode)
+        res = []
+        for t in tpos:
+            AbsorbTPO(ctx, t, p)
+            it, r = bp_loop(ctx, p, 30, 1e-12)
+            res.append((it, r[-1]))
+            Truncation(ctx, p, 4, 1e-10)
+        z = Measure(ctx, p, list(range(12)), [pc.op_tensor(pc.PAULI["Z"])] * 12)
+        zz = MeasureEdges(ctx, p, edges, [pc.op_tensor(np.kron(pc.PAULI["X"], pc.PAULI["Z"]))] * len(edges))
+        out.append((res, z, zz))
+        p.close()
+    assert out[0][0] == out[1][0]
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+    np.testing.assert_array_equal(out[0][2], out[1][2])
+
+
+@pytest.mark.skip(reason="grouped sweep disabled: its first GPU run left the device unusable "
+                         "and the cause is not yet found")
+@pytest.mark.parametrize("precision,tol", [(0, 1e-12), (1, 1e-6)])
+def test_grouped_sweep_matches_per_site(ctx, precision, tol):
+    """The grouped sweep (one launch per level for all sites) against the
+    per-site launches on a 4x4 lattice with random gates (degrees 2-4)."""
+    from tnbp_amd import circuits as pc
+    from tnbp_amd.peps import (HBP_SWEEP_GROUPED, HBP_SWEEP_PER_SITE, AbsorbTPO,
+                               InitTensorProductState, Measure, MeasureEdges, Truncation, bp_loop)
+    announce(f"grouped vs per-site sweep precision {precision}")
+    rng = np.random.default_rng(13)
+    edges = pc.square_lattice(4, 4)
+    tpos = [pc.layer_tpo(edges, pc.random_layer(rng, edges, 16)) for _ in range(4)]
+    out = []
+    for kind in (HBP_SWEEP_PER_SITE, HBP_SWEEP_GROUPED):
+        p = InitTensorProductState(ctx, edges, [2] * 16, precision)
+        p.set_sweep_kernel(kind)
+        res = []
+        for t in tpos:
+            AbsorbTPO(ctx, t, p)
+            it, r = bp_loop(ctx, p, 6, 0.0)
+            res.append(r)
+            Truncation(ctx, p, 8, 1e-10)
+        z = Measure(ctx, p, list(range(16)), [pc.op_tensor(pc.PAULI["Y"])] * 16)
+        zz = MeasureEdges(ctx, p, edges, [pc.op_tensor(np.kron(pc.PAULI["Z"], pc.PAULI["X"]))] * len(edges))
+        out.append((res, z, zz, p.get_message(5)))
+        p.close()
+    for ra, rb in zip(out[0][0], out[1][0]):
+        np.testing.assert_allclose(ra, rb, rtol=1e-6 if precision else 1e-9, atol=1e-14)
+    assert rel_close(out[1][1], out[0][1], tol) <= 1.0
+    assert rel_close(out[1][2], out[0][2], tol) <= 1.0
+    np.testing.assert_allclose(out[1][3], out[0][3], atol=tol)
+
+
+def test_checkpoint_round_trip(ctx, tmp_path):
+    """save_peps / load_peps: a restored state continues exactly like the
+    original (bitwise equal messages and expectation values)."""
+    from tnbp_amd import circuits as pc
+    from tnbp_amd.peps import AbsorbTPO, InitTensorProductState, Measure, Truncation, bp_loop, load_peps, save_peps
+    announce("checkpoint round trip")
+    rng = np.random.default_rng(6)
+    edges = pc.square_lattice(3, 3)
+    tpos = [pc.layer_tpo(edges, pc.random_layer(rng, edges, 9)) for _ in range(3)]
+    p = InitTensorProductState(ctx, edges)
+    for t in tpos[:2]:
+        AbsorbTPO(ctx, t, p)
+        bp_loop(ctx, p, 20, 1e-12)
+        Truncation(ctx, p, 4, 1e-10)
+    path = tmp_path / "state.hbpp"
+    save_peps(p, path)
+    q = load_peps(ctx, path)
+    for s in range(9):
+        np.testing.assert_array_equal(q.get_site(s), p.get_site(s))
+    zop = pc.op_tensor(pc.PAULI["Z"])
+    outs = []
+    for st in (p, q):
+        AbsorbTPO(ctx, tpos[2], st)
+        it, r = bp_loop(ctx, st, 20, 1e-12)
+        Truncation(ctx, st, 4, 1e-10)
+        outs.append((it, r, Measure(ctx, st, list(range(9)), [zop] * 9), st.get_message(7)))
+    assert outs[0][0] == outs[1][0]
+    np.testing.assert_array_equal(outs[0][1], outs[1][1])
+    np.testing.assert_array_equal(outs[0][2], outs[1][2])
+    np.testing.assert_array_equal(outs[0][3], outs[1][3])
+    p.close()
+    q.close()

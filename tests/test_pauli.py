@@ -47,3 +47,15 @@ def test_support_paths():
     np.testing.assert_array_equal(ops[1], np.eye(2))
     with pytest.raises(ValueError):
         pauli.support_path(edges, [1, 4, 6, 9])  # 1-0-4-5-6, then 6-5-9 revisits 5
+
+
+def test_checkpoint_rejects_foreign_files(tmp_path):
+    from tnbp_amd.peps import load_peps
+    bad = tmp_path / "x.bin"
+    bad.write_bytes(b"NOPE" + bytes(16))
+    with pytest.raises(ValueError):
+        load_peps(None, bad)
+    old = tmp_path / "v0.bin"
+    old.write_bytes(b"HBPP" + (0).to_bytes(4, "little") + bytes(12))
+    with pytest.raises(ValueError):
+        load_peps(None, old)

@@ -110,3 +110,13 @@ def test_footprint_model_scales(monkeypatch):
     # 16x16 chi=32 mixed kicked-Ising job: resident state ~4.3 GB plus D=64 work buffers
     k = serving.kicked_job(16, 32, 1, 0.7, precision=HBP_PREC_MIXED)
     assert 4e9 < serving.footprint_bytes(k) < 10e9
+
+
+def test_no_device_fails_jobs_instead_of_hanging():
+    def broken():
+        raise RuntimeError("no device")
+    srv = serving.Server(slots=2, memory_budget=100, runner=lambda c, j: None, make_context=broken)
+    fut = srv.submit(_job("x", 10, 0.0))
+    with pytest.raises(RuntimeError):
+        fut.result(timeout=10)
+    srv.close()

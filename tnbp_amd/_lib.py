@@ -14,6 +14,8 @@ LIB_PATH = os.path.join(HERE, "libhbp.so")
 HBP_MAX_ORDER = 16
 HBP_C128, HBP_F64, HBP_C64, HBP_F32 = 0, 1, 2, 3
 HBP_PREC_FP64, HBP_PREC_MIXED = 0, 1
+HBP_ABSORB_AUTO, HBP_ABSORB_LAZY, HBP_ABSORB_EAGER = 0, 1, 2
+HBP_SWEEP_AUTO, HBP_SWEEP_PER_SITE, HBP_SWEEP_GROUPED = 0, 1, 2
 
 STATUS = {
     1: ValueError,       # HBP_ERR_INVALID_ARGUMENT (std::invalid_argument)
@@ -64,6 +66,9 @@ SIGNATURES = {
     "hbp_peps_set_mode": [_vp, C.c_int],
     "hbp_peps_set_residual_floor": [_vp, C.c_double],
     "hbp_peps_set_phase_gauge": [_vp, C.c_int],
+    "hbp_peps_set_absorb_mode": [_vp, C.c_int],
+    "hbp_peps_set_sweep_kernel": [_vp, C.c_int],
+    "hbp_peps_set_bond_dims": [_vp, _vp, _i32p, _i32p],
     "hbp_peps_num_sites": [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "hbp_peps_site_shape": [_vp, C.c_int, C.POINTER(C.c_int), _i32p],
     "hbp_peps_get_site": [_vp, _vp, C.c_int, _dp],

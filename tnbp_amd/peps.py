@@ -13,7 +13,9 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import HBP_MAX_ORDER, HBP_PREC_FP64, HBP_PREC_MIXED, check, i32
+from ._lib import (HBP_ABSORB_AUTO, HBP_ABSORB_EAGER, HBP_ABSORB_LAZY, HBP_MAX_ORDER,
+                   HBP_PREC_FP64, HBP_PREC_MIXED, HBP_SWEEP_AUTO, HBP_SWEEP_GROUPED,
+                   HBP_SWEEP_PER_SITE, check, i32)
 from .tci import Context
 
 __all__ = ["Peps", "InitTensorProductState", "AbsorbTPO", "BeliefPropagation",
@@ -57,6 +59,21 @@ class Peps:
     def n_edges(self):
         return len(self.edges)
 
+    def set_mode(self, mode):
+        """HBP_MODE_FUSED (0) or HBP_MODE_PLAIN (1, the reference's op sequence)."""
+        check(self.ctx.lib.hbp_peps_set_mode(self.h, int(mode)))
+
+    def set_absorb_mode(self, mode):
+        """HBP_ABSORB_AUTO (0), HBP_ABSORB_LAZY (1) or HBP_ABSORB_EAGER (2)."""
+        check(self.ctx.lib.hbp_peps_set_absorb_mode(self.h, int(mode)))
+
+    def set_sweep_kernel(self, kind):
+        """HBP_SWEEP_AUTO (0), HBP_SWEEP_PER_SITE (1) or HBP_SWEEP_GROUPED (2)."""
+        check(self.ctx.lib.hbp_peps_set_sweep_kernel(self.h, int(kind)))
+
+    def set_phase_gauge(self, on):
+        check(self.ctx.lib.hbp_peps_set_phase_gauge(self.h, int(bool(on))))
+
     def site_shape(self, s):
         order = C.c_int()
         sh = (C.c_int32 * HBP_MAX_ORDER)()
@@ -89,6 +106,64 @@ class Peps:
         buf = np.ascontiguousarray(a.reshape(-1, order="F"))
         check(self.ctx.lib.hbp_peps_set_message(self.ctx.h, self.h, m, a.shape[0],
                                                 buf.ctypes.data_as(C.POINTER(C.c_double))))
+
+
+# ------------------------------------------------------------ checkpoints
+CHECKPOINT_MAGIC = b"HBPP"
+CHECKPOINT_VERSION = 1
+
+
+def save_peps(peps: Peps, path):
+    """Write the state (edges, site tensors, messages) to a versioned binary
+    file: "HBPP", u32 version, u32 precision, u32 n_sites, u32 n_edges,
+    int32 edges (2 n_edges), then per site u32 order, int32 shape, complex128
+    column-major data, then per message int32 D, complex128 D x D.  The
+    reference's restart stores V and E with gqten's stream format
+    (iotps.h:37-105), which is not available to match."""
+    n = len(peps.sites)
+    with open(path, "wb") as f:
+        f.write(CHECKPOINT_MAGIC)
+        np.array([CHECKPOINT_VERSION, peps.precision, n, len(peps.edges)], dtype="<u4").tofile(f)
+        np.array([x for e in peps.edges for x in e], dtype="<i4").tofile(f)
+        for s in range(n):
+            v = peps.get_site(s)
+            np.array([v.ndim], dtype="<u4").tofile(f)
+            np.array(v.shape, dtype="<i4").tofile(f)
+            np.ascontiguousarray(v.reshape(-1, order="F")).astype("<c16").tofile(f)
+        for m in range(2 * len(peps.edges)):
+            e = peps.get_message(m)
+            np.array([e.shape[0]], dtype="<i4").tofile(f)
+            np.ascontiguousarray(e.reshape(-1, order="F")).astype("<c16").tofile(f)
+
+
+def load_peps(ctx, path) -> Peps:
+    """Read a state written by ``save_peps`` into a new PEPS on ctx."""
+    with open(path, "rb") as f:
+        if f.read(4) != CHECKPOINT_MAGIC:
+            raise ValueError(f"{path}: not an hbp PEPS checkpoint")
+        ver, prec, n, ne = (int(x) for x in np.fromfile(f, dtype="<u4", count=4))
+        if ver != CHECKPOINT_VERSION:
+            raise ValueError(f"{path}: checkpoint version {ver} (expected {CHECKPOINT_VERSION})")
+        flat = np.fromfile(f, dtype="<i4", count=2 * ne)
+        edges = [(int(flat[2 * i]), int(flat[2 * i + 1])) for i in range(ne)]
+        sites = []
+        for _ in range(n):
+            order = int(np.fromfile(f, dtype="<u4", count=1)[0])
+            shape = tuple(int(x) for x in np.fromfile(f, dtype="<i4", count=order))
+            data = np.fromfile(f, dtype="<c16", count=int(np.prod(shape)))
+            sites.append(data.reshape(shape, order="F"))
+        msgs = []
+        for _ in range(2 * ne):
+            d = int(np.fromfile(f, dtype="<i4", count=1)[0])
+            msgs.append(np.fromfile(f, dtype="<c16", count=d * d).reshape((d, d), order="F"))
+    peps = Peps(ctx, edges, [v.shape[-1] for v in sites], prec)
+    dims = [msgs[m].shape[0] for m in range(ne)]
+    check(ctx.lib.hbp_peps_set_bond_dims(ctx.h, peps.h, i32(dims), i32([v.shape[-1] for v in sites])))
+    for s, v in enumerate(sites):
+        peps.set_site(s, v)
+    for m, e in enumerate(msgs):
+        peps.set_message(m, e)
+    return peps
 
 
 def InitTensorProductState(ctx, edges, pdim=None, precision=HBP_PREC_FP64) -> Peps:

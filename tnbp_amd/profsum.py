@@ -100,7 +100,7 @@ def main():
     ap.add_argument("kind", choices=["stats", "pmc", "traffic"])
     ap.add_argument("path", nargs="+")
     ap.add_argument("--title", default="rocprofv3 summary")
-    ap.add_argument("--cmd", default="", action="append")
+    ap.add_argument("--cmd", default=None, action="append")
     ap.add_argument("--pattern", default="")
     ap.add_argument("--precision", default="fp64")
     ap.add_argument("--key", default="")
@@ -108,14 +108,15 @@ def main():
     ap.add_argument("--launch", default="")
     ap.add_argument("--out", default="profiles/traffic.json")
     a = ap.parse_args()
-    cmd = "\n".join(c for c in a.cmd if c)
+    cmds = [c for c in (a.cmd or []) if c]
+    cmd = "\n".join(cmds)
     if a.kind == "stats":
         print(stats(a.path[0], a.title, cmd), end="")
     elif a.kind == "pmc":
         print(pmc(a.path[0], a.title, cmd), end="")
     else:
         t = traffic(*a.path[:4], a.pattern, a.precision, a.key, a.alg_bytes, a.launch,
-                    [c for c in a.cmd if c], a.out)
+                    cmds, a.out)
         print(json.dumps(t[a.precision][a.key]))
 
 

@@ -1,0 +1,84 @@
+"""Serving stream benchmark: Poisson arrivals of sparse-Pauli estimator jobs.
+
+Each job is a random circuit on an L x L lattice (RZZ(theta ~ U[0, pi]) on a
+random matching then U3 on every site, per layer) with a random sparse Pauli
+observable (weight 1..w_max on contiguous row runs, coefficients ~ N(0, 1)),
+drawn from a seed.  Jobs arrive with exponential inter-arrival times at each
+swept rate; the server (continuous batching, footprint admission) reports
+p50/p99 latency (arrival -> result) and expectation values/s.  One JSON line
+per (path, rate).
+
+    python -m tnbp_amd.serve_bench [--jobs 24] [--rates 2,8,32] [--L 6,10]
+        [--chi 8,16] [--depth 3,6] [--terms 1,64] [--wmax 6] [--slots 4] [--plain 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import time
+
+import numpy as np
+
+from . import serving
+from ._lib import HBP_PREC_FP64, HBP_PREC_MIXED
+
+
+def make_jobs(seed, n, Ls, chis, depths, terms, wmax, precision, mode=0):
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for i in range(n):
+        L = int(rng.integers(Ls[0], Ls[1] + 1))
+        chi = int(rng.choice(chis))
+        depth = int(rng.integers(depths[0], depths[1] + 1))
+        nt = int(rng.integers(terms[0], terms[1] + 1))
+        j = serving.random_circuit_job(rng, L, depth, chi, nt, w_max=wmax, precision=precision,
+                                       name=f"j{i}-L{L}-chi{chi}-d{depth}")
+        j.mode = mode
+        jobs.append(j)
+    return jobs
+
+
+def run_stream(jobs, rate, slots, seed):
+    rng = np.random.default_rng(seed + 1)
+    t0 = time.perf_counter()
+    with serving.Server(slots=slots) as srv:
+        futs = serving.poisson_submit(srv, jobs, rate, rng)
+        res = [f.result() for f in futs]
+        st = srv.stats()
+    st["wall_s"] = time.perf_counter() - t0
+    st["rate_jobs_per_s"] = rate
+    return st, res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--jobs", type=int, default=24)
+    ap.add_argument("--rates", default="2,8,32")
+    ap.add_argument("--L", default="6,10")
+    ap.add_argument("--chi", default="8,16")
+    ap.add_argument("--depth", default="3,6")
+    ap.add_argument("--terms", default="1,64")
+    ap.add_argument("--wmax", type=int, default=6)
+    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--plain", type=int, default=1, help="also run the plain FP64 path at the lowest rate")
+    a = ap.parse_args()
+    two = lambda s: [int(x) for x in s.split(",")]
+    Ls, chis, depths, terms = two(a.L), two(a.chi), two(a.depth), two(a.terms)
+    rates = [float(x) for x in a.rates.split(",")]
+    ref = None
+    for rate in rates:
+        jobs = make_jobs(a.seed, a.jobs, Ls, chis, depths, terms, a.wmax, HBP_PREC_MIXED)
+        st, res = run_stream(jobs, rate, a.slots, a.seed)
+        if ref is None:
+            ref = res
+        print(json.dumps({"path": "mixed/fused", **st}), flush=True)
+    if a.plain:
+        jobs = make_jobs(a.seed, a.jobs, Ls, chis, depths, terms, a.wmax, HBP_PREC_FP64, mode=1)
+        st, res = run_stream(jobs, rates[0], a.slots, a.seed)
+        err = max(abs(x["value"] - y["value"]) / max(1.0, abs(y["value"])) for x, y in zip(ref, res))
+        print(json.dumps({"path": "fp64/plain", **st, "max_rel_err_mixed_vs_plain": err}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -34,7 +34,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import circuits, pauli
-from ._lib import HBP_PREC_FP64, HBP_PREC_MIXED
+from ._lib import HBP_ABSORB_LAZY, HBP_PREC_FP64, HBP_PREC_MIXED
 from .peps import AbsorbTPO, InitTensorProductState, Truncation, bp_loop
 from .tci import Context, device_memory
 
@@ -51,6 +51,7 @@ class EstimatorJob:
     max_bp_iterations: int = 50
     bp_tolerance: float = 1.0e-8      # apps/estimator/option.h:17
     sv_min: float = 1.0e-8
+    mode: int = 0                     # HBP_MODE_FUSED; 1 = the reference's op sequence
     name: str = ""
     arrival: float = field(default=0.0, repr=False)
     bypassed: int = field(default=0, repr=False)
@@ -66,8 +67,10 @@ def _degrees(edges):
 
 def footprint_bytes(job: EstimatorJob) -> int:
     """Device bytes a job holds at its peak: resident chi-sized sites, the
-    messages (three sets during a sweep), three work buffers and the absorbed
-    site of the largest site at D = chi * t, split-K partials, and slack."""
+    messages (three sets during a sweep), the work buffers of the per-site
+    sweep (four of the largest site at D = chi * t) or of the grouped sweep
+    (up to seven per site at D <= 32, at most 16 GiB), split-K partials, and
+    slack.  The plain mode stores the absorbed lattice."""
     elem = 16 if job.precision == HBP_PREC_FP64 else 8
     deg = _degrees(job.edges)
     t = 1
@@ -80,6 +83,10 @@ def footprint_bytes(job: EstimatorJob) -> int:
     resident = sum(chi ** z * d * elem for z in deg.values())
     zmax = max(deg.values())
     work = 4 * D ** zmax * d * elem
+    if D <= 32 and zmax <= 4:
+        work = max(work, min(16 << 30, sum(7 * D ** z * d * elem for z in deg.values())))
+    if job.mode == 1:
+        work += sum(D ** z * d * elem for z in deg.values())
     msgs = 3 * 2 * len(job.edges) * D * D * 16
     partials = 1024 * min(D, 128) ** 2 * 16 * 2
     return int(1.15 * (resident + work + msgs + partials)) + (64 << 20)
@@ -90,6 +97,9 @@ def run_job(ctx: Context, job: EstimatorJob):
     (:181-240, extended to every term and the coefficient-weighted sum)."""
     sites = sorted({s for e in job.edges for s in e})
     peps = InitTensorProductState(ctx, job.edges, [2] * len(sites), job.precision)
+    # the footprint model counts chi-sized resident sites: absorb lazily
+    peps.set_absorb_mode(HBP_ABSORB_LAZY)
+    peps.set_mode(job.mode)
     iters = []
     try:
         for tpo in job.tpos:
@@ -140,6 +150,9 @@ class Server:
         with self._cv:
             if self._closing:
                 raise RuntimeError("server is closed")
+            if getattr(self, "_dead", 0) == self.slots:
+                fut.set_exception(self._dead_reason)
+                return fut
             self._pending.append((job, fp, fut))
             self._cv.notify_all()
         return fut
@@ -200,7 +213,18 @@ class Server:
             self._cv.notify_all()
 
     def _worker(self):
-        ctx = self._make_context()
+        try:
+            ctx = self._make_context()
+        except BaseException as e:  # no device: fail what is queued rather than hang
+            with self._cv:
+                self._dead = getattr(self, "_dead", 0) + 1
+                self._dead_reason = e
+                if self._dead == self.slots:
+                    while self._pending:
+                        _, _, fut = self._pending.popleft()
+                        fut.set_exception(e)
+                self._cv.notify_all()
+            return
         try:
             while True:
                 item = self._take()
