@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import sys
 import time
 
 import numpy as np
@@ -43,6 +44,11 @@ def run_stream(jobs, rate, slots, seed):
     t0 = time.perf_counter()
     with serving.Server(slots=slots) as srv:
         futs = serving.poisson_submit(srv, jobs, rate, rng)
+        for f in futs:  # one progress line per finished job
+            f.add_done_callback(lambda f: print(json.dumps(
+                {"done": f.result()["name"], "latency_s": round(f.result()["latency"], 3)})
+                if f.exception() is None else json.dumps({"failed": str(f.exception())}),
+                file=sys.stderr, flush=True))
         res = [f.result() for f in futs]
         st = srv.stats()
     st["wall_s"] = time.perf_counter() - t0
