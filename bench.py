@@ -299,6 +299,8 @@ def main():
                              "roofline": roofline(comp, "fp64")}
         out["speedup_vs_fp64_fused"] = head["value"] / comp["value"]
         out["max_abs_diff_z_vs_fp64"] = float(np.max(np.abs(zh - zc)))
+    from tnbp_amd.tci import autotune_report
+    out["autotune"] = autotune_report().splitlines()
     print(json.dumps(out))
 
 

@@ -75,6 +75,13 @@ hbp_status hbp_ctx_memory(hbp_ctx *ctx, size_t *in_use, size_t *peak);
 /* Return the pool's unused device memory to the device (after a job ends) and
  * reset the peak to the bytes still in use.  No reference counterpart. */
 hbp_status hbp_ctx_trim(hbp_ctx *ctx);
+/* Kernel launch variants that compute bitwise the same result (rows per
+ * wave, loads in flight, occupancy) are timed on the first launch of each
+ * shape class and the fastest is kept for the process (HBP_AUTOTUNE=0 in the
+ * environment turns this off).  Writes the choices so far, one line per shape
+ * class, NUL-terminated, into buf (len bytes); needed receives the size the
+ * full report takes.  No reference counterpart. */
+hbp_status hbp_autotune_report(char *buf, size_t len, size_t *needed);
 /* Free and total device memory of a device (hipMemGetInfo). */
 hbp_status hbp_device_memory(int device, size_t *free_bytes, size_t *total_bytes);
 /* Kernel timing (no reference counterpart; the reference times nothing but

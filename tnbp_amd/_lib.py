@@ -38,6 +38,7 @@ SIGNATURES = {
     "hbp_ctx_stream": [_vp, C.POINTER(_vp)],
     "hbp_ctx_memory": [_vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)],
     "hbp_ctx_trim": [_vp],
+    "hbp_autotune_report": [C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)],
     "hbp_device_memory": [C.c_int, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)],
     "hbp_ctx_profile": [_vp, C.c_int],
     "hbp_ctx_profile_read": [_vp, C.c_int, C.POINTER(C.c_int64), _dp, _dp, _dp],

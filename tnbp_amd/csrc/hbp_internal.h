@@ -148,6 +148,8 @@ struct RedJob {
   int nch, n;
   int64_t first_block;
 };
+// the autotuner's choices so far, one line per shape class
+std::string autotune_report();
 int mode_product_col_blocks(int Dout);
 int64_t mode_product_tiles_of(int Dout, int64_t Q);
 template <typename S>

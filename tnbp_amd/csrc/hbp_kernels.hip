@@ -13,6 +13,12 @@
 // what else runs on the device.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <type_traits>
+
 #include <algorithm>
 #include <cstdio>
 
@@ -48,8 +54,9 @@ template <> struct Mfma<float> {
 // Workgroups per CU the mode-product kernel asks the compiler to fit: two for
 // FP64 (its LDS tile takes 66 KB), three for FP32 (33 KB; at four its widest
 // variant spills registers), so that FP32 has more waves per SIMD to hide HBM
-// latency.  The message kernel stays at two: above that its FP32 variants spill.
+// latency.
 template <typename S> struct Occ { static constexpr int v = sizeof(S) == 4 ? 3 : 2; };
+
 
 __device__ inline double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -80,7 +87,7 @@ constexpr int MP_KC = 64;
 // k-step's loads issued before the current step's MFMAs; addresses are
 // clamped in bounds and out-of-range rows/k are zeroed after the load.
 // Tiles b0, b0 + nb, ... of one mode product; er/ei/red are the caller's LDS.
-template <typename S, int NCB, int RB>
+template <typename S, int NCB, int RB, int PFN>
 __device__ __forceinline__ void mode_product_tiles(
     const typename Cplx<S>::type *__restrict__ T, const double2 *__restrict__ E,
     typename Cplx<S>::type *__restrict__ Out, int64_t L, int Din, int Dout, int64_t Q,
@@ -171,13 +178,31 @@ __device__ __forceinline__ void mode_product_tiles(
           }
         }
       };
-      S re0[RB], im0[RB], re1[RB], im1[RB];
-      load(0, re0, im0);
-      for (int k4 = 0; k4 < kend; k4 += 8) {
-        load(k4 + 4, re1, im1);  // k >= Din is zeroed; LDS rows k4+4.. are zero there too
-        step(k4, re0, im0);
-        load(k4 + 8, re0, im0);
-        step(k4 + 4, re1, im1);
+      // a ring of PF k-steps in flight: the loads for k-step s + PF are issued
+      // right after k-step s has used its registers (k >= Din is zeroed, and
+      // the LDS rows beyond kend are zero there too)
+      // PFN = 4: four k-steps in flight; PFN = 0: the next k-step only
+      if constexpr (PFN > 0) {
+        constexpr int PF = PFN;
+        S re[PF][RB], im[PF][RB];
+#pragma unroll
+        for (int p = 0; p < PF; ++p) load(4 * p, re[p], im[p]);
+        for (int k4 = 0; k4 < kend; k4 += 4 * PF) {
+#pragma unroll
+          for (int p = 0; p < PF; ++p) {
+            step(k4 + 4 * p, re[p], im[p]);
+            load(k4 + 4 * p + 4 * PF, re[p], im[p]);
+          }
+        }
+      } else {
+        S re0[RB], im0[RB], re1[RB], im1[RB];
+        load(0, re0, im0);
+        for (int k4 = 0; k4 < kend; k4 += 8) {
+          load(k4 + 4, re1, im1);
+          step(k4, re0, im0);
+          load(k4 + 8, re0, im0);
+          step(k4 + 4, re1, im1);
+        }
       }
     }
 
@@ -210,15 +235,15 @@ __device__ __forceinline__ void mode_product_tiles(
   }
 }
 
-template <typename S, int NCB, int RB>
-__global__ __launch_bounds__(256, Occ<S>::v) void k_mode_product(
+template <typename S, int NCB, int RB, int PFN, int OCC>
+__global__ __launch_bounds__(256, OCC) void k_mode_product(
     const typename Cplx<S>::type *__restrict__ T, const double2 *__restrict__ E,
     typename Cplx<S>::type *__restrict__ Out, int64_t L, int Din, int Dout, int64_t Q,
     int e_trans, S scale, double *__restrict__ norm_partials) {
   __shared__ S er[MP_KC][16 * NCB + 1];
   __shared__ S ei[MP_KC][16 * NCB + 1];
   __shared__ double red[8];
-  mode_product_tiles<S, NCB, RB>(T, E, Out, L, Din, Dout, Q, e_trans, scale,
+  mode_product_tiles<S, NCB, RB, PFN>(T, E, Out, L, Din, Dout, Q, e_trans, scale,
                                  norm_partials ? norm_partials + blockIdx.x : nullptr,
                                  blockIdx.x, gridDim.x, er, ei, red);
 }
@@ -238,7 +263,7 @@ __global__ __launch_bounds__(256, Occ<S>::v) void k_mode_product_grouped(
   }
   const MpJob j = jobs[lo];
   using C = typename Cplx<S>::type;
-  mode_product_tiles<S, NCB, RB>((const C *)j.T, j.E, (C *)j.Out, j.L, j.Din, j.Dout, j.Q, 0,
+  mode_product_tiles<S, NCB, RB, 0>((const C *)j.T, j.E, (C *)j.Out, j.L, j.Din, j.Dout, j.Q, 0,
                                  (S)1, nullptr, (int64_t)blockIdx.x - j.first_block, j.nblocks,
                                  er, ei, red);
 }
@@ -421,8 +446,8 @@ __device__ __forceinline__ void message_chunk(
   }
 }
 
-template <typename S, int NCB>
-__global__ __launch_bounds__(256, 2) void k_message_partials(
+template <typename S, int NCB, int OCC>
+__global__ __launch_bounds__(256, OCC) void k_message_partials(
     const typename Cplx<S>::type *__restrict__ T, const typename Cplx<S>::type *__restrict__ V,
     double2 *__restrict__ partials, int64_t L, int D1, int D2, int64_t Q, int64_t chunk,
     int64_t ldT_R, int64_t ldV_R, int conjV, const double2 *__restrict__ wL,
@@ -864,7 +889,7 @@ inline int grid_for(size_t n, int block = 256, int cap = 4096) {
 }  // namespace
 
 // =================================================================== host
-template <typename S, int NCB, int RB>
+template <typename S, int NCB, int RB, int PFN, int OCC>
 static void mp_launch(hipStream_t st, const void *T, const double2 *E, void *Out, int64_t L,
                       int Din, int Dout, int64_t Q, int e_trans, double scale, double *parts,
                       int *np) {
@@ -873,10 +898,114 @@ static void mp_launch(hipStream_t st, const void *T, const double2 *E, void *Out
   // persistent tiles: E is staged once per workgroup and reused across tiles
   const int64_t nb = std::min<int64_t>(ntiles, 2048);
   using C = typename Cplx<S>::type;
-  hipLaunchKernelGGL((k_mode_product<S, NCB, RB>), dim3((unsigned)nb), dim3(256), 0, st,
-                     (const C *)T, E, (C *)Out, L, Din, Dout, Q, e_trans, (S)scale, parts);
+  hipLaunchKernelGGL((k_mode_product<S, NCB, RB, PFN, OCC>), dim3((unsigned)nb), dim3(256), 0,
+                     st, (const C *)T, E, (C *)Out, L, Din, Dout, Q, e_trans, (S)scale, parts);
   HBP_HIP(hipGetLastError());
   if (np) *np = (int)nb;
+}
+
+// ---------------------------------------------------------------- autotuner
+// Launch variants of one kernel that compute bitwise the same result (row
+// blocks per wave, loads in flight, occupancy target) are timed on the first
+// launch of each shape class and the fastest is kept for the process.  Off
+// with HBP_AUTOTUNE=0 (then variant 0, the default, always runs).
+namespace {
+struct TuneKey {
+  int kind, elem, ncb, qlog, aux;
+  bool operator<(const TuneKey &o) const {
+    return std::tie(kind, elem, ncb, qlog, aux) < std::tie(o.kind, o.elem, o.ncb, o.qlog, o.aux);
+  }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, int> g_tune;
+bool tune_enabled() {
+  static const bool on = [] {
+    const char *e = getenv("HBP_AUTOTUNE");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+int ilog2(int64_t x) {
+  int r = 0;
+  while (x > 1) {
+    x >>= 1;
+    ++r;
+  }
+  return r;
+}
+// run(v) launches variant v on `st`; returns the variant to use for `key`,
+// timing all nv variants (twice each, best of two) the first time.  The
+// chosen variant is launched last, so the output is its output.
+template <typename F>
+void tuned_launch(hipStream_t st, const TuneKey &key, int nv, F run) {
+  if (nv <= 1 || !tune_enabled()) {
+    run(0);
+    return;
+  }
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+      run(it->second);
+      return;
+    }
+  }
+  hipEvent_t a, b;
+  HBP_HIP(hipEventCreate(&a));
+  HBP_HIP(hipEventCreate(&b));
+  int best = 0;
+  float best_ms = 1e30f;
+  for (int v = 0; v < nv; ++v) {
+    for (int rep = 0; rep < 2; ++rep) {
+      HBP_HIP(hipEventRecord(a, st));
+      run(v);
+      HBP_HIP(hipEventRecord(b, st));
+      HBP_HIP(hipEventSynchronize(b));
+      float ms = 0.0f;
+      HBP_HIP(hipEventElapsedTime(&ms, a, b));
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = v;
+      }
+    }
+  }
+  HBP_HIP(hipEventDestroy(a));
+  HBP_HIP(hipEventDestroy(b));
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune[key] = best;
+  }
+  run(best);
+}
+}  // namespace
+
+std::string autotune_report() {
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  std::string out;
+  for (const auto &kv : g_tune) {
+    const TuneKey &k = kv.first;
+    out += std::string(k.kind == 0 ? "mode_product" : "message") + " elem=" +
+           std::to_string(k.elem) + " ncb=" + std::to_string(k.ncb) + " log2Q=" +
+           std::to_string(k.qlog) + " aux=" + std::to_string(k.aux) + " -> variant " +
+           std::to_string(kv.second) + "\n";
+  }
+  return out;
+}
+
+// Mode-product variants for up to 64 output columns: (rows per wave, loads in
+// flight, occupancy) = (32, 1 step, base), (32, 4 steps, base), (16, 1 step,
+// base + 1), (16, 4 steps, base + 1); base is 2 (FP64) or 3 (FP32).
+template <typename S, int NCB>
+static void mp_variant(int v, hipStream_t st, const void *T, const double2 *E, void *Out,
+                       int64_t L, int Din, int Dout, int64_t Q, int e_trans, double scale,
+                       double *parts, int *np) {
+  constexpr int B = Occ<S>::v;
+  switch (v) {
+    case 0: mp_launch<S, NCB, 2, 0, B>(st, T, E, Out, L, Din, Dout, Q, e_trans, scale, parts, np); break;
+    case 1: mp_launch<S, NCB, 2, 4, B>(st, T, E, Out, L, Din, Dout, Q, e_trans, scale, parts, np); break;
+    case 2: mp_launch<S, NCB, 1, 0, B + 1>(st, T, E, Out, L, Din, Dout, Q, e_trans, scale, parts, np); break;
+    default: mp_launch<S, NCB, 1, 4, B + 1>(st, T, E, Out, L, Din, Dout, Q, e_trans, scale, parts, np); break;
+  }
 }
 
 template <typename S>
@@ -888,15 +1017,21 @@ void launch_mode_product(hipStream_t st, const void *T, const double2 *E, void *
   HBP_REQUIRE(Dout <= 128, HBP_ERR_UNSUPPORTED, "mode product: output bond > 128");
   const int64_t Q = L * R;
   const int ncb = (Dout + 15) / 16;
+  constexpr int B = Occ<S>::v;
+  const TuneKey key{0, (int)sizeof(S), ncb, ilog2(Q), (L == 1 ? 1 : 0) + 2 * (Din > MP_KC ? 1 : 0)};
+  auto go = [&](auto ncb_tag, int v) {
+    constexpr int N = decltype(ncb_tag)::value;
+    mp_variant<S, N>(v, st, T, E, Out, L, Din, Dout, Q, e_trans, scale, norm_partials, n_partials);
+  };
   switch (ncb) {
-    case 1: mp_launch<S, 1, 2>(st, T, E, Out, L, Din, Dout, Q, e_trans, scale, norm_partials, n_partials); break;
-    case 2: mp_launch<S, 2, 2>(st, T, E, Out, L, Din, Dout, Q, e_trans, scale, norm_partials, n_partials); break;
-    case 3: mp_launch<S, 3, 2>(st, T, E, Out, L, Din, Dout, Q, e_trans, scale, norm_partials, n_partials); break;
-    case 4: mp_launch<S, 4, 2>(st, T, E, Out, L, Din, Dout, Q, e_trans, scale, norm_partials, n_partials); break;
-    case 5: mp_launch<S, 5, 1>(st, T, E, Out, L, Din, Dout, Q, e_trans, scale, norm_partials, n_partials); break;
-    case 6: mp_launch<S, 6, 1>(st, T, E, Out, L, Din, Dout, Q, e_trans, scale, norm_partials, n_partials); break;
-    case 7: mp_launch<S, 7, 1>(st, T, E, Out, L, Din, Dout, Q, e_trans, scale, norm_partials, n_partials); break;
-    default: mp_launch<S, 8, 1>(st, T, E, Out, L, Din, Dout, Q, e_trans, scale, norm_partials, n_partials); break;
+    case 1: tuned_launch(st, key, 4, [&](int v) { go(std::integral_constant<int, 1>{}, v); }); break;
+    case 2: tuned_launch(st, key, 4, [&](int v) { go(std::integral_constant<int, 2>{}, v); }); break;
+    case 3: tuned_launch(st, key, 4, [&](int v) { go(std::integral_constant<int, 3>{}, v); }); break;
+    case 4: tuned_launch(st, key, 4, [&](int v) { go(std::integral_constant<int, 4>{}, v); }); break;
+    case 5: mp_launch<S, 5, 1, 0, B>(st, T, E, Out, L, Din, Dout, Q, e_trans, scale, norm_partials, n_partials); break;
+    case 6: mp_launch<S, 6, 1, 0, B>(st, T, E, Out, L, Din, Dout, Q, e_trans, scale, norm_partials, n_partials); break;
+    case 7: mp_launch<S, 7, 1, 0, B>(st, T, E, Out, L, Din, Dout, Q, e_trans, scale, norm_partials, n_partials); break;
+    default: mp_launch<S, 8, 1, 0, B>(st, T, E, Out, L, Din, Dout, Q, e_trans, scale, norm_partials, n_partials); break;
   }
 }
 template void launch_mode_product<double>(hipStream_t, const void *, const double2 *, void *,
@@ -976,10 +1111,21 @@ static void msg_launch(hipStream_t st, const void *T, const void *V, double2 *pa
                        const double2 *wR, int Da, int64_t Sb) {
   using C = typename Cplx<S>::type;
   dim3 grid((unsigned)nch, (unsigned)((D1 + MSG_IB - 1) / MSG_IB));
-  hipLaunchKernelGGL((k_message_partials<S, NCB>), grid, dim3(256), 0, st, (const C *)T,
-                     (const C *)V, partials, L, D1, D2, Q, chunk, ldT_R, ldV_R, conjV, wL, wR,
-                     Da, Sb);
-  HBP_HIP(hipGetLastError());
+  // occupancy variants of the same code (FP32 up to 64 columns): 2 or 3
+  // workgroups per CU; the result does not depend on the choice
+  auto run = [&](int v) {
+    if (v == 0 || sizeof(S) != 4 || NCB > 4)
+      hipLaunchKernelGGL((k_message_partials<S, NCB, 2>), grid, dim3(256), 0, st, (const C *)T,
+                         (const C *)V, partials, L, D1, D2, Q, chunk, ldT_R, ldV_R, conjV, wL, wR,
+                         Da, Sb);
+    else
+      hipLaunchKernelGGL((k_message_partials<S, NCB, 3>), grid, dim3(256), 0, st, (const C *)T,
+                         (const C *)V, partials, L, D1, D2, Q, chunk, ldT_R, ldV_R, conjV, wL, wR,
+                         Da, Sb);
+    HBP_HIP(hipGetLastError());
+  };
+  const TuneKey key{1, (int)sizeof(S), NCB, ilog2(Q), (L == 1 ? 1 : 0) + 2 * (wL || wR ? 1 : 0)};
+  tuned_launch(st, key, (sizeof(S) == 4 && NCB <= 4) ? 2 : 1, run);
 }
 
 template <typename S>

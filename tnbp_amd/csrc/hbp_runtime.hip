@@ -538,6 +538,18 @@ hbp_status hbp_ctx_memory(hbp_ctx *ctx, size_t *in_use, size_t *peak) {
   HBP_API_END
 }
 
+hbp_status hbp_autotune_report(char *buf, size_t len, size_t *needed) {
+  HBP_API_BEGIN
+  const std::string r = autotune_report();
+  if (needed) *needed = r.size() + 1;
+  if (buf && len > 0) {
+    const size_t n = std::min(len - 1, r.size());
+    std::memcpy(buf, r.data(), n);
+    buf[n] = 0;
+  }
+  HBP_API_END
+}
+
 hbp_status hbp_ctx_trim(hbp_ctx *ctx) {
   HBP_API_BEGIN
   if (ctx->scratch) {

@@ -43,6 +43,16 @@ class Context:
         check(self.lib.hbp_ctx_trim(self.h))
 
 
+def autotune_report():
+    """The kernel autotuner's choices so far (one line per shape class)."""
+    lib = _lib.load()
+    need = C.c_size_t()
+    check(lib.hbp_autotune_report(None, 0, C.byref(need)))
+    buf = C.create_string_buffer(need.value + 1)
+    check(lib.hbp_autotune_report(buf, need.value + 1, C.byref(need)))
+    return buf.value.decode()
+
+
 def device_memory(device: int = 0):
     """(free, total) bytes of a device."""
     a, b = C.c_size_t(), C.c_size_t()
