@@ -35,7 +35,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "hbm": 8.0e12}
-KERNELS = {0: "k_mode_product", 1: "k_message_partials"}
+KERNELS = {0: "k_mode_product", 1: "k_message_partials", 2: "k_product_message"}
 
 
 def parse():
@@ -48,8 +48,9 @@ def parse():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
     ap.add_argument("--mode", choices=["fused", "plain"], default="fused",
                     help="plain: the reference's op sequence one TCI call at a time")
-    ap.add_argument("--sweep", choices=["auto", "per-site", "grouped"], default="auto",
-                    help="BP sweep launch form (grouped needs bond dims <= 32)")
+    ap.add_argument("--sweep", choices=["auto", "per-site", "grouped", "merged"], default="auto",
+                    help="BP sweep launch form (grouped needs bond dims <= 32; merged computes "
+                         "each message's last mode product inside the message kernel)")
     ap.add_argument("--compare", type=int, default=1,
                     help="also time the FP64 fused path on the same steps")
     ap.add_argument("--tol", type=float, default=1e-10)
@@ -156,7 +157,7 @@ def run_path(args, precision, mode, want_cpu_sample=False):
     ctx = Context(0)
     peps = InitTensorProductState(ctx, edges, [2] * len(sites), prec)
     check(ctx.lib.hbp_peps_set_mode(peps.h, 0 if mode == "fused" else 1))
-    peps.set_sweep_kernel({"auto": 0, "per-site": 1, "grouped": 2}[args.sweep])
+    peps.set_sweep_kernel({"auto": 0, "per-site": 1, "grouped": 2, "merged": 3}[args.sweep])
 
     def step(record):
         t0 = time.perf_counter()

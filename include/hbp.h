@@ -89,8 +89,10 @@ hbp_status hbp_device_memory(int device, size_t *free_bytes, size_t *total_bytes
  * bracketed by HIP events on the context stream.  profile_read synchronises,
  * returns launch count, summed duration, and the algorithmic flops and bytes
  * of all recorded launches of one kind, and forgets them.  Kinds:
- * HBP_PROF_MODE_PRODUCT, HBP_PROF_MESSAGE (partials + fixed-order reduce). */
-enum { HBP_PROF_MODE_PRODUCT = 0, HBP_PROF_MESSAGE = 1 };
+ * HBP_PROF_MODE_PRODUCT, HBP_PROF_MESSAGE (partials + fixed-order reduce),
+ * HBP_PROF_PRODUCT_MESSAGE (mode product merged into the message kernel,
+ * partials + reduce). */
+enum { HBP_PROF_MODE_PRODUCT = 0, HBP_PROF_MESSAGE = 1, HBP_PROF_PRODUCT_MESSAGE = 2 };
 hbp_status hbp_ctx_profile(hbp_ctx *ctx, int enable);
 hbp_status hbp_ctx_profile_read(hbp_ctx *ctx, int kind, int64_t *count, double *total_ms,
                                 double *flops, double *bytes);
@@ -194,13 +196,17 @@ enum { HBP_ABSORB_AUTO = 0, HBP_ABSORB_LAZY = 1, HBP_ABSORB_EAGER = 2 };
 hbp_status hbp_peps_set_absorb_mode(hbp_peps *p, int mode);
 /* FUSED mode: how a BP sweep is launched.  PER_SITE issues each site's mode
  * products and messages as separate launches (efficient once bonds reach
- * ~64); GROUPED runs each level of the sweep for all sites (or a group of
+ * ~64); MERGED is PER_SITE with the last mode product of each outgoing
+ * message computed inside the message kernel (bonds in (32, 64] on both of
+ * the two bonds involved; other messages as PER_SITE), which saves writing
+ * and re-reading that product; GROUPED runs each level of the sweep for all sites (or a group of
  * sites that fits a workspace of min(free/4, 16 GiB)) as one launch per
  * column-block count - five to ten launches per sweep instead of ~12 per
  * site; needs site degree <= 4 and bond dims <= 32.  AUTO (default) is
- * PER_SITE: GROUPED is experimental and taken only on request.  No reference
+ * MERGED (which falls back to PER_SITE launches wherever bonds are outside
+ * (32, 64]): GROUPED is experimental and taken only on request.  No reference
  * counterpart. */
-enum { HBP_SWEEP_AUTO = 0, HBP_SWEEP_PER_SITE = 1, HBP_SWEEP_GROUPED = 2 };
+enum { HBP_SWEEP_AUTO = 0, HBP_SWEEP_PER_SITE = 1, HBP_SWEEP_GROUPED = 2, HBP_SWEEP_MERGED = 3 };
 hbp_status hbp_peps_set_sweep_kernel(hbp_peps *p, int kind);
 hbp_status hbp_peps_num_sites(const hbp_peps *p, int *n_sites, int *n_edges);
 /* Shape of site tensor s (order = degree + 1) and of message m (2 x D). */

@@ -4,6 +4,7 @@ Tolerances (stated per test): the plain FP64 path must match the oracle to
 1e-10 relative per expectation value (floor 1 for values near zero); the mixed
 path to 1e-5.
 """
+import ctypes
 import json
 import math
 import os
@@ -263,6 +264,66 @@ def test_grouped_sweep_matches_per_site(ctx, precision, tol):
     assert rel_close(out[1][1], out[0][1], tol) <= 1.0
     assert rel_close(out[1][2], out[0][2], tol) <= 1.0
     np.testing.assert_allclose(out[1][3], out[0][3], atol=tol)
+
+
+@pytest.mark.parametrize("precision,tol,res_rtol,res_atol", [(0, 1e-10, 1e-9, 1e-13),
+                                                            (1, 2e-5, 0.0, 5e-8)])
+def test_merged_sweep_matches_per_site(ctx, precision, tol, res_rtol, res_atol, tmp_path):
+    """Merged product-message kernels (the last mode product of each message
+    inside the message kernel) against separate launches, from one state: a
+    4x4 kicked-Ising PEPS grown to chi = 20 and absorbed once more (bonds of
+    40, the merged kernel's zero-padded case), checkpointed and restored into a
+    second PEPS; then the same BP iterations with either sweep.  Starting from
+    one state keeps the bond gauge common, so the messages themselves compare
+    (after separate truncations only gauge-invariant quantities would)."""
+    from tnbp_amd import circuits as pc
+    from tnbp_amd._lib import check
+    from tnbp_amd.peps import (HBP_SWEEP_MERGED, HBP_SWEEP_PER_SITE, AbsorbTPO,
+                               InitTensorProductState, Measure, MeasureEdges, Truncation, bp_loop,
+                               load_peps, save_peps)
+    announce(f"merged vs per-site sweep precision {precision}")
+    edges = pc.square_lattice(4, 4)
+    tpo = pc.kicked_ising_tpo(edges, 0.125, 0.0, 0.11, math.pi)
+    zop = pc.op_tensor(pc.PAULI["Z"])
+    zzop = pc.op_tensor(np.kron(pc.PAULI["Z"], pc.PAULI["Z"]))
+    p = InitTensorProductState(ctx, edges, [2] * 16, precision)
+    for _ in range(6):
+        AbsorbTPO(ctx, tpo, p)
+        bp_loop(ctx, p, 30, 1e-9)
+        Truncation(ctx, p, 20, 1e-12)
+    AbsorbTPO(ctx, tpo, p)
+    path = tmp_path / "merged.hbpp"
+    save_peps(p, path)
+    q = load_peps(ctx, path)
+    assert max(max(p.site_shape(s)[:-1]) for s in range(16)) == 40
+    out = []
+    for st, kind in ((p, HBP_SWEEP_PER_SITE), (q, HBP_SWEEP_MERGED)):
+        st.set_sweep_kernel(kind)
+        check(ctx.lib.hbp_ctx_profile(ctx.h, 1))
+        it, r = bp_loop(ctx, st, 6, 0.0)
+        counts = []
+        for k in (0, 1, 2):  # reading a kind also releases its records
+            n = ctypes.c_int64()
+            check(ctx.lib.hbp_ctx_profile_read(ctx.h, k, ctypes.byref(n), None, None, None))
+            counts.append(n.value)
+        check(ctx.lib.hbp_ctx_profile(ctx.h, 0))
+        msgs = [st.get_message(m) for m in range(2 * len(edges))]
+        Truncation(ctx, st, 20, 1e-12)
+        z = Measure(ctx, st, list(range(16)), [zop] * 16)
+        zz = MeasureEdges(ctx, st, edges, [zzop] * len(edges))
+        out.append((r, msgs, z, zz, counts[2]))
+        st.close()
+    # only the merged sweep runs merged launches (7 sweeps x at least one per site)
+    assert out[0][4] == 0 and out[1][4] >= 7 * 4
+    # mixed precision may stop one iteration apart at its 1e-6 residual floor;
+    # its residuals near the floor carry FP32 rounding (absolute tolerance)
+    n = min(len(out[0][0]), len(out[1][0]))
+    assert n >= 2
+    np.testing.assert_allclose(out[1][0][:n], out[0][0][:n], rtol=res_rtol, atol=res_atol)
+    for ma, mb in zip(out[0][1], out[1][1]):
+        np.testing.assert_allclose(mb, ma, atol=tol)
+    assert rel_close(out[1][2], out[0][2], tol) <= 1.0
+    assert rel_close(out[1][3], out[0][3], tol) <= 1.0
 
 
 def test_checkpoint_round_trip(ctx, tmp_path):

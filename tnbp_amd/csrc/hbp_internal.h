@@ -114,8 +114,24 @@ int message_partials(hipStream_t st, const void *T, const void *V, double2 *part
 void launch_build_weights(hipStream_t st, const double2 *E, double2 *w, int64_t n, int nb,
                           const int64_t *dims, const int64_t *diag_off);
 int64_t message_num_chunks(int64_t Q);
-// M = sum over chunks (fixed order), complex<double>.
+// Partials of M[i, i'] = sum_{o, a'} (sum_a P[o; i, a] E(a, a')) conj V[o; i', a']
+// for adjacent bonds k (i, dim Dk) and j (a, dim Dj), both <= 64, of two
+// column-major tensors of the same shape viewed as (L, i, a, R) when k_first
+// and (L, a, i, R) otherwise, E(a, a') = E[a + Dj a'].  Returns the number of
+// partials written (each Dk*Dk complex of scalar S, at most max_blocks).
+template <typename S>
+int product_message_partials(hipStream_t st, const void *P, const void *V, const double2 *E,
+                             void *partials, int64_t L, int64_t R, int Dk, int Dj,
+                             int k_first, int max_blocks);
+// Y[c + Cn (l + L r)] = X[l + L (c + Cn r)] for complex elements of elem_bytes
+// (8 or 16): moves the middle index block of an (L, Cn, R) view to the front.
+void launch_swap_blocks(hipStream_t st, const void *X, void *Y, int64_t L, int64_t Cn,
+                        int64_t R, int elem_bytes);
+// M = sum over chunks (fixed order), complex<double>; complex<float>
+// partials are summed in double.
 void launch_reduce_partials(hipStream_t st, const double2 *partials, int nchunks, int n,
+                            double2 *M);
+void launch_reduce_partials(hipStream_t st, const float2 *partials, int nchunks, int n,
                             double2 *M);
 
 // ---- grouped launches: many independent products in one kernel launch.

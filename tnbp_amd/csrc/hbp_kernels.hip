@@ -484,6 +484,218 @@ __global__ __launch_bounds__(256, 2) void k_message_grouped(const MsgGJob *__res
                         j.L * j.D2, 1, nullptr, nullptr, max(j.D1, j.D2), 0, tre, tim, vre, vim);
 }
 
+// ------------------------------------------------ product + message, merged
+// For two adjacent bonds k (index i) and j (index a) of column-major site
+// tensors P and V of the same shape, with o = (l, r) the remaining index,
+//   M[i, i'] = sum_{o, a'} T_o[i, a'] conj V_o[i', a'],
+//   T_o[i, a'] = sum_a P_o[i, a] E(a, a'),       E(a, a') = E[a + Dj a'],
+// i.e. the last mode product of an outgoing message (bp.h:148/150) and the
+// message itself (bp.h:165) in one pass: the product tile T_o never leaves
+// registers, which saves writing and re-reading the whole product tensor.
+// Per o the workgroup runs two 64 x 64 x 64 complex products: wave w owns rows
+// i in [16 w, 16 w + 16).  The first computes T_o^T = E^T P_o^T (E from LDS
+// as the A operand, the wave's rows of P_o straight from global memory as B),
+// so that register v of lane (li, lk) holds T_o[16 w + li][16 cb + row(lk, v)]
+// - exactly the B-operand layout of the second product
+// M^T += conj(V_o) T_o^T, contracted over a' in the order the first product
+// left it in registers; V_o is staged through LDS for all four waves.  Bonds
+// up to 64 (zero padded).  Blocks walk o = n0, n0 + nb, ... and each writes
+// one partial; the fixed-order reduction makes the result reproducible.
+constexpr int PM_D = 64;
+template <typename S> struct OccPM { static constexpr int v = sizeof(S) == 4 ? 2 : 1; };
+
+// contraction index of step s (0..15) for lane group lk: each lane's four
+// consecutive steps read four consecutive a (one 32-byte run when a is the
+// contiguous index)
+__device__ __forceinline__ int pm_a(int s, int lk) { return 16 * (s >> 2) + 4 * lk + (s & 3); }
+
+// LAY: 0 general strides, 1 a contiguous (L = 1, j before k), 2 i contiguous
+// (L = 1, k before j).  FULL: Dk = Dj = 64, no bounds masks.  Offsets inside
+// one o slice are 32-bit byte offsets from the slice base (the host checks
+// that a slice spans less than 2^31 bytes).
+template <typename S, int LAY, bool FULL>
+__global__ __launch_bounds__(256, OccPM<S>::v) void k_product_message(
+    const typename Cplx<S>::type *__restrict__ P, const typename Cplx<S>::type *__restrict__ V,
+    const double2 *__restrict__ E, typename Cplx<S>::type *__restrict__ partials, int64_t L,
+    int64_t nO, int Dk_rt, int Dj_rt, int64_t si, int64_t sa_rt, int64_t sr) {
+  using C = typename Cplx<S>::type;
+  using M = Mfma<S>;
+  using v4 = typename M::v4;
+  __shared__ C Es[PM_D][PM_D];
+  __shared__ C Vs[PM_D][PM_D + 1];
+  constexpr bool AC = LAY == 1;
+  const int Dk = FULL ? PM_D : Dk_rt, Dj = FULL ? PM_D : Dj_rt;
+  const int64_t sa = AC ? 1 : sa_rt;
+  if (LAY == 2) si = 1;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int nb = gridDim.x, b = blockIdx.x;
+  // workgroups go round-robin to the 8 XCDs: the blocks of one XCD take
+  // consecutive o, so that cache lines shared by neighbouring o (l runs
+  // fastest) are read while they sit in that XCD's L2
+  const int64_t n0 = (nb % 8 == 0) ? (int64_t)(b % 8) * (nb / 8) + b / 8 : b;
+
+  for (int idx = tid; idx < PM_D * PM_D; idx += 256) {
+    const int a = idx & (PM_D - 1), a2 = idx / PM_D;
+    double2 e = make_double2(0.0, 0.0);
+    if (FULL || (a < Dj && a2 < Dj)) e = E[a + (int64_t)Dj * a2];
+    C c;
+    c.x = (S)e.x;
+    c.y = (S)e.y;
+    Es[a][a2] = c;
+  }
+
+  const int i = wave * 16 + li;
+  const bool iok = FULL || i < Dk;
+  // V staging runs along the index with the smaller stride
+  const bool afirst = AC || sa < si;
+  const uint32_t si_b = (uint32_t)(si * sizeof(C)), sa_b = (uint32_t)(sa * sizeof(C));
+  C pc[16], vr[16];
+  auto slice = [&](const C *X, int64_t o) {
+    const int64_t l = o % L, r = o / L;
+    return (const char *)(X + (l + sr * r));
+  };
+  auto load_p = [&](int64_t o) {
+    const char *base = slice(P, o);
+    const uint32_t ib = (uint32_t)(iok ? i : 0) * si_b;
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      const int a = pm_a(s, lk);
+      const bool ok = FULL || (iok && a < Dj);
+      const C x = *(const C *)(base + (ib + (uint32_t)(ok ? a : 0) * sa_b));
+      pc[s] = ok ? x : C{0, 0};
+    }
+  };
+  auto vidx = [&](int n, int &ii, int &aa) {
+    const int idx = tid + 256 * n;
+    ii = afirst ? idx / PM_D : idx & (PM_D - 1);
+    aa = afirst ? idx & (PM_D - 1) : idx / PM_D;
+  };
+  auto load_v = [&](int64_t o) {
+    const char *base = slice(V, o);
+#pragma unroll
+    for (int n = 0; n < 16; ++n) {
+      int ii, aa;
+      vidx(n, ii, aa);
+      const bool ok = FULL || (ii < Dk && aa < Dj);
+      const C x = *(const C *)(base + ((uint32_t)(ok ? ii : 0) * si_b +
+                                       (uint32_t)(ok ? aa : 0) * sa_b));
+      vr[n] = ok ? x : C{0, 0};
+    }
+  };
+  auto commit_v = [&]() {
+#pragma unroll
+    for (int n = 0; n < 16; ++n) {
+      int ii, aa;
+      vidx(n, ii, aa);
+      Vs[aa][ii] = vr[n];
+    }
+  };
+
+  v4 m_re[4], m_im[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    m_re[c] = v4{0, 0, 0, 0};
+    m_im[c] = v4{0, 0, 0, 0};
+  }
+  int64_t o = n0;
+  if (o < nO) {
+    load_p(o);
+    load_v(o);
+  }
+  for (; o < nO; o += nb) {
+    const bool more = o + nb < nO;
+    __syncthreads();  // E staged; every wave done with the previous V_o
+    commit_v();
+    __syncthreads();
+    if (more) load_v(o + nb);  // in flight until the next commit
+    v4 t_re[4], t_im[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      t_re[c] = v4{0, 0, 0, 0};
+      t_im[c] = v4{0, 0, 0, 0};
+    }
+    // T_o^T = E^T P_o^T:  T = P E = (Pr Er - Pi Ei) + i (Pr Ei + Pi Er)
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      const int a = pm_a(s, lk);
+      const S pr = pc[s].x, pi = pc[s].y, npi = -pc[s].y;
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) {
+        const C e = Es[a][cb * 16 + li];
+        t_re[cb] = M::op(e.x, pr, t_re[cb]);
+        t_im[cb] = M::op(e.x, pi, t_im[cb]);
+        t_re[cb] = M::op(e.y, npi, t_re[cb]);
+        t_im[cb] = M::op(e.y, pr, t_im[cb]);
+      }
+    }
+    if (more) load_p(o + nb);  // registers free again; in flight during the second product
+    // M^T += conj(V_o) T_o^T over a' = 16 cb + row(lk, v):
+    // conj(V) T = (Vr Tr + Vi Ti) + i (Vr Ti - Vi Tr)
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int a2 = cb * 16 + M::row(lk, v);
+        const S tr = t_re[cb][v], ti = t_im[cb][v], ntr = -t_re[cb][v];
+#pragma unroll
+        for (int c2 = 0; c2 < 4; ++c2) {
+          const C x = Vs[a2][c2 * 16 + li];
+          m_re[c2] = M::op(x.x, tr, m_re[c2]);
+          m_im[c2] = M::op(x.x, ti, m_im[c2]);
+          m_re[c2] = M::op(x.y, ti, m_re[c2]);
+          m_im[c2] = M::op(x.y, ntr, m_im[c2]);
+        }
+      }
+    }
+  }
+
+  // register v of lane (li, lk) of m[c2]: M[i][i'] with i' = 16 c2 + row(lk, v)
+  // partials in the accumulators' own precision (float2 for FP32)
+  C *out = partials + (int64_t)b * Dk * Dk;
+  if (iok) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+#pragma unroll
+      for (int c2 = 0; c2 < 4; ++c2) {
+        const int i2 = c2 * 16 + M::row(lk, v);
+        if (!FULL && i2 >= Dk) continue;
+        C x;
+        x.x = m_re[c2][v];
+        x.y = m_im[c2][v];
+        out[i + (int64_t)Dk * i2] = x;
+      }
+    }
+  }
+}
+
+// Y[c + C (l + L r)] = X[l + L (c + C r)]: R column-major L x C matrices
+// transposed through 64 x 64 LDS tiles (reads along l, writes along c), so
+// that a pair of bonds behind a block of L leading indices moves to the front.
+template <typename T>
+__global__ __launch_bounds__(256) void k_swap_blocks(const T *__restrict__ X, T *__restrict__ Y,
+                                                    int64_t L, int64_t Cn, int64_t R) {
+  __shared__ T tile[64][65];
+  const int64_t tl = (L + 63) / 64, tc = (Cn + 63) / 64, ntiles = tl * tc * R;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int64_t r = t / (tl * tc), rem = t - r * tl * tc;
+    const int64_t c0 = (rem / tl) * 64, l0 = (rem % tl) * 64;
+    const T *xr = X + L * Cn * r;
+    T *yr = Y + L * Cn * r;
+    __syncthreads();
+    for (int cc = ty; cc < 64; cc += 4) {
+      const int64_t l = l0 + tx, c = c0 + cc;
+      if (l < L && c < Cn) tile[cc][tx] = xr[l + L * c];
+    }
+    __syncthreads();
+    for (int ll = ty; ll < 64; ll += 4) {
+      const int64_t c = c0 + tx, l = l0 + ll;
+      if (l < L && c < Cn) yr[c + Cn * l] = tile[tx][ll];
+    }
+  }
+}
+
 // w[i] = prod_b E_b[c_b, c_b] over the column-major decomposition of i into
 // (dims[0], ..., dims[nb-1], unit); the trailing unit factor covers indices
 // that carry no message (the physical bond).
@@ -514,7 +726,10 @@ __global__ void k_build_weights(const double2 *__restrict__ E, double2 *__restri
 // each sum the chunks g, g + 16, ... in order and the 16 group sums are added
 // in a fixed order, so the result is bitwise reproducible.
 constexpr int RED_E = RED_ENTRIES, RED_G = 16;
-__device__ __forceinline__ void reduce_entries(const double2 *__restrict__ p, int nchunks,
+// Partials of FP32 accumulators may be stored as float2: the conversion to
+// double is exact, so the sums equal those of double2 partials bitwise.
+template <typename PT = double2>
+__device__ __forceinline__ void reduce_entries(const PT *__restrict__ p, int nchunks,
                                                int n, double2 *__restrict__ M, int64_t blk) {
   __shared__ double sre[RED_G][RED_E + 1], sim[RED_G][RED_E + 1];
   const int el = threadIdx.x % RED_E, g = threadIdx.x / RED_E;
@@ -522,9 +737,9 @@ __device__ __forceinline__ void reduce_entries(const double2 *__restrict__ p, in
   double re = 0.0, im = 0.0;
   if (e < n) {
     for (int c = g; c < nchunks; c += RED_G) {
-      const double2 v = p[(int64_t)c * n + e];
-      re += v.x;
-      im += v.y;
+      const PT v = p[(int64_t)c * n + e];
+      re += (double)v.x;
+      im += (double)v.y;
     }
   }
   sre[g][el] = re;
@@ -540,10 +755,10 @@ __device__ __forceinline__ void reduce_entries(const double2 *__restrict__ p, in
   }
 }
 
-__global__ __launch_bounds__(256) void k_reduce_partials(const double2 *__restrict__ p,
-                                                          int nchunks, int n,
-                                                          double2 *__restrict__ M) {
-  reduce_entries(p, nchunks, n, M, blockIdx.x);
+template <typename PT>
+__global__ __launch_bounds__(256) void k_reduce_partials(const PT *__restrict__ p, int nchunks,
+                                                          int n, double2 *__restrict__ M) {
+  reduce_entries<PT>(p, nchunks, n, M, blockIdx.x);
 }
 
 __global__ __launch_bounds__(256) void k_reduce_grouped(const RedJob *__restrict__ jobs,
@@ -1164,6 +1379,63 @@ template int message_partials<float>(hipStream_t, const void *, const void *, do
                                      int64_t, int, int, int64_t, int64_t, int64_t, int64_t,
                                      int, const double2 *, const double2 *, int, int64_t);
 
+template <typename S>
+int product_message_partials(hipStream_t st, const void *P, const void *V, const double2 *E,
+                             void *partials, int64_t L, int64_t R, int Dk, int Dj,
+                             int k_first, int max_blocks) {
+  HBP_REQUIRE(Dk >= 1 && Dk <= PM_D && Dj >= 1 && Dj <= PM_D, HBP_ERR_UNSUPPORTED,
+              "merged product-message: bonds must be <= 64");
+  const int64_t nO = L * R;
+  HBP_REQUIRE(nO >= 1 && max_blocks >= 1, HBP_ERR_INVALID_ARGUMENT,
+              "merged product-message: empty problem");
+  // (l, i, a, r) for k before j, (l, a, i, r) otherwise
+  const int64_t si = k_first ? L : L * Dj;
+  const int64_t sa = k_first ? L * Dk : L;
+  const int64_t sr = L * Dk * Dj;
+  // one pass of workgroups (two per CU on 256 CUs), a multiple of the 8 XCDs
+  int64_t nb = std::min<int64_t>(std::min<int64_t>(max_blocks, 512), nO);
+  if (nb >= 8) nb -= nb % 8;
+  using C = typename Cplx<S>::type;
+  HBP_REQUIRE(sr * (int64_t)sizeof(C) < ((int64_t)1 << 31), HBP_ERR_UNSUPPORTED,
+              "merged product-message: slice beyond 2 GiB");
+  const int lay = sa == 1 ? 1 : si == 1 ? 2 : 0;
+  const bool full = Dk == PM_D && Dj == PM_D;
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(256), 0, st, (const C *)P, (const C *)V, E,
+                       (C *)partials, L, nO, Dk, Dj, si, sa, sr);
+  };
+  if (full) {
+    if (lay == 1) go(k_product_message<S, 1, true>);
+    else if (lay == 2) go(k_product_message<S, 2, true>);
+    else go(k_product_message<S, 0, true>);
+  } else {
+    if (lay == 1) go(k_product_message<S, 1, false>);
+    else if (lay == 2) go(k_product_message<S, 2, false>);
+    else go(k_product_message<S, 0, false>);
+  }
+  HBP_HIP(hipGetLastError());
+  return (int)nb;
+}
+void launch_swap_blocks(hipStream_t st, const void *X, void *Y, int64_t L, int64_t Cn,
+                        int64_t R, int elem_bytes) {
+  const int64_t ntiles = ((L + 63) / 64) * ((Cn + 63) / 64) * R;
+  const unsigned grid = (unsigned)std::min<int64_t>(ntiles, 8192);
+  if (elem_bytes == 16)
+    hipLaunchKernelGGL(k_swap_blocks<double2>, dim3(grid), dim3(256), 0, st, (const double2 *)X,
+                       (double2 *)Y, L, Cn, R);
+  else
+    hipLaunchKernelGGL(k_swap_blocks<float2>, dim3(grid), dim3(256), 0, st, (const float2 *)X,
+                       (float2 *)Y, L, Cn, R);
+  HBP_HIP(hipGetLastError());
+}
+
+template int product_message_partials<double>(hipStream_t, const void *, const void *,
+                                              const double2 *, void *, int64_t, int64_t, int, int,
+                                              int, int);
+template int product_message_partials<float>(hipStream_t, const void *, const void *,
+                                             const double2 *, void *, int64_t, int64_t, int, int,
+                                             int, int);
+
 void launch_build_weights(hipStream_t st, const double2 *E, double2 *w, int64_t n, int nb,
                           const int64_t *dims, const int64_t *diag_off) {
   WeightArgs a{};
@@ -1178,7 +1450,14 @@ void launch_build_weights(hipStream_t st, const double2 *E, double2 *w, int64_t 
 
 void launch_reduce_partials(hipStream_t st, const double2 *partials, int nchunks, int n,
                             double2 *M) {
-  hipLaunchKernelGGL(k_reduce_partials, dim3((n + RED_E - 1) / RED_E), dim3(256), 0, st,
+  hipLaunchKernelGGL(k_reduce_partials<double2>, dim3((n + RED_E - 1) / RED_E), dim3(256), 0, st,
+                     partials, nchunks, n, M);
+  HBP_HIP(hipGetLastError());
+}
+
+void launch_reduce_partials(hipStream_t st, const float2 *partials, int nchunks, int n,
+                            double2 *M) {
+  hipLaunchKernelGGL(k_reduce_partials<float2>, dim3((n + RED_E - 1) / RED_E), dim3(256), 0, st,
                      partials, nchunks, n, M);
   HBP_HIP(hipGetLastError());
 }

@@ -19,6 +19,7 @@
 #include <cstring>
 #include <map>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 
 #include "hbp_internal.h"
@@ -700,6 +701,44 @@ static void message_raw(Ctx *ctx, const void *T, const void *V, const int32_t *s
   ctx->prof_end(pr);
 }
 
+// Merged form of T = P x_j E followed by the message on bond k (k, j adjacent):
+// the product stays in registers inside the message kernel.
+static bool merge_ok(const Peps *p, const int32_t *sh, int k, int j) {
+  if (p->sweep_kernel != HBP_SWEEP_MERGED && p->sweep_kernel != HBP_SWEEP_AUTO) return false;
+  return (k - j == 1 || j - k == 1) && sh[k] > 32 && sh[k] <= 64 && sh[j] > 32 && sh[j] <= 64;
+}
+
+// P, V viewed as (L, i, a, R) (k_first) or (L, a, i, R)
+template <typename S>
+static void product_message(Ctx *ctx, const void *P, const double2 *E, const void *V, int64_t L,
+                            int64_t R, int Dk, int Dj, int k_first, double2 *M) {
+  constexpr int kMaxBlocks = 512;
+  using C = std::conditional_t<sizeof(S) == 4, float2, double2>;
+  C *parts = (C *)ctx->get_scratch((size_t)kMaxBlocks * Dk * Dk * sizeof(C) + 64);
+  const double n = (double)L * R * Dk * Dj;
+  const int pr = ctx->prof_begin(HBP_PROF_PRODUCT_MESSAGE, 8.0 * n * (Dk + Dj),
+                                 2.0 * sizeof(S) * 2.0 * n + 16.0 * Dj * Dj);
+  const int nb = product_message_partials<S>(ctx->stream, P, V, E, parts, L, R, Dk, Dj, k_first,
+                                             kMaxBlocks);
+  launch_reduce_partials(ctx->stream, parts, nb, Dk * Dk, M);
+  ctx->prof_end(pr);
+}
+
+// Both messages of the target pair (t, t + 1) of site s from the half-product
+// P (the site with every bond absorbed but the pair's): for k in the pair,
+// M_k = <P x_j E_j, V> over all bonds but k, j the other one.  P and V have
+// the pair in front: (D_t, D_{t+1}, R) views.
+template <typename S>
+static void merged_pair(Ctx *ctx, Peps *p, int s, const void *P, const void *V,
+                        const int32_t *sh, int t, int64_t R, double2 *raw,
+                        const std::vector<int64_t> &raw_off) {
+  for (int k = t; k <= t + 1; ++k) {
+    const int j = k == t ? t + 1 : t;
+    product_message<S>(ctx, P, p->E + p->E_off[p->in_msg(s, j)], V, 1, R, sh[k], sh[j],
+                       k == t ? 1 : 0, raw + raw_off[p->out_msg(s, k)]);
+  }
+}
+
 // T = V_s with every incoming message absorbed except on bond `skip` (-1: none),
 // returns the buffer holding it (V_s itself when nothing is absorbed).
 template <typename S>
@@ -752,6 +791,35 @@ static void sweep_raw(Ctx *ctx, Peps *p, double2 *raw, const std::vector<int64_t
       const void *Vs = site_data(ctx, p, s);
       const void *cur = Vs;
       int nprod = a1 - a0, w = (nprod % 2 == 1) ? 0 : 1;
+      if (t1 - t0 == 2 && merge_ok(p, sh, t0, t0 + 1)) {
+        // merged messages want the target pair in front (whole contiguous
+        // D_t x D_{t+1} slices): when bonds precede it, rearrange V once into
+        // buf[2] as (pair, leading bonds, rest) and absorb the other bonds there
+        int64_t Lb = 1, R = 1;
+        for (int d = 0; d < t0; ++d) Lb *= sh[d];
+        for (int d = t0 + 2; d < order; ++d) R *= sh[d];
+        int32_t shp[HBP_MAX_ORDER];
+        int pos[HBP_MAX_ORDER];
+        for (int d = 0; d < order; ++d) {
+          pos[d] = d < t0 ? d + 2 : d <= t0 + 1 ? d - t0 : d;
+          shp[pos[d]] = sh[d];
+        }
+        if (Lb > 1) {
+          launch_swap_blocks(ctx->stream, Vs, p->buf[2], Lb, (int64_t)sh[t0] * sh[t0 + 1], R,
+                             (int)(2 * sizeof(S)));
+          Vs = p->buf[2];
+          cur = Vs;
+        }
+        for (int j = a0; j < a1; ++j) {
+          void *dst = p->buf[w];
+          mode_prod<S>(ctx, cur, p->E + p->E_off[p->in_msg(s, j)], dst, shp, order, pos[j], sh[j],
+                       0);
+          cur = dst;
+          w ^= 1;
+        }
+        merged_pair<S>(ctx, p, s, cur, Vs, sh, t0, Lb * R, raw, raw_off);
+        continue;
+      }
       for (int j = a0; j < a1; ++j) {
         void *dst = p->buf[w];
         mode_prod<S>(ctx, cur, p->E + p->E_off[p->in_msg(s, j)], dst, sh, order, j, sh[j], 0);
@@ -1096,8 +1164,9 @@ static void sweep(Ctx *ctx, Peps *p, double2 *newE, bool with_residual, double *
                   "grouped sweep: needs site degree <= 4 and bond dims <= 32");
       grouped = true;
     }
-    // AUTO runs the per-site launches: the grouped form is taken only on request
-    // (the first GPU run that used it left the device unusable, cause not found)
+    // AUTO runs the per-site launches with merged product-message kernels where
+    // the bonds allow: the grouped form is taken only on request (the first GPU
+    // run that used it left the device unusable, cause not found)
     if (p->precision == HBP_PREC_FP64) {
       if (grouped) sweep_raw_grouped<double>(ctx, p, raw, p->E_off);
       else sweep_raw<double>(ctx, p, raw, p->E_off);
@@ -1901,7 +1970,8 @@ hbp_status hbp_peps_set_mode(hbp_peps *p, int mode) {
 
 hbp_status hbp_peps_set_sweep_kernel(hbp_peps *p, int kind) {
   API_BEGIN
-  HBP_REQUIRE(kind == HBP_SWEEP_AUTO || kind == HBP_SWEEP_PER_SITE || kind == HBP_SWEEP_GROUPED,
+  HBP_REQUIRE(kind == HBP_SWEEP_AUTO || kind == HBP_SWEEP_PER_SITE || kind == HBP_SWEEP_GROUPED ||
+                  kind == HBP_SWEEP_MERGED,
               HBP_ERR_INVALID_ARGUMENT, "peps: unknown sweep kernel");
   p->sweep_kernel = kind;
   API_END

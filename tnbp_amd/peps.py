@@ -15,7 +15,7 @@ import numpy as np
 from . import _lib
 from ._lib import (HBP_ABSORB_AUTO, HBP_ABSORB_EAGER, HBP_ABSORB_LAZY, HBP_MAX_ORDER,
                    HBP_PREC_FP64, HBP_PREC_MIXED, HBP_SWEEP_AUTO, HBP_SWEEP_GROUPED,
-                   HBP_SWEEP_PER_SITE, check, i32)
+                   HBP_SWEEP_MERGED, HBP_SWEEP_PER_SITE, check, i32)
 from .tci import Context
 
 __all__ = ["Peps", "InitTensorProductState", "AbsorbTPO", "BeliefPropagation",
@@ -68,7 +68,8 @@ class Peps:
         check(self.ctx.lib.hbp_peps_set_absorb_mode(self.h, int(mode)))
 
     def set_sweep_kernel(self, kind):
-        """HBP_SWEEP_AUTO (0), HBP_SWEEP_PER_SITE (1) or HBP_SWEEP_GROUPED (2)."""
+        """HBP_SWEEP_AUTO (0), HBP_SWEEP_PER_SITE (1), HBP_SWEEP_GROUPED (2) or
+        HBP_SWEEP_MERGED (3)."""
         check(self.ctx.lib.hbp_peps_set_sweep_kernel(self.h, int(kind)))
 
     def set_phase_gauge(self, on):
