@@ -47,3 +47,15 @@ def test_missing_library_fails_loudly(tmp_path):
             _lib.load(str(tmp_path / "nope.so"))
     finally:
         _lib._lib = saved
+
+
+def test_host_classes_keep_their_protocols():
+    """Context is a context manager (smoke() and the drivers use `with`), and
+    the module-level helpers stay functions (no method displaced into them)."""
+    from tnbp_amd import tci
+    assert hasattr(tci.Context, "__enter__") and hasattr(tci.Context, "__exit__")
+    for name in ("close", "synchronize", "memory", "trim"):
+        assert callable(getattr(tci.Context, name)), name
+    import types
+    for fn in (tci.device_memory, tci.autotune_report):
+        assert not any(isinstance(c, types.CodeType) for c in fn.__code__.co_consts), fn

@@ -42,6 +42,12 @@ class Context:
         """Give the pool's unused memory back to the device."""
         check(self.lib.hbp_ctx_trim(self.h))
 
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
 
 def autotune_report():
     """The kernel autotuner's choices so far (one line per shape class)."""
@@ -58,12 +64,6 @@ def device_memory(device: int = 0):
     a, b = C.c_size_t(), C.c_size_t()
     check(_lib.load().hbp_device_memory(device, C.byref(a), C.byref(b)))
     return a.value, b.value
-
-    def __enter__(self):
-        return self
-
-    def __exit__(self, *exc):
-        self.close()
 
 
 class Tensor:
