@@ -85,13 +85,14 @@ hbp_status hbp_autotune_report(char *buf, size_t len, size_t *needed);
 /* Free and total device memory of a device (hipMemGetInfo). */
 hbp_status hbp_device_memory(int device, size_t *free_bytes, size_t *total_bytes);
 /* Kernel timing (no reference counterpart; the reference times nothing but
- * one SVD, apps/tci_test/main.cc:39-48).  While enabled, the hot kernels are
- * bracketed by HIP events on the context stream.  profile_read synchronises,
- * returns launch count, summed duration, and the algorithmic flops and bytes
- * of all recorded launches of one kind, and forgets them.  Kinds:
- * HBP_PROF_MODE_PRODUCT, HBP_PROF_MESSAGE (partials + fixed-order reduce),
- * HBP_PROF_PRODUCT_MESSAGE (mode product merged into the message kernel,
- * partials + reduce). */
+ * one SVD, apps/tci_test/main.cc:39-48).  While enabled, each launch of the
+ * hot kernels carries a pair of HIP events in its own dispatch on the context
+ * stream (no extra queue packets).  profile_read synchronises, returns launch
+ * count, summed kernel duration, and the algorithmic flops and bytes of all
+ * timed launches of one kind, and forgets them.  Kinds (the main kernel of
+ * each; split-K reductions are not included): HBP_PROF_MODE_PRODUCT,
+ * HBP_PROF_MESSAGE (message partials), HBP_PROF_PRODUCT_MESSAGE (mode product
+ * merged into the message kernel). */
 enum { HBP_PROF_MODE_PRODUCT = 0, HBP_PROF_MESSAGE = 1, HBP_PROF_PRODUCT_MESSAGE = 2 };
 hbp_status hbp_ctx_profile(hbp_ctx *ctx, int enable);
 hbp_status hbp_ctx_profile_read(hbp_ctx *ctx, int kind, int64_t *count, double *total_ms,

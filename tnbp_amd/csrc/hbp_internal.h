@@ -1,6 +1,7 @@
 // Internal declarations shared by the hbp HIP sources (not part of the C-ABI).
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -62,6 +63,7 @@ struct Ctx {
     hipEvent_t a, b;
     int kind;
     double flops, bytes;
+    bool timed;  // the bracket's main kernel carried a and b in its dispatch
   };
   bool prof = false;
   std::vector<ProfRec> prof_recs;
@@ -86,6 +88,30 @@ struct Tensor {
 
 size_t shape_size(int order, const int32_t *shape);
 void tensor_resize(Ctx *ctx, Tensor *t, int dtype, int order, const int32_t *shape);
+
+// Kernel timing slot of the calling thread: Ctx::prof_begin arms it with a
+// (start, stop) event pair and the next main-kernel launch of a profiled
+// bracket (mode product, message partials, merged product-message) carries
+// the pair in its own dispatch (hipExtLaunchKernelGGL), so that timing puts
+// no extra packets into the queue.  Launches made while the autotuner times
+// its variants do not take the events.
+struct LaunchTiming {
+  hipEvent_t start = nullptr, stop = nullptr;
+  bool armed = false, used = false;
+};
+LaunchTiming &launch_timing();
+// true while tuned_launch times variants on this thread
+bool &launch_tuning();
+template <typename K, typename... Args>
+inline void timed_launch(K kern, dim3 grid, dim3 block, hipStream_t st, Args... args) {
+  LaunchTiming &t = launch_timing();
+  if (t.armed && !t.used && !launch_tuning()) {
+    hipExtLaunchKernelGGL(kern, grid, block, 0, st, t.start, t.stop, 0, args...);
+    t.used = true;
+  } else {
+    hipLaunchKernelGGL(kern, grid, block, 0, st, args...);
+  }
+}
 
 // ------------------------------------------------------------ kernels
 // Out[q, j] = scale * sum_i T[q, i] E(i, j), with the row q = l + L r of a

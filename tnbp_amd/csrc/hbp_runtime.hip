@@ -94,17 +94,34 @@ hipEvent_t Ctx::prof_event() {
   return e;
 }
 
+LaunchTiming &launch_timing() {
+  static thread_local LaunchTiming t;
+  return t;
+}
+bool &launch_tuning() {
+  static thread_local bool on = false;
+  return on;
+}
+
 int Ctx::prof_begin(int kind, double flops, double bytes) {
   if (!prof) return -1;
-  ProfRec r{prof_event(), prof_event(), kind, flops, bytes};
-  HBP_HIP(hipEventRecord(r.a, stream));
+  ProfRec r{prof_event(), prof_event(), kind, flops, bytes, false};
+  LaunchTiming &t = launch_timing();
+  t.start = r.a;
+  t.stop = r.b;
+  t.armed = true;
+  t.used = false;
   prof_recs.push_back(r);
   return (int)prof_recs.size() - 1;
 }
 
 void Ctx::prof_end(int idx) {
   if (idx < 0) return;
-  HBP_HIP(hipEventRecord(prof_recs[idx].b, stream));
+  LaunchTiming &t = launch_timing();
+  // a bracket whose kernel did not take the events (none of the three main
+  // kernels ran in it) is not counted
+  prof_recs[idx].timed = t.armed && t.used;
+  t = LaunchTiming{};
 }
 
 void Ctx::upload(void *dst, const void *src, size_t bytes) {
@@ -595,12 +612,14 @@ hbp_status hbp_ctx_profile_read(hbp_ctx *ctx, int kind, int64_t *count, double *
       keep.push_back(r);
       continue;
     }
-    float t = 0.0f;
-    HBP_HIP(hipEventElapsedTime(&t, r.a, r.b));
-    ++n;
-    ms += t;
-    fl += r.flops;
-    by += r.bytes;
+    if (r.timed) {
+      float t = 0.0f;
+      HBP_HIP(hipEventElapsedTime(&t, r.a, r.b));
+      ++n;
+      ms += t;
+      fl += r.flops;
+      by += r.bytes;
+    }
     ctx->prof_free.push_back(r.a);
     ctx->prof_free.push_back(r.b);
   }
