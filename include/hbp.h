@@ -70,10 +70,12 @@ hbp_status hbp_ctx_destroy(hbp_ctx *ctx);
 hbp_status hbp_ctx_synchronize(hbp_ctx *ctx);
 /* The HIP stream all work of this context runs on (a hipStream_t). */
 hbp_status hbp_ctx_stream(hbp_ctx *ctx, void **stream);
-/* Bytes currently held by the context's device pool. */
+/* Bytes of live allocations of the context (and their peak).  Freed large
+ * blocks (>= 32 MiB) are kept for reuse by the context and do not count. */
 hbp_status hbp_ctx_memory(hbp_ctx *ctx, size_t *in_use, size_t *peak);
-/* Return the pool's unused device memory to the device (after a job ends) and
- * reset the peak to the bytes still in use.  No reference counterpart. */
+/* Return the context's unused device memory (kept large blocks and the
+ * pool's reserve) to the device (after a job ends) and reset the peak to the
+ * bytes still in use.  No reference counterpart. */
 hbp_status hbp_ctx_trim(hbp_ctx *ctx);
 /* Kernel launch variants that compute bitwise the same result (rows per
  * wave, loads in flight, occupancy) are timed on the first launch of each
@@ -89,7 +91,8 @@ hbp_status hbp_device_memory(int device, size_t *free_bytes, size_t *total_bytes
  * hot kernels carries a pair of HIP events in its own dispatch on the context
  * stream (no extra queue packets).  profile_read synchronises, returns launch
  * count, summed kernel duration, and the algorithmic flops and bytes of all
- * timed launches of one kind, and forgets them.  Kinds (the main kernel of
+ * timed launches of one kind, and forgets them; disabling drops records not
+ * read yet (their events keep the launches' memory alive).  Kinds (the main kernel of
  * each; split-K reductions are not included): HBP_PROF_MODE_PRODUCT,
  * HBP_PROF_MESSAGE (message partials), HBP_PROF_PRODUCT_MESSAGE (mode product
  * merged into the message kernel). */

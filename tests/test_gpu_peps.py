@@ -326,6 +326,40 @@ def test_merged_sweep_matches_per_site(ctx, precision, tol, res_rtol, res_atol, 
     assert rel_close(out[1][3], out[0][3], tol) <= 1.0
 
 
+def test_pair_cache_follows_site_changes(ctx, tmp_path):
+    """The merged sweep keeps each site with its target pair moved to the front
+    for the whole BP loop; changing a site must drop that copy.  A PEPS whose
+    site 5 is replaced after a BP loop continues exactly (bitwise) like a fresh
+    PEPS restored from a checkpoint of the changed state."""
+    from tnbp_amd import circuits as pc
+    from tnbp_amd.peps import AbsorbTPO, InitTensorProductState, Truncation, bp_loop, load_peps, save_peps
+    announce("pair cache invalidation")
+    edges = pc.square_lattice(4, 4)
+    tpo = pc.kicked_ising_tpo(edges, 0.125, 0.0, 0.11, math.pi)
+    q = InitTensorProductState(ctx, edges, [2] * 16, 1)
+    for _ in range(6):
+        AbsorbTPO(ctx, tpo, q)
+        bp_loop(ctx, q, 20, 1e-9)
+        Truncation(ctx, q, 20, 1e-12)
+    AbsorbTPO(ctx, tpo, q)
+    assert max(q.site_shape(5)[:-1]) == 40
+    bp_loop(ctx, q, 3, 0.0)                      # builds the pair-front copies
+    v = q.get_site(5)
+    rng = np.random.default_rng(3)
+    q.set_site(5, v + 0.05 * np.abs(v).max() * rng.standard_normal(v.shape))
+    path = tmp_path / "changed.hbpp"
+    save_peps(q, path)
+    r = load_peps(ctx, path)
+    out = []
+    for st in (q, r):
+        it, res = bp_loop(ctx, st, 3, 0.0)
+        out.append((res, [st.get_message(m) for m in range(2 * len(edges))]))
+        st.close()
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    for a, b in zip(out[0][1], out[1][1]):
+        np.testing.assert_array_equal(a, b)
+
+
 def test_checkpoint_round_trip(ctx, tmp_path):
     """save_peps / load_peps: a restored state continues exactly like the
     original (bitwise equal messages and expectation values)."""

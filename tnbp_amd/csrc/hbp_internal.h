@@ -5,8 +5,10 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "hbp.h"
@@ -55,6 +57,19 @@ struct Ctx {
 
   void *alloc(size_t bytes);
   void release(void *p, size_t bytes);
+  // device memory this context can still allocate: the device's free memory
+  // plus what its pool and its block cache hold in reserve
+  size_t free_bytes(size_t *total = nullptr);
+  // Large blocks (>= 32 MiB, sized in 2 MiB steps) are kept on release and
+  // handed out again for the same size: every step of a PEPS run frees and
+  // re-allocates ~D^4 d-sized site tensors, and getting that memory back from
+  // the driver each time stalls the host for seconds.  Reuse is safe because
+  // all of a context's work is ordered on its one stream.  flush_blocks()
+  // gives the cache back (trim, failed allocations, destroy).
+  std::multimap<size_t, void *> free_blocks;
+  std::unordered_map<void *, size_t> big_sizes;  // live large block -> size
+  size_t cached = 0;
+  void flush_blocks();
   void *get_scratch(size_t bytes);
   void sync();
 
@@ -67,8 +82,8 @@ struct Ctx {
   };
   bool prof = false;
   std::vector<ProfRec> prof_recs;
-  std::vector<hipEvent_t> prof_free;
   hipEvent_t prof_event();
+  void prof_clear();  // destroys the events of every open record
   // returns the index of the open record, or -1 when profiling is off
   int prof_begin(int kind, double flops, double bytes);
   void prof_end(int idx);

@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <numeric>
@@ -498,6 +499,14 @@ struct Peps {
   // work buffers, each as large as the largest site tensor
   void *buf[3] = {nullptr, nullptr, nullptr};
   size_t buf_bytes = 0;
+  // Site tensors with the target pair of their first half moved to the front
+  // (merged sweep): built on first use and reused by every sweep while the
+  // sites stay unchanged (the sites are constant during a BP loop); dropped by
+  // anything that changes a site.  Only for resident sites, and only while
+  // device memory stays ample.
+  std::vector<void *> Vpair;
+  std::vector<size_t> Vpair_bytes;
+  int pair_cache = -1;  // -1 undecided (decided once per set of sites), 0 off, 1 on
   int mode = HBP_MODE_FUSED;
   // mixed precision: BP stops once the residual is below this (FP32 messages
   // cannot reach the FP64 tolerances), or below the caller's tolerance
@@ -599,9 +608,20 @@ static void make_resident(Ctx *ctx, Peps *p, int s) {
   if (p->abs_site == s) p->abs_site = -1;
 }
 
+static void drop_pair_cache(Ctx *ctx, Peps *p) {
+  for (size_t s = 0; s < p->Vpair.size(); ++s)
+    if (p->Vpair[s]) {
+      ctx->release(p->Vpair[s], p->Vpair_bytes[s]);
+      p->Vpair[s] = nullptr;
+      p->Vpair_bytes[s] = 0;
+    }
+  p->pair_cache = -1;
+}
+
 // Replace site s's tensor (already allocated at the current shape) and drop
 // any pending operator.
 static void set_resident(Ctx *ctx, Peps *p, int s, void *nv, size_t bytes) {
+  drop_pair_cache(ctx, p);
   ctx->release(p->V[s], p->V_bytes[s]);
   p->V[s] = nv;
   p->V_bytes[s] = bytes;
@@ -758,6 +778,37 @@ static const void *absorb_env(Ctx *ctx, Peps *p, int s, int skip, int32_t *sh) {
   return cur;
 }
 
+// Whether the pair-front copies of the sites fit: decided once per set of
+// sites from the bytes all of them need and the free device memory, keeping
+// max(16 GiB, 15 %) of the device free.  HBP_PAIR_CACHE=0 in the environment
+// turns the copies off (each sweep then rearranges into a work buffer).
+static void plan_pair_cache(Ctx *ctx, Peps *p) {
+  if ((int)p->Vpair.size() != p->n_sites) {
+    p->Vpair.assign(p->n_sites, nullptr);
+    p->Vpair_bytes.assign(p->n_sites, 0);
+  }
+  static const bool allowed = [] {
+    const char *e = getenv("HBP_PAIR_CACHE");
+    return !(e && e[0] == '0');
+  }();
+  if (!allowed) {
+    p->pair_cache = 0;
+    return;
+  }
+  int32_t sh[HBP_MAX_ORDER];
+  size_t need = 0;
+  for (int s = 0; s < p->n_sites; ++s) {
+    const int z = p->z(s), h = z / 2;
+    if (z < 3 || z - h != 2 || p->pending[s]) continue;
+    p->shape(s, sh);
+    if (merge_ok(p, sh, h, h + 1)) need += p->site_size(s) * p->elem_bytes();
+  }
+  size_t tot = 0;
+  const size_t fr = ctx->free_bytes(&tot);
+  const size_t keep = std::max<size_t>((size_t)16 << 30, tot / 100 * 15);
+  p->pair_cache = need + keep < fr ? 1 : 0;
+}
+
 template <typename S>
 static void sweep_raw(Ctx *ctx, Peps *p, double2 *raw, const std::vector<int64_t> &raw_off) {
   size_t maxb = 16;
@@ -805,9 +856,22 @@ static void sweep_raw(Ctx *ctx, Peps *p, double2 *raw, const std::vector<int64_t
           shp[pos[d]] = sh[d];
         }
         if (Lb > 1) {
-          launch_swap_blocks(ctx->stream, Vs, p->buf[2], Lb, (int64_t)sh[t0] * sh[t0 + 1], R,
-                             (int)(2 * sizeof(S)));
-          Vs = p->buf[2];
+          const size_t vb = p->site_size(s) * p->elem_bytes();
+          void *dst = p->buf[2];
+          bool have = false;
+          if (p->pair_cache < 0) plan_pair_cache(ctx, p);
+          if (p->Vpair[s]) {
+            dst = p->Vpair[s];
+            have = true;
+          } else if (p->pair_cache == 1 && !p->pending[s]) {
+            p->Vpair[s] = ctx->alloc(vb);
+            p->Vpair_bytes[s] = vb;
+            dst = p->Vpair[s];
+          }
+          if (!have)
+            launch_swap_blocks(ctx->stream, Vs, dst, Lb, (int64_t)sh[t0] * sh[t0 + 1], R,
+                               (int)(2 * sizeof(S)));
+          Vs = dst;
           cur = Vs;
         }
         for (int j = a0; j < a1; ++j) {
@@ -1720,6 +1784,7 @@ hbp_status hbp_peps_destroy(hbp_ctx *ctx, hbp_peps *p) {
     if (p->pend_O[s]) ctx->release(p->pend_O[s], p->pend_O_bytes[s]);
   }
   if (p->abs_buf) ctx->release(p->abs_buf, p->abs_bytes);
+  drop_pair_cache(ctx, p);
   ctx->release(p->E, p->E_bytes);
   for (int i = 0; i < 3; ++i)
     if (p->buf[i]) ctx->release(p->buf[i], p->buf_bytes);
@@ -1867,6 +1932,7 @@ hbp_status hbp_peps_absorb_tpo(hbp_ctx *ctx, hbp_peps *p, const double *const *o
       t[m] = sh[k];
     }
   }
+  drop_pair_cache(ctx, p);
   // a second absorption before BP composes with the first: store the first
   for (int s = 0; s < ns; ++s) make_resident(ctx, p, s);
   // lazy: keep V_s and O_s, materialise per sweep; eager: store the absorbed
@@ -1882,9 +1948,8 @@ hbp_status hbp_peps_absorb_tpo(hbp_ctx *ctx, hbp_peps *p, const double *const *o
         for (int k = 0; k < p->z(s); ++k) n *= (size_t)p->bond_dim[p->bonds[s][k]] * osh[k];
         total += n * p->elem_bytes();
       }
-      size_t free_b = 0, total_b = 0;
-      HBP_HIP(hipMemGetInfo(&free_b, &total_b));
-      lazy = total > free_b / 2;
+      // (memory the context pool holds in reserve counts as free)
+      lazy = total > ctx->free_bytes() / 2;
     }
   }
   for (int s = 0; s < ns; ++s) {

@@ -50,15 +50,40 @@ size_t shape_size(int order, const int32_t *shape) {
   return n;
 }
 
+namespace {
+constexpr size_t kBigBlock = (size_t)32 << 20;
+constexpr size_t kBlockStep = (size_t)2 << 20;
+}  // namespace
+
 void *Ctx::alloc(size_t bytes) {
   if (bytes == 0) bytes = 16;
+  const bool big = bytes >= kBigBlock;
+  if (big) {
+    bytes = (bytes + kBlockStep - 1) / kBlockStep * kBlockStep;
+    auto it = free_blocks.find(bytes);
+    if (it != free_blocks.end()) {
+      void *p = it->second;
+      free_blocks.erase(it);
+      cached -= bytes;
+      big_sizes[p] = bytes;
+      in_use += bytes;
+      peak = std::max(peak, in_use);
+      return p;
+    }
+  }
   void *p = nullptr;
   hipError_t e = hipMallocFromPoolAsync(&p, bytes, pool, stream);
+  if (e != hipSuccess && cached > 0) {
+    (void)hipGetLastError();
+    flush_blocks();
+    e = hipMallocFromPoolAsync(&p, bytes, pool, stream);
+  }
   if (e != hipSuccess) {
     (void)hipGetLastError();
     throw Error(HBP_ERR_BAD_ALLOC, std::string("device allocation of ") + std::to_string(bytes) +
                                        " bytes failed: " + hipGetErrorString(e));
   }
+  if (big) big_sizes[p] = bytes;
   in_use += bytes;
   peak = std::max(peak, in_use);
   return p;
@@ -66,8 +91,24 @@ void *Ctx::alloc(size_t bytes) {
 
 void Ctx::release(void *p, size_t bytes) {
   if (!p) return;
+  auto it = big_sizes.find(p);
+  if (it != big_sizes.end()) {
+    // the size recorded at allocation decides the bucket, not the caller's
+    const size_t b = it->second;
+    big_sizes.erase(it);
+    free_blocks.emplace(b, p);
+    cached += b;
+    in_use -= std::min(in_use, b);
+    return;
+  }
   HBP_HIP(hipFreeAsync(p, stream));
   in_use -= std::min(in_use, bytes == 0 ? (size_t)16 : bytes);
+}
+
+void Ctx::flush_blocks() {
+  for (auto &kv : free_blocks) (void)hipFreeAsync(kv.second, stream);
+  free_blocks.clear();
+  cached = 0;
 }
 
 void *Ctx::get_scratch(size_t bytes) {
@@ -83,15 +124,31 @@ void *Ctx::get_scratch(size_t bytes) {
 
 void Ctx::sync() { HBP_HIP(hipStreamSynchronize(stream)); }
 
+size_t Ctx::free_bytes(size_t *total) {
+  size_t fr = 0, tot = 0;
+  HBP_HIP(hipMemGetInfo(&fr, &tot));
+  uint64_t reserved = 0, used = 0;
+  HBP_HIP(hipMemPoolGetAttribute(pool, hipMemPoolAttrReservedMemCurrent, &reserved));
+  HBP_HIP(hipMemPoolGetAttribute(pool, hipMemPoolAttrUsedMemCurrent, &used));
+  if (total) *total = tot;
+  return fr + (size_t)(reserved > used ? reserved - used : 0) + cached;
+}
+
+// Events are not recycled: an event that a kernel dispatch carried keeps that
+// dispatch (and the device memory its arguments point to) alive until the
+// event is destroyed, so read-out records destroy theirs.
 hipEvent_t Ctx::prof_event() {
-  if (!prof_free.empty()) {
-    hipEvent_t e = prof_free.back();
-    prof_free.pop_back();
-    return e;
-  }
   hipEvent_t e;
   HBP_HIP(hipEventCreate(&e));
   return e;
+}
+
+void Ctx::prof_clear() {
+  for (auto &r : prof_recs) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  prof_recs.clear();
 }
 
 LaunchTiming &launch_timing() {
@@ -529,7 +586,12 @@ hbp_status hbp_ctx_destroy(hbp_ctx *ctx) {
   if (!ctx) return HBP_OK;
   HBP_HIP(hipSetDevice(ctx->device));
   if (ctx->scratch) ctx->release(ctx->scratch, ctx->scratch_bytes);
+  ctx->flush_blocks();
   HBP_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->prof_clear();
+  // give the reserve back before the pool goes, so that a context created
+  // next in this process sees the memory as free
+  HBP_HIP(hipMemPoolTrimTo(ctx->pool, 0));
   HBP_HIP(hipMemPoolDestroy(ctx->pool));
   HBP_HIP(hipStreamDestroy(ctx->stream));
   delete ctx;
@@ -574,6 +636,7 @@ hbp_status hbp_ctx_trim(hbp_ctx *ctx) {
     ctx->scratch = nullptr;
     ctx->scratch_bytes = 0;
   }
+  ctx->flush_blocks();
   ctx->sync();
   HBP_HIP(hipMemPoolTrimTo(ctx->pool, 0));
   ctx->peak = ctx->in_use;
@@ -597,6 +660,11 @@ hbp_status hbp_device_memory(int device, size_t *free_bytes, size_t *total_bytes
 hbp_status hbp_ctx_profile(hbp_ctx *ctx, int enable) {
   HBP_API_BEGIN
   ctx->prof = enable != 0;
+  // records not read by now are dropped (their events pin device memory)
+  if (!ctx->prof) {
+    ctx->sync();
+    ctx->prof_clear();
+  }
   HBP_API_END
 }
 
@@ -620,8 +688,8 @@ hbp_status hbp_ctx_profile_read(hbp_ctx *ctx, int kind, int64_t *count, double *
       fl += r.flops;
       by += r.bytes;
     }
-    ctx->prof_free.push_back(r.a);
-    ctx->prof_free.push_back(r.b);
+    HBP_HIP(hipEventDestroy(r.a));
+    HBP_HIP(hipEventDestroy(r.b));
   }
   ctx->prof_recs.swap(keep);
   if (count) *count = n;
