@@ -268,36 +268,37 @@ def test_grouped_sweep_matches_per_site(ctx, precision, tol):
 
 @pytest.mark.parametrize("precision,tol,res_rtol,res_atol", [(0, 1e-10, 1e-9, 1e-13),
                                                             (1, 2e-5, 0.0, 5e-8)])
-def test_merged_sweep_matches_per_site(ctx, precision, tol, res_rtol, res_atol, tmp_path):
+def test_merged_sweep_matches_per_site(ctx, precision, tol, res_rtol, res_atol):
     """Merged product-message kernels (the last mode product of each message
-    inside the message kernel) against separate launches, from one state: a
-    4x4 kicked-Ising PEPS grown to chi = 20 and absorbed once more (bonds of
-    40, the merged kernel's zero-padded case), checkpointed and restored into a
-    second PEPS; then the same BP iterations with either sweep.  Starting from
-    one state keeps the bond gauge common, so the messages themselves compare
-    (after separate truncations only gauge-invariant quantities would)."""
+    inside the message kernel, and both messages of a bond pair from one pass
+    in the first sweep after an absorption) against separate launches: two
+    4x4 kicked-Ising PEPS grown identically with separate launches to chi = 20
+    and absorbed once more (bonds of 40, the merged kernels' zero-padded
+    case); then the same BP iterations with either sweep.  Equal states keep
+    the bond gauge common, so the messages themselves compare."""
     from tnbp_amd import circuits as pc
     from tnbp_amd._lib import check
     from tnbp_amd.peps import (HBP_SWEEP_MERGED, HBP_SWEEP_PER_SITE, AbsorbTPO,
-                               InitTensorProductState, Measure, MeasureEdges, Truncation, bp_loop,
-                               load_peps, save_peps)
+                               InitTensorProductState, Measure, MeasureEdges, Truncation, bp_loop)
     announce(f"merged vs per-site sweep precision {precision}")
     edges = pc.square_lattice(4, 4)
     tpo = pc.kicked_ising_tpo(edges, 0.125, 0.0, 0.11, math.pi)
     zop = pc.op_tensor(pc.PAULI["Z"])
     zzop = pc.op_tensor(np.kron(pc.PAULI["Z"], pc.PAULI["Z"]))
-    p = InitTensorProductState(ctx, edges, [2] * 16, precision)
-    for _ in range(6):
+    states = []
+    for _ in range(2):
+        p = InitTensorProductState(ctx, edges, [2] * 16, precision)
+        p.set_sweep_kernel(HBP_SWEEP_PER_SITE)
+        for _ in range(6):
+            AbsorbTPO(ctx, tpo, p)
+            bp_loop(ctx, p, 30, 1e-9)
+            Truncation(ctx, p, 20, 1e-12)
         AbsorbTPO(ctx, tpo, p)
-        bp_loop(ctx, p, 30, 1e-9)
-        Truncation(ctx, p, 20, 1e-12)
-    AbsorbTPO(ctx, tpo, p)
-    path = tmp_path / "merged.hbpp"
-    save_peps(p, path)
-    q = load_peps(ctx, path)
-    assert max(max(p.site_shape(s)[:-1]) for s in range(16)) == 40
+        states.append(p)
+    assert max(max(states[0].site_shape(s)[:-1]) for s in range(16)) == 40
+    np.testing.assert_array_equal(states[0].get_site(5), states[1].get_site(5))
     out = []
-    for st, kind in ((p, HBP_SWEEP_PER_SITE), (q, HBP_SWEEP_MERGED)):
+    for st, kind in zip(states, (HBP_SWEEP_PER_SITE, HBP_SWEEP_MERGED)):
         st.set_sweep_kernel(kind)
         check(ctx.lib.hbp_ctx_profile(ctx.h, 1))
         it, r = bp_loop(ctx, st, 6, 0.0)
@@ -311,10 +312,12 @@ def test_merged_sweep_matches_per_site(ctx, precision, tol, res_rtol, res_atol, 
         Truncation(ctx, st, 20, 1e-12)
         z = Measure(ctx, st, list(range(16)), [zop] * 16)
         zz = MeasureEdges(ctx, st, edges, [zzop] * len(edges))
-        out.append((r, msgs, z, zz, counts[2]))
+        out.append((r, msgs, z, zz, counts))
         st.close()
-    # only the merged sweep runs merged launches (7 sweeps x at least one per site)
-    assert out[0][4] == 0 and out[1][4] >= 7 * 4
+    # only the merged sweep runs merged launches (at least one per site and sweep),
+    # and its first sweep takes fewer message launches (pairs in one pass)
+    assert out[0][4][2] == 0 and out[1][4][2] >= 7 * 4
+    assert out[1][4][1] < out[0][4][1]
     # mixed precision may stop one iteration apart at its 1e-6 residual floor;
     # its residuals near the floor carry FP32 rounding (absolute tolerance)
     n = min(len(out[0][0]), len(out[1][0]))

@@ -164,6 +164,16 @@ template <typename S>
 int product_message_partials(hipStream_t st, const void *P, const void *V, const double2 *E,
                              void *partials, int64_t L, int64_t R, int Dk, int Dj,
                              int k_first, int max_blocks);
+// Partials of both messages of a bond pair (x, y) at the front of a tensor
+// whose environment is diagonal: slices X_o (Dx x Dy, contiguous, o < nO),
+// M_x = sum_o w_o X_o diag(ey) X_o^H, M_y = sum_o w_o X_o^T diag(ex) conj(X_o),
+// ex / ey read as E[c (D + 1)] (diagonals of the messages), wo may be null
+// (all ones).  Returns the number of partials (each Dx*Dx resp. Dy*Dy complex
+// of scalar S, at most max_blocks).
+template <typename S>
+int pair_gram_partials(hipStream_t st, const void *X, int64_t nO, int Dx, int Dy,
+                       const double2 *Ex, const double2 *Ey, const double2 *wo, void *part_x,
+                       void *part_y, int max_blocks);
 // Y[c + Cn (l + L r)] = X[l + L (c + Cn r)] for complex elements of elem_bytes
 // (8 or 16): moves the middle index block of an (L, Cn, R) view to the front.
 void launch_swap_blocks(hipStream_t st, const void *X, void *Y, int64_t L, int64_t Cn,

@@ -788,6 +788,146 @@ __global__ __launch_bounds__(256, OccPM<S>::v) void k_product_message(
   }
 }
 
+// Both messages of a bond pair (x, y) at the front of a site tensor whose
+// environment is diagonal (the first BP sweep after AbsorbTPO or Truncation,
+// bp.h:147-165 with E = diag(Lambda) (x) I):
+//   M_x[i, i'] = sum_{o, y} w_o[o] ey[y] X_o[i, y] conj X_o[i', y]
+//   M_y[j, j'] = sum_{o, x} w_o[o] ex[x] X_o[x, j] conj X_o[x, j']
+// with X_o the Dx x Dy slice o (column-major, slices contiguous), ex / ey the
+// diagonals of the messages entering on x / y and w_o the product of the other
+// bonds' diagonals.  Each slice is read from HBM once for both messages: it is
+// staged in LDS (double buffered, one barrier per slice) and both products
+// take their MFMA operands from that tile, in either orientation.  Partials
+// per workgroup as in k_product_message.
+template <typename S, bool FULL>
+__global__ __launch_bounds__(256, OccPM<S>::v) void k_pair_gram(
+    const typename Cplx<S>::type *__restrict__ X, int64_t nO, int Dx_rt, int Dy_rt,
+    const double2 *__restrict__ Ex, const double2 *__restrict__ Ey,
+    const double2 *__restrict__ wo, typename Cplx<S>::type *__restrict__ part_x,
+    typename Cplx<S>::type *__restrict__ part_y) {
+  using C = typename Cplx<S>::type;
+  using M = Mfma<S>;
+  using v4 = typename M::v4;
+  __shared__ C Xs[2][PM_D][PM_D + 1];  // [buffer][y][x]
+  const int Dx = FULL ? PM_D : Dx_rt, Dy = FULL ? PM_D : Dy_rt;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int64_t n0 = (nb % 8 == 0) ? (int64_t)(b % 8) * (nb / 8) + b / 8 : b;
+  const int64_t slice = (int64_t)Dx * Dy;
+
+  // diagonal weights of this lane's contraction indices
+  C wy[16], wx[16];
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    const int a = pm_a(s, lk);
+    double2 ey = make_double2(0.0, 0.0), ex = make_double2(0.0, 0.0);
+    if (FULL || a < Dy) ey = Ey[(int64_t)a * (Dy + 1)];
+    if (FULL || a < Dx) ex = Ex[(int64_t)a * (Dx + 1)];
+    wy[s].x = (S)ey.x;
+    wy[s].y = (S)ey.y;
+    wx[s].x = (S)ex.x;
+    wx[s].y = (S)ex.y;
+  }
+  C vr[16];
+  auto load = [&](int64_t o) {
+    const C *base = X + o * slice;
+#pragma unroll
+    for (int n = 0; n < 16; ++n) {
+      const int idx = tid + 256 * n;
+      const int x = idx & (PM_D - 1), y = idx / PM_D;
+      const bool ok = FULL || (x < Dx && y < Dy);
+      const C v = base[ok ? x + Dx * y : 0];
+      vr[n] = ok ? v : C{0, 0};
+    }
+  };
+  auto cmul = [](C a, C b) {
+    C r;
+    r.x = a.x * b.x - a.y * b.y;
+    r.y = a.x * b.y + a.y * b.x;
+    return r;
+  };
+
+  v4 mx_re[4], mx_im[4], my_re[4], my_im[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    mx_re[c] = mx_im[c] = my_re[c] = my_im[c] = v4{0, 0, 0, 0};
+  }
+  int buf = 0;
+  int64_t o = n0;
+  if (o < nO) load(o);
+  for (; o < nO; o += nb, buf ^= 1) {
+#pragma unroll
+    for (int n = 0; n < 16; ++n) {
+      const int idx = tid + 256 * n;
+      Xs[buf][idx / PM_D][idx & (PM_D - 1)] = vr[n];
+    }
+    // one barrier: the other buffer was last read before this point
+    __syncthreads();
+    if (o + nb < nO) load(o + nb);
+    C w = C{1, 0};
+    if (wo) {
+      const double2 t = wo[o];
+      w.x = (S)t.x;
+      w.y = (S)t.y;
+    }
+    // M_x^T += conj(X) T over y, T[i][y] = w ey[y] X[i][y]:
+    // conj(A) T = (Ar Tr + Ai Ti) + i (Ar Ti - Ai Tr)
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      const int y = pm_a(s, lk);
+      const C t = cmul(cmul(w, wy[s]), Xs[buf][y][wave * 16 + li]);
+      const S tr = t.x, ti = t.y, ntr = -t.x;
+#pragma unroll
+      for (int c2 = 0; c2 < 4; ++c2) {
+        const C a = Xs[buf][y][c2 * 16 + li];
+        mx_re[c2] = M::op(a.x, tr, mx_re[c2]);
+        mx_im[c2] = M::op(a.x, ti, mx_im[c2]);
+        mx_re[c2] = M::op(a.y, ti, mx_re[c2]);
+        mx_im[c2] = M::op(a.y, ntr, mx_im[c2]);
+      }
+    }
+    // M_y^T += conj(X^T) T' over x, T'[j][x] = w ex[x] X[x][j]
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      const int x = pm_a(s, lk);
+      const C t = cmul(cmul(w, wx[s]), Xs[buf][wave * 16 + li][x]);
+      const S tr = t.x, ti = t.y, ntr = -t.x;
+#pragma unroll
+      for (int c2 = 0; c2 < 4; ++c2) {
+        const C a = Xs[buf][c2 * 16 + li][x];
+        my_re[c2] = M::op(a.x, tr, my_re[c2]);
+        my_im[c2] = M::op(a.x, ti, my_im[c2]);
+        my_re[c2] = M::op(a.y, ti, my_re[c2]);
+        my_im[c2] = M::op(a.y, ntr, my_im[c2]);
+      }
+    }
+  }
+
+  // register v of lane (li, lk): M[16 w + li][16 c2 + row(lk, v)]
+  const int i = wave * 16 + li;
+  C *ox = part_x + (int64_t)b * Dx * Dx, *oy = part_y + (int64_t)b * Dy * Dy;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+#pragma unroll
+    for (int c2 = 0; c2 < 4; ++c2) {
+      const int i2 = c2 * 16 + M::row(lk, v);
+      if (FULL || (i < Dx && i2 < Dx)) {
+        C t;
+        t.x = mx_re[c2][v];
+        t.y = mx_im[c2][v];
+        ox[i + (int64_t)Dx * i2] = t;
+      }
+      if (FULL || (i < Dy && i2 < Dy)) {
+        C t;
+        t.x = my_re[c2][v];
+        t.y = my_im[c2][v];
+        oy[i + (int64_t)Dy * i2] = t;
+      }
+    }
+  }
+}
+
 // Y[c + C (l + L r)] = X[l + L (c + C r)]: R column-major L x C matrices
 // transposed through 64 x 64 LDS tiles (reads along l, writes along c), so
 // that a pair of bonds behind a block of L leading indices moves to the front.
@@ -1559,6 +1699,32 @@ int product_message_partials(hipStream_t st, const void *P, const void *V, const
   HBP_HIP(hipGetLastError());
   return (int)nb;
 }
+template <typename S>
+int pair_gram_partials(hipStream_t st, const void *X, int64_t nO, int Dx, int Dy,
+                       const double2 *Ex, const double2 *Ey, const double2 *wo, void *part_x,
+                       void *part_y, int max_blocks) {
+  HBP_REQUIRE(Dx >= 1 && Dx <= PM_D && Dy >= 1 && Dy <= PM_D, HBP_ERR_UNSUPPORTED,
+              "pair gram: bonds must be <= 64");
+  HBP_REQUIRE(nO >= 1 && max_blocks >= 1, HBP_ERR_INVALID_ARGUMENT, "pair gram: empty problem");
+  int64_t nb = std::min<int64_t>(std::min<int64_t>(max_blocks, 512), nO);
+  if (nb >= 8) nb -= nb % 8;
+  using C = typename Cplx<S>::type;
+  auto go = [&](auto kern) {
+    timed_launch(kern, dim3((unsigned)nb), dim3(256), st, (const C *)X, nO, Dx, Dy, Ex, Ey, wo,
+                 (C *)part_x, (C *)part_y);
+  };
+  if (Dx == PM_D && Dy == PM_D) go(k_pair_gram<S, true>);
+  else go(k_pair_gram<S, false>);
+  HBP_HIP(hipGetLastError());
+  return (int)nb;
+}
+template int pair_gram_partials<double>(hipStream_t, const void *, int64_t, int, int,
+                                        const double2 *, const double2 *, const double2 *, void *,
+                                        void *, int);
+template int pair_gram_partials<float>(hipStream_t, const void *, int64_t, int, int,
+                                       const double2 *, const double2 *, const double2 *, void *,
+                                       void *, int);
+
 void launch_swap_blocks(hipStream_t st, const void *X, void *Y, int64_t L, int64_t Cn,
                         int64_t R, int elem_bytes) {
   const int64_t ntiles = ((L + 63) / 64) * ((Cn + 63) / 64) * R;

@@ -809,6 +809,60 @@ static void plan_pair_cache(Ctx *ctx, Peps *p) {
   p->pair_cache = need + keep < fr ? 1 : 0;
 }
 
+// Site s (data Vs) with its bond pair (t, t + 1) moved to the front:
+// (D_t, D_{t+1}, bonds before t, bonds after t + 1, phys).  Vs itself when no
+// bond precedes the pair; else the cached copy (built here on first use) or a
+// rearrangement into work buffer 2.
+template <typename S>
+static const void *pair_front(Ctx *ctx, Peps *p, int s, const void *Vs, const int32_t *sh,
+                              int order, int t) {
+  int64_t Lb = 1, R = 1;
+  for (int d = 0; d < t; ++d) Lb *= sh[d];
+  for (int d = t + 2; d < order; ++d) R *= sh[d];
+  if (Lb == 1) return Vs;
+  if (p->pair_cache < 0) plan_pair_cache(ctx, p);
+  if (p->Vpair[s]) return p->Vpair[s];
+  void *dst = p->buf[2];
+  if (p->pair_cache == 1 && !p->pending[s]) {
+    const size_t vb = p->site_size(s) * p->elem_bytes();
+    p->Vpair[s] = ctx->alloc(vb);
+    p->Vpair_bytes[s] = vb;
+    dst = p->Vpair[s];
+  }
+  launch_swap_blocks(ctx->stream, Vs, dst, Lb, (int64_t)sh[t] * sh[t + 1], R, (int)(2 * sizeof(S)));
+  return dst;
+}
+
+// Both messages of the pair (t, t + 1) of site s under a diagonal environment
+// (first sweep after AbsorbTPO / Truncation) with one pass over the site:
+// pair_gram_partials on the pair-front tensor.  Needs t == 0 or t + 2 == z, so
+// that the remaining bonds form one block.
+template <typename S>
+static void pair_messages_diag(Ctx *ctx, Peps *p, int s, const int32_t *sh, int order, int t,
+                               double2 *Mx, double2 *My) {
+  const int z = p->z(s);
+  const void *X = pair_front<S>(ctx, p, s, site_data(ctx, p, s), sh, order, t);
+  int64_t nw = 0;
+  double2 *wo = t == 0 ? diag_weights(ctx, p, s, 2, z, p->phys[s], &nw)
+                       : diag_weights(ctx, p, s, 0, t, p->phys[s], &nw);
+  const int Dx = sh[t], Dy = sh[t + 1];
+  const int64_t nO = (int64_t)(p->site_size(s) / ((size_t)Dx * Dy));
+  constexpr int kMaxBlocks = 512;
+  using C = std::conditional_t<sizeof(S) == 4, float2, double2>;
+  C *px = (C *)ctx->get_scratch((size_t)kMaxBlocks * (Dx * Dx + Dy * Dy) * sizeof(C) + 64);
+  C *py = px + (size_t)kMaxBlocks * Dx * Dx;
+  const double n = (double)nO * Dx * Dy;
+  const int pr = ctx->prof_begin(HBP_PROF_MESSAGE, 8.0 * n * (Dx + Dy), sizeof(S) * 2.0 * n);
+  const int nb = pair_gram_partials<S>(ctx->stream, X, nO, Dx, Dy,
+                                       p->E + p->E_off[p->in_msg(s, t)],
+                                       p->E + p->E_off[p->in_msg(s, t + 1)], wo, px, py,
+                                       kMaxBlocks);
+  launch_reduce_partials(ctx->stream, px, nb, Dx * Dx, Mx);
+  launch_reduce_partials(ctx->stream, py, nb, Dy * Dy, My);
+  ctx->prof_end(pr);
+  if (wo) ctx->release(wo, (size_t)nw * 16);
+}
+
 template <typename S>
 static void sweep_raw(Ctx *ctx, Peps *p, double2 *raw, const std::vector<int64_t> &raw_off) {
   size_t maxb = 16;
@@ -819,8 +873,20 @@ static void sweep_raw(Ctx *ctx, Peps *p, double2 *raw, const std::vector<int64_t
     const int z = p->z(s), order = z + 1;
     p->shape(s, sh);
     if (diag_env(p, s, -1)) {
-      // first sweep after AbsorbTPO / Truncation: E = diag(L) (x) I on every bond
-      for (int k = 0; k < z; ++k) weighted_message<S>(ctx, p, s, k, raw + raw_off[p->out_msg(s, k)]);
+      // first sweep after AbsorbTPO / Truncation: E = diag(L) (x) I on every bond;
+      // pairs that the merged sweep takes get both messages from one pass
+      bool done[HBP_MAX_ORDER] = {false};
+      if (z >= 3) {
+        const int pairs[2] = {z == 4 ? 0 : -1, z / 2};
+        for (int t : pairs) {
+          if (t < 0 || !(t == 0 || t + 2 == z) || !merge_ok(p, sh, t, t + 1)) continue;
+          pair_messages_diag<S>(ctx, p, s, sh, order, t, raw + raw_off[p->out_msg(s, t)],
+                                raw + raw_off[p->out_msg(s, t + 1)]);
+          done[t] = done[t + 1] = true;
+        }
+      }
+      for (int k = 0; k < z; ++k)
+        if (!done[k]) weighted_message<S>(ctx, p, s, k, raw + raw_off[p->out_msg(s, k)]);
       continue;
     }
     if (z <= 2) {
@@ -844,8 +910,8 @@ static void sweep_raw(Ctx *ctx, Peps *p, double2 *raw, const std::vector<int64_t
       int nprod = a1 - a0, w = (nprod % 2 == 1) ? 0 : 1;
       if (t1 - t0 == 2 && merge_ok(p, sh, t0, t0 + 1)) {
         // merged messages want the target pair in front (whole contiguous
-        // D_t x D_{t+1} slices): when bonds precede it, rearrange V once into
-        // buf[2] as (pair, leading bonds, rest) and absorb the other bonds there
+        // D_t x D_{t+1} slices): when bonds precede it, take V as (pair,
+        // leading bonds, rest) and absorb the other bonds in that layout
         int64_t Lb = 1, R = 1;
         for (int d = 0; d < t0; ++d) Lb *= sh[d];
         for (int d = t0 + 2; d < order; ++d) R *= sh[d];
@@ -855,25 +921,8 @@ static void sweep_raw(Ctx *ctx, Peps *p, double2 *raw, const std::vector<int64_t
           pos[d] = d < t0 ? d + 2 : d <= t0 + 1 ? d - t0 : d;
           shp[pos[d]] = sh[d];
         }
-        if (Lb > 1) {
-          const size_t vb = p->site_size(s) * p->elem_bytes();
-          void *dst = p->buf[2];
-          bool have = false;
-          if (p->pair_cache < 0) plan_pair_cache(ctx, p);
-          if (p->Vpair[s]) {
-            dst = p->Vpair[s];
-            have = true;
-          } else if (p->pair_cache == 1 && !p->pending[s]) {
-            p->Vpair[s] = ctx->alloc(vb);
-            p->Vpair_bytes[s] = vb;
-            dst = p->Vpair[s];
-          }
-          if (!have)
-            launch_swap_blocks(ctx->stream, Vs, dst, Lb, (int64_t)sh[t0] * sh[t0 + 1], R,
-                               (int)(2 * sizeof(S)));
-          Vs = dst;
-          cur = Vs;
-        }
+        Vs = pair_front<S>(ctx, p, s, Vs, sh, order, t0);
+        cur = Vs;
         for (int j = a0; j < a1; ++j) {
           void *dst = p->buf[w];
           mode_prod<S>(ctx, cur, p->E + p->E_off[p->in_msg(s, j)], dst, shp, order, pos[j], sh[j],
