@@ -61,6 +61,9 @@ def parse():
                     help="untimed steps allowed for the bond dimension to reach chi")
     ap.add_argument("--cpu-baseline", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--kernel-timing", type=int, default=1,
+                    help="time the hot kernels with HIP events carried in their dispatches "
+                         "(0 for runs under an external profiler, which times them itself)")
     return ap.parse_args()
 
 
@@ -188,7 +191,7 @@ def run_path(args, precision, mode, want_cpu_sample=False):
     for _ in range(args.warmup):
         step(None)
     ctx.synchronize()
-    check(ctx.lib.hbp_ctx_profile(ctx.h, 1))
+    check(ctx.lib.hbp_ctx_profile(ctx.h, 1 if args.kernel_timing else 0))
     timed = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
