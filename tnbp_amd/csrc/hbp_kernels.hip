@@ -239,11 +239,14 @@ __device__ __forceinline__ void mode_product_tiles(
 // its k-steps) are in flight while the current tile's MFMAs run, so HBM
 // latency is hidden by one tile of work (256 MFMAs per wave at 64 columns)
 // rather than by one k-step.  The same k order and the same step() as
-// mode_product_tiles: outputs are bitwise equal.
-template <typename S, int NCB, int RB>
+// mode_product_tiles: outputs are bitwise equal.  FULL: Din = Dout = 64 and
+// every tile whole (Q a multiple of the tile rows), so that no load or store
+// needs a bounds test: with them the compiler issues the tile's loads one at a
+// time, each followed by a wait.
+template <typename S, int NCB, int RB, bool FULL>
 __device__ __forceinline__ void mode_product_tiles_pf(
     const typename Cplx<S>::type *__restrict__ T, const double2 *__restrict__ E,
-    typename Cplx<S>::type *__restrict__ Out, int64_t L, int Din, int Dout, int64_t Q,
+    typename Cplx<S>::type *__restrict__ Out, int64_t L, int Din_rt, int Dout_rt, int64_t Q,
     int e_trans, S scale, double *__restrict__ norm_out, int64_t b0, int64_t nb,
     S (*er)[16 * NCB + 1], S (*ei)[16 * NCB + 1], double *red) {
   using C = typename Cplx<S>::type;
@@ -252,6 +255,7 @@ __device__ __forceinline__ void mode_product_tiles_pf(
   constexpr int NC = 16 * NCB;
   constexpr int64_t ROWS = 4 * 16 * RB;
   constexpr int KS = MP_KC / 4;  // k-steps of a tile
+  const int Din = FULL ? MP_KC : Din_rt, Dout = FULL ? NC : Dout_rt;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int li = lane & 15, lk = lane >> 4;
   const int64_t ntiles = (Q + ROWS - 1) / ROWS;
@@ -259,7 +263,8 @@ __device__ __forceinline__ void mode_product_tiles_pf(
   for (int idx = threadIdx.x; idx < MP_KC * NC; idx += blockDim.x) {
     const int k = idx / NC, j = idx - k * NC;
     double2 e = make_double2(0.0, 0.0);
-    if (k < Din && j < Dout) e = e_trans ? E[j + (int64_t)Dout * k] : E[k + (int64_t)Din * j];
+    if (FULL || (k < Din && j < Dout))
+      e = e_trans ? E[j + (int64_t)Dout * k] : E[k + (int64_t)Din * j];
     er[k][j] = (S)e.x;
     ei[k][j] = (S)e.y;
   }
@@ -270,16 +275,22 @@ __device__ __forceinline__ void mode_product_tiles_pf(
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb) {
       const int64_t q = row0 + rb * 16 + li;
-      const bool valid = q < Q;
+      const bool valid = FULL || q < Q;
       const int64_t qq = valid ? q : 0;
-      const C *base = T + (qq % L) + L * (int64_t)Din * (qq / L);
+      const C *base = T + (qq % L) + L * (int64_t)Din * (qq / L) + L * (int64_t)lk;
 #pragma unroll
       for (int s = 0; s < KS; ++s) {
-        const int k = 4 * s + lk;
-        const bool ok = valid && k < Din;
-        const C a = base[L * (int64_t)(ok ? k : 0)];
-        re[s][rb] = ok ? (S)a.x : (S)0;
-        im[s][rb] = ok ? (S)a.y : (S)0;
+        if (FULL) {
+          const C a = base[L * (int64_t)(4 * s)];
+          re[s][rb] = (S)a.x;
+          im[s][rb] = (S)a.y;
+        } else {
+          const int k = 4 * s + lk;
+          const bool ok = valid && k < Din;
+          const C a = base[L * (int64_t)(ok ? 4 * s : -lk)];
+          re[s][rb] = ok ? (S)a.x : (S)0;
+          im[s][rb] = ok ? (S)a.y : (S)0;
+        }
       }
     }
   };
@@ -295,7 +306,7 @@ __device__ __forceinline__ void mode_product_tiles_pf(
       }
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
-      if (4 * s >= Din) break;
+      if (!FULL && 4 * s >= Din) break;
 #pragma unroll
       for (int cb = 0; cb < NCB; ++cb) {
         const S bre = er[4 * s + lk][cb * 16 + li];
@@ -314,19 +325,19 @@ __device__ __forceinline__ void mode_product_tiles_pf(
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb) {
       const int64_t q = row0 + rb * 16 + li;
-      if (q >= Q) continue;
+      if (!FULL && q >= Q) continue;
       C *ob = Out + (q % L) + L * (int64_t)Dout * (q / L);
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
 #pragma unroll
         for (int cb = 0; cb < NCB; ++cb) {
           const int j = cb * 16 + M::row(lk, v);
-          if (j >= Dout) continue;
+          if (!FULL && j >= Dout) continue;
           C o;
           o.x = acc_re[rb][cb][v] * scale;
           o.y = acc_im[rb][cb][v] * scale;
           ob[L * (int64_t)j] = o;
-          nrm += (double)o.x * (double)o.x + (double)o.y * (double)o.y;
+          if (norm_out) nrm += (double)o.x * (double)o.x + (double)o.y * (double)o.y;
         }
       }
     }
@@ -335,13 +346,19 @@ __device__ __forceinline__ void mode_product_tiles_pf(
   S reA[KS][RB], imA[KS][RB], reB[KS][RB], imB[KS][RB];
   int64_t tile = b0;
   if (tile < ntiles) load_tile(tile, reA, imA);
+  // the scheduler is kept from interleaving the two tiles' phases (it would
+  // keep both tiles' accumulators and addresses live and spill)
   for (; tile < ntiles; tile += 2 * nb) {
     const int64_t t1 = tile + nb;
     if (t1 < ntiles) load_tile(t1, reB, imB);
+    __builtin_amdgcn_sched_barrier(0);
     compute_store(tile, reA, imA);
+    __builtin_amdgcn_sched_barrier(0);
     if (t1 >= ntiles) break;
     if (t1 + nb < ntiles) load_tile(t1 + nb, reA, imA);
+    __builtin_amdgcn_sched_barrier(0);
     compute_store(t1, reB, imB);
+    __builtin_amdgcn_sched_barrier(0);
   }
   if (norm_out) {
     const double t = block_sum(nrm, red);
@@ -357,10 +374,11 @@ __global__ __launch_bounds__(256, OCC) void k_mode_product(
   __shared__ S er[MP_KC][16 * NCB + 1];
   __shared__ S ei[MP_KC][16 * NCB + 1];
   __shared__ double red[8];
-  if constexpr (PFN == 16)
-    mode_product_tiles_pf<S, NCB, RB>(T, E, Out, L, Din, Dout, Q, e_trans, scale,
-                                      norm_partials ? norm_partials + blockIdx.x : nullptr,
-                                      blockIdx.x, gridDim.x, er, ei, red);
+  if constexpr (PFN == 16 || PFN == 17)  // 17: the FULL form
+    mode_product_tiles_pf<S, NCB, RB, PFN == 17>(T, E, Out, L, Din, Dout, Q, e_trans, scale,
+                                                 norm_partials ? norm_partials + blockIdx.x
+                                                               : nullptr,
+                                                 blockIdx.x, gridDim.x, er, ei, red);
   else
     mode_product_tiles<S, NCB, RB, PFN>(T, E, Out, L, Din, Dout, Q, e_trans, scale,
                                         norm_partials ? norm_partials + blockIdx.x : nullptr,
@@ -1410,6 +1428,14 @@ int ilog2(int64_t x) {
 // run(v) launches variant v on `st`; returns the variant to use for `key`,
 // timing all nv variants (twice each, best of two) the first time.  The
 // chosen variant is launched last, so the output is its output.
+// HBP_AUTOTUNE_LOG=1 prints every timed trial to stderr.
+bool tune_log() {
+  static const bool on = [] {
+    const char *e = getenv("HBP_AUTOTUNE_LOG");
+    return e && e[0] == '1';
+  }();
+  return on;
+}
 // HBP_FORCE_VARIANT=v runs variant v wherever a shape class has it (else 0):
 // for tests that compare the variants' outputs bitwise.
 int forced_variant() {
@@ -1451,6 +1477,9 @@ void tuned_launch(hipStream_t st, const TuneKey &key, int nv, F run) {
       HBP_HIP(hipEventSynchronize(b));
       float ms = 0.0f;
       HBP_HIP(hipEventElapsedTime(&ms, a, b));
+      if (tune_log())
+        fprintf(stderr, "[hbp autotune] kind=%d elem=%d ncb=%d log2Q=%d aux=%d variant %d: %.4f ms\n",
+                key.kind, key.elem, key.ncb, key.qlog, key.aux, v, ms);
       if (ms < best_ms) {
         best_ms = ms;
         best = v;
@@ -1484,7 +1513,9 @@ std::string autotune_report() {
 // Mode-product variants for up to 64 output columns: (rows per wave, loads in
 // flight, occupancy) = (32, 1 step, base), (32, 4 steps, base), (16, 1 step,
 // base + 1), (16, 4 steps, base + 1); base is 2 (FP64) or 3 (FP32).  For
-// FP32 with Din <= 64 also (16, next tile, base) (in FP64 its registers spill).
+// FP32 with Din <= 64 also (16, next tile, base) (in FP64 its registers
+// spill), and for FP32 64 -> 64 on whole tiles (16, next tile, 2) without
+// bounds tests.
 template <typename S, int NCB>
 static void mp_variant(int v, hipStream_t st, const void *T, const double2 *E, void *Out,
                        int64_t L, int Din, int Dout, int64_t Q, int e_trans, double scale,
@@ -1495,9 +1526,17 @@ static void mp_variant(int v, hipStream_t st, const void *T, const double2 *E, v
     case 1: mp_launch<S, NCB, 2, 4, B>(st, T, E, Out, L, Din, Dout, Q, e_trans, scale, parts, np); break;
     case 2: mp_launch<S, NCB, 1, 0, B + 1>(st, T, E, Out, L, Din, Dout, Q, e_trans, scale, parts, np); break;
     case 3: mp_launch<S, NCB, 1, 4, B + 1>(st, T, E, Out, L, Din, Dout, Q, e_trans, scale, parts, np); break;
-    default:
+    case 4:
       if constexpr (sizeof(S) == 4)
         mp_launch<S, NCB, 1, 16, B>(st, T, E, Out, L, Din, Dout, Q, e_trans, scale, parts, np);
+      else
+        HBP_REQUIRE(false, HBP_ERR_LOGIC, "mode product: no such variant");
+      break;
+    default:
+      // FP32, whole tiles of 64 -> 64 columns (the host checks the shape), two
+      // workgroups per CU (two tiles of operands per wave in registers)
+      if constexpr (NCB == 4 && sizeof(S) == 4)
+        mp_launch<S, NCB, 1, 17, 2>(st, T, E, Out, L, Din, Dout, Q, e_trans, scale, parts, np);
       else
         HBP_REQUIRE(false, HBP_ERR_LOGIC, "mode product: no such variant");
       break;
@@ -1514,13 +1553,21 @@ void launch_mode_product(hipStream_t st, const void *T, const double2 *E, void *
   const int64_t Q = L * R;
   const int ncb = (Dout + 15) / 16;
   constexpr int B = Occ<S>::v;
-  const TuneKey key{0, (int)sizeof(S), ncb, ilog2(Q), (L == 1 ? 1 : 0) + 2 * (Din > MP_KC ? 1 : 0)};
+  // the whole-tile prefetch variants need the full k range in one LDS stage;
+  // variant 5 (no bounds tests) whole 64-row tiles of 64 -> 64 columns.  The
+  // shape class records which of them exist (the tuned choice is an index).
+  const bool full = sizeof(S) == 4 && Din == MP_KC && Dout == 64 && Q % 64 == 0 && !e_trans;
+  int vars[6] = {0, 1, 2, 3, 0, 0}, nv = 4;
+  if (Din <= MP_KC && sizeof(S) == 4) vars[nv++] = 4;
+  if (full) vars[nv++] = 5;
+  const TuneKey key{0, (int)sizeof(S), ncb, ilog2(Q),
+                    (L == 1 ? 1 : 0) + 2 * (Din > MP_KC ? 1 : 0) + 4 * (full ? 1 : 0) +
+                        8 * (Din <= MP_KC ? 1 : 0)};
   auto go = [&](auto ncb_tag, int v) {
     constexpr int N = decltype(ncb_tag)::value;
-    mp_variant<S, N>(v, st, T, E, Out, L, Din, Dout, Q, e_trans, scale, norm_partials, n_partials);
+    mp_variant<S, N>(vars[v], st, T, E, Out, L, Din, Dout, Q, e_trans, scale, norm_partials,
+                     n_partials);
   };
-  // the whole-tile prefetch variants need the full k range in one LDS stage
-  const int nv = (Din <= MP_KC && sizeof(S) == 4) ? 5 : 4;
   switch (ncb) {
     case 1: tuned_launch(st, key, nv, [&](int v) { go(std::integral_constant<int, 1>{}, v); }); break;
     case 2: tuned_launch(st, key, nv, [&](int v) { go(std::integral_constant<int, 2>{}, v); }); break;
