@@ -62,10 +62,10 @@ def _run(variant):
 
 
 def test_forced_variants_bitwise_equal():
-    announce("launch variants 0..5 forced, bitwise comparison")
+    announce("launch variants 0..6 forced, bitwise comparison")
     ref = _run(0)
     # the absorbed bonds reached 2 chi in both lattices
     assert [ref[k][1] for k in ("0/20", "0/32", "1/20", "1/32")] == [40, 64, 40, 64]
-    for v in range(1, 6):
+    for v in range(1, 7):
         got = _run(v)
         assert got == ref, (v, got, ref)
