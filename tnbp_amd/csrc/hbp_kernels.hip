@@ -1122,21 +1122,22 @@ __global__ void k_build_weights(const double2 *__restrict__ E, double2 *__restri
   }
 }
 
-// M[e] = sum_c p[c n + e].  A block owns 16 entries; its 16 thread groups
-// each sum the chunks g, g + 16, ... in order and the 16 group sums are added
+// M[e] = sum_c p[c n + e].  A block owns 16 entries; its G thread groups
+// each sum the chunks g, g + G, ... in order and the G group sums are added
 // in a fixed order, so the result is bitwise reproducible.
-constexpr int RED_E = RED_ENTRIES, RED_G = 16;
+constexpr int RED_E = RED_ENTRIES;
 // Partials of FP32 accumulators may be stored as float2: the conversion to
 // double is exact, so the sums equal those of double2 partials bitwise.
-template <typename PT = double2>
+template <typename PT, int G>
 __device__ __forceinline__ void reduce_entries(const PT *__restrict__ p, int nchunks,
                                                int n, double2 *__restrict__ M, int64_t blk) {
-  __shared__ double sre[RED_G][RED_E + 1], sim[RED_G][RED_E + 1];
+  __shared__ double sre[G][RED_E + 1], sim[G][RED_E + 1];
   const int el = threadIdx.x % RED_E, g = threadIdx.x / RED_E;
   const int64_t e = blk * RED_E + el;
   double re = 0.0, im = 0.0;
   if (e < n) {
-    for (int c = g; c < nchunks; c += RED_G) {
+#pragma unroll 4
+    for (int c = g; c < nchunks; c += G) {
       const PT v = p[(int64_t)c * n + e];
       re += (double)v.x;
       im += (double)v.y;
@@ -1147,7 +1148,7 @@ __device__ __forceinline__ void reduce_entries(const PT *__restrict__ p, int nch
   __syncthreads();
   if (g == 0 && e < n) {
     double tr = 0.0, ti = 0.0;
-    for (int k = 0; k < RED_G; ++k) {
+    for (int k = 0; k < G; ++k) {
       tr += sre[k][el];
       ti += sim[k][el];
     }
@@ -1155,10 +1156,14 @@ __device__ __forceinline__ void reduce_entries(const PT *__restrict__ p, int nch
   }
 }
 
+// 64 groups of 16 threads: each thread sums at most nchunks / 64 partials
+// (8 of the merged kernels' 512), so that enough loads are in flight
+constexpr int RED_G = 64;
 template <typename PT>
-__global__ __launch_bounds__(256) void k_reduce_partials(const PT *__restrict__ p, int nchunks,
-                                                          int n, double2 *__restrict__ M) {
-  reduce_entries<PT>(p, nchunks, n, M, blockIdx.x);
+__global__ __launch_bounds__(RED_G * RED_E) void k_reduce_partials(const PT *__restrict__ p,
+                                                                    int nchunks, int n,
+                                                                    double2 *__restrict__ M) {
+  reduce_entries<PT, RED_G>(p, nchunks, n, M, blockIdx.x);
 }
 
 __global__ __launch_bounds__(256) void k_reduce_grouped(const RedJob *__restrict__ jobs,
@@ -1169,7 +1174,7 @@ __global__ __launch_bounds__(256) void k_reduce_grouped(const RedJob *__restrict
     if (jobs[mid].first_block <= (int64_t)blockIdx.x) lo = mid; else hi = mid - 1;
   }
   const RedJob j = jobs[lo];
-  reduce_entries(j.partials, j.nch, j.n, j.M, (int64_t)blockIdx.x - j.first_block);
+  reduce_entries<double2, 16>(j.partials, j.nch, j.n, j.M, (int64_t)blockIdx.x - j.first_block);
 }
 
 // ------------------------------------------------------------ elementwise
@@ -1944,14 +1949,16 @@ void launch_build_weights(hipStream_t st, const double2 *E, double2 *w, int64_t 
 
 void launch_reduce_partials(hipStream_t st, const double2 *partials, int nchunks, int n,
                             double2 *M) {
-  hipLaunchKernelGGL(k_reduce_partials<double2>, dim3((n + RED_E - 1) / RED_E), dim3(256), 0, st,
+  hipLaunchKernelGGL(k_reduce_partials<double2>, dim3((n + RED_E - 1) / RED_E),
+                     dim3(RED_G * RED_E), 0, st,
                      partials, nchunks, n, M);
   HBP_HIP(hipGetLastError());
 }
 
 void launch_reduce_partials(hipStream_t st, const float2 *partials, int nchunks, int n,
                             double2 *M) {
-  hipLaunchKernelGGL(k_reduce_partials<float2>, dim3((n + RED_E - 1) / RED_E), dim3(256), 0, st,
+  hipLaunchKernelGGL(k_reduce_partials<float2>, dim3((n + RED_E - 1) / RED_E),
+                     dim3(RED_G * RED_E), 0, st,
                      partials, nchunks, n, M);
   HBP_HIP(hipGetLastError());
 }
