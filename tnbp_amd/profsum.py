@@ -3,6 +3,7 @@ and per-dispatch PMC counters), so that only a few KB of text need keeping.
 
     python -m tnbp_amd.profsum stats <results.db> --title T --cmd "C"
     python -m tnbp_amd.profsum pmc <counter_collection.csv> --title T --cmd "C"
+    python -m tnbp_amd.profsum util <counter_collection.csv> --title T --cmd "C"
     python -m tnbp_amd.profsum traffic <fetch.csv> <write.csv> <probe_fetch.csv> \
         <probe_write.csv> --pattern REGEX --precision P --key K --alg-bytes B --launch TEXT
 """
@@ -59,6 +60,48 @@ def pmc(path, title, cmd):
     return "\n".join(out) + "\n"
 
 
+def util(path, title, cmd, simds=1024, xcds=8, min_us=20.0):
+    """Per kernel and grid size (dispatches longer than min_us): mean duration, the shader
+    clock (GRBM_GUI_ACTIVE, which rocprofv3 sums over the XCDs, / XCDs /
+    duration), the fraction of SIMD cycles the MFMA unit is busy
+    (SQ_VALU_MFMA_BUSY_CYCLES / (GUI_ACTIVE per XCD x SIMDs)), the MFMA flops
+    (SQ_INSTS_VALU_MFMA_MOPS_* x 512) and LDS bank-conflict cycles per active
+    LDS cycle."""
+    disp = defaultdict(dict)
+    meta = {}
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            d = int(r["Dispatch_Id"])
+            disp[d][r["Counter_Name"]] = float(r["Counter_Value"])
+            meta[d] = ((short(r["Kernel_Name"]), int(r["Grid_Size"])),
+                       (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-9)
+    agg = defaultdict(lambda: defaultdict(float))
+    for d, c in disp.items():
+        name, dur = meta[d]
+        if dur * 1e6 < min_us:
+            continue
+        a = agg[name]
+        a["n"] += 1
+        a["dur"] += dur
+        for k, v in c.items():
+            a[k] += v
+    out = [f"# {title}\n", f"Command (repository root):\n\n```\n{cmd}\n```\n",
+           f"Dispatches of at least {min_us:g} us; {xcds} XCDs, {simds} SIMDs.\n",
+           "| kernel | grid | dispatches | mean us | clock GHz | MFMA busy | MFMA GFLOP / "
+           "dispatch | MFMA TFLOP/s | LDS conflict / active |\n|---|---|---|---|---|---|---|---|---|"]
+    for (name, grid), a in sorted(agg.items(), key=lambda kv: -kv[1]["dur"]):
+        n, dur = a["n"], a["dur"] / a["n"]
+        gui = a.get("GRBM_GUI_ACTIVE", 0.0) / n / xcds
+        busy = a.get("SQ_VALU_MFMA_BUSY_CYCLES", 0.0) / n
+        mops = sum(v for k, v in a.items() if k.startswith("SQ_INSTS_VALU_MFMA_MOPS")) / n * 512
+        lds = a.get("SQ_LDS_IDX_ACTIVE", 0.0)
+        conf = a.get("SQ_LDS_BANK_CONFLICT", 0.0) / lds if lds else float("nan")
+        out.append(f"| `{name}` | {grid} | {int(n)} | {dur * 1e6:.1f} | {gui / dur / 1e9:.3f} | "
+                   f"{busy / (gui * simds) if gui else float('nan'):.3f} | {mops / 1e9:.2f} | "
+                   f"{mops / dur / 1e12:.1f} | {conf:.3f} |")
+    return "\n".join(out) + "\n"
+
+
 def _max_counter(path, pattern):
     best = 0.0
     with open(path) as f:
@@ -97,7 +140,7 @@ def traffic(fetch, write, probe_fetch, probe_write, pattern, precision, key, alg
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("kind", choices=["stats", "pmc", "traffic"])
+    ap.add_argument("kind", choices=["stats", "pmc", "util", "traffic"])
     ap.add_argument("path", nargs="+")
     ap.add_argument("--title", default="rocprofv3 summary")
     ap.add_argument("--cmd", default=None, action="append")
@@ -114,6 +157,8 @@ def main():
         print(stats(a.path[0], a.title, cmd), end="")
     elif a.kind == "pmc":
         print(pmc(a.path[0], a.title, cmd), end="")
+    elif a.kind == "util":
+        print(util(a.path[0], a.title, cmd), end="")
     else:
         t = traffic(*a.path[:4], a.pattern, a.precision, a.key, a.alg_bytes, a.launch,
                     cmds, a.out)
