@@ -48,7 +48,7 @@ def parse():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
     ap.add_argument("--mode", choices=["fused", "plain"], default="fused",
                     help="plain: the reference's op sequence one TCI call at a time")
-    ap.add_argument("--sweep", choices=["auto", "per-site", "grouped", "merged"], default="auto",
+    ap.add_argument("--sweep", choices=["auto", "per-site", "merged"], default="auto",
                     help="BP sweep launch form (grouped needs bond dims <= 32; merged computes "
                          "each message's last mode product inside the message kernel)")
     ap.add_argument("--compare", type=int, default=1,
@@ -160,7 +160,7 @@ def run_path(args, precision, mode, want_cpu_sample=False):
     ctx = Context(0)
     peps = InitTensorProductState(ctx, edges, [2] * len(sites), prec)
     check(ctx.lib.hbp_peps_set_mode(peps.h, 0 if mode == "fused" else 1))
-    peps.set_sweep_kernel({"auto": 0, "per-site": 1, "grouped": 2, "merged": 3}[args.sweep])
+    peps.set_sweep_kernel({"auto": 0, "per-site": 1, "merged": 3}[args.sweep])
 
     def step(record):
         t0 = time.perf_counter()

@@ -208,8 +208,9 @@ hbp_status hbp_peps_set_absorb_mode(hbp_peps *p, int mode);
  * column-block count - five to ten launches per sweep instead of ~12 per
  * site; needs site degree <= 4 and bond dims <= 32.  AUTO (default) is
  * MERGED (which falls back to PER_SITE launches wherever bonds are outside
- * (32, 64]): GROUPED is experimental and taken only on request.  No reference
- * counterpart. */
+ * (32, 64]).  GROUPED is disabled: the call returns HBP_ERR_UNSUPPORTED (its
+ * first GPU run left the device unusable and the cause is not found).  No
+ * reference counterpart. */
 enum { HBP_SWEEP_AUTO = 0, HBP_SWEEP_PER_SITE = 1, HBP_SWEEP_GROUPED = 2, HBP_SWEEP_MERGED = 3 };
 hbp_status hbp_peps_set_sweep_kernel(hbp_peps *p, int kind);
 hbp_status hbp_peps_num_sites(const hbp_peps *p, int *n_sites, int *n_edges);

@@ -264,6 +264,19 @@ def test_lazy_and_eager_agree_on_merged_sweeps(ctx, precision):
         np.testing.assert_array_equal(a, b)
 
 
+def test_grouped_sweep_is_rejected(ctx):
+    """The grouped sweep is switched off at the API (its first GPU run left the
+    device unusable and the cause is not found): asking for it is an error and
+    the PEPS keeps its sweep kernel."""
+    from tnbp_amd import circuits as pc
+    from tnbp_amd.peps import HBP_SWEEP_GROUPED, InitTensorProductState
+    announce("grouped sweep rejected")
+    p = InitTensorProductState(ctx, pc.square_lattice(2, 2), [2] * 4, 0)
+    with pytest.raises(NotImplementedError, match="grouped sweep is disabled"):
+        p.set_sweep_kernel(HBP_SWEEP_GROUPED)
+    p.close()
+
+
 @pytest.mark.skip(reason="grouped sweep disabled: its first GPU run left the device unusable "
                          "and the cause is not yet found")
 @pytest.mark.parametrize("precision,tol", [(0, 1e-12), (1, 1e-6)])
