@@ -70,11 +70,12 @@ hbp_status hbp_ctx_destroy(hbp_ctx *ctx);
 hbp_status hbp_ctx_synchronize(hbp_ctx *ctx);
 /* The HIP stream all work of this context runs on (a hipStream_t). */
 hbp_status hbp_ctx_stream(hbp_ctx *ctx, void **stream);
-/* Bytes of live allocations of the context (and their peak).  Freed large
- * blocks (>= 32 MiB) are kept for reuse by the context and do not count. */
+/* Bytes of live allocations of the context (and their peak), each counted at
+ * its size class (a power of two from 256 B below 32 MiB, 2 MiB steps above).
+ * Freed blocks kept by the context for reuse do not count. */
 hbp_status hbp_ctx_memory(hbp_ctx *ctx, size_t *in_use, size_t *peak);
-/* Return the context's unused device memory (kept large blocks and the
- * pool's reserve) to the device (after a job ends) and reset the peak to the
+/* Return the context's unused device memory (kept blocks and the pool's
+ * reserve) to the device (after a job ends) and reset the peak to the
  * bytes still in use.  No reference counterpart. */
 hbp_status hbp_ctx_trim(hbp_ctx *ctx);
 /* Kernel launch variants that compute bitwise the same result (rows per

@@ -112,6 +112,19 @@ def test_footprint_model_scales(monkeypatch):
     assert 4e9 < serving.footprint_bytes(k) < 10e9
 
 
+def test_small_jobs_absorb_eagerly(monkeypatch):
+    monkeypatch.undo()
+    rng = np.random.default_rng(1)
+    small = serving.random_circuit_job(rng, 4, 2, 8, 4)
+    assert serving.absorbs_eagerly(small)
+    # the eager footprint holds the absorbed lattice on top of the lazy one
+    assert serving.footprint_bytes(small) > serving.absorbed_bytes(small)
+    # 16x16 chi=32: the absorbed lattice (~52 GB mixed) is absorbed site by site
+    k = serving.kicked_job(16, 32, 1, 0.7, precision=HBP_PREC_MIXED)
+    assert serving.absorbed_bytes(k) > 40e9
+    assert not serving.absorbs_eagerly(k)
+
+
 def test_no_device_fails_jobs_instead_of_hanging():
     def broken():
         raise RuntimeError("no device")

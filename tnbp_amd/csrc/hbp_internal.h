@@ -60,15 +60,19 @@ struct Ctx {
   // device memory this context can still allocate: the device's free memory
   // plus what its pool and its block cache hold in reserve
   size_t free_bytes(size_t *total = nullptr);
-  // Large blocks (>= 32 MiB, sized in 2 MiB steps) are kept on release and
-  // handed out again for the same size: every step of a PEPS run frees and
-  // re-allocates ~D^4 d-sized site tensors, and getting that memory back from
-  // the driver each time stalls the host for seconds.  Reuse is safe because
-  // all of a context's work is ordered on its one stream.  flush_blocks()
-  // gives the cache back (trim, failed allocations, destroy).
+  // Released blocks are kept and handed out again for the same size class:
+  // large blocks (>= 32 MiB, 2 MiB steps) because every step of a PEPS run
+  // frees and re-allocates ~D^4 d-sized site tensors and getting that memory
+  // back from the driver each time stalls the host for seconds; small blocks
+  // (powers of two from 256 B, at most 256 MiB of them kept) because a run at
+  // small bond dimensions allocates and frees work arrays thousands of times
+  // per step, and each pool round trip costs more host time than the kernels
+  // it serves.  Reuse is safe because all of a context's work is ordered on
+  // its one stream.  flush_blocks() gives the cache back (trim, failed
+  // allocations, destroy).
   std::multimap<size_t, void *> free_blocks;
-  std::unordered_map<void *, size_t> big_sizes;  // live large block -> size
-  size_t cached = 0;
+  std::unordered_map<void *, size_t> block_sizes;  // live cached-class block -> size
+  size_t cached = 0, cached_small = 0;
   void flush_blocks();
   void *get_scratch(size_t bytes);
   void sync();

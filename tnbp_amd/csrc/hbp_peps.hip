@@ -1,7 +1,7 @@
 // PEPS layer: the state (site tensors V, messages E) lives in HBM and each
 // tnbp algorithm step runs as a handful of kernel launches.
 //
-//   AbsorbTPO            apply.h:14-177        k_absorb_site, k_expand_message
+//   AbsorbTPO            apply.h:14-177        k_absorb_site, k_expand_messages
 //   BeliefPropagation    bp.h:65-261           mode products + message kernel
 //   BP condition         bp.h:266-355          reused from the next sweep (same messages)
 //   Truncation           truncation.h:17-222   batched Jacobi SVD, k_roots, k_rr, k_projectors
@@ -182,13 +182,39 @@ __global__ void k_absorb_site(const C *__restrict__ V, const double2 *__restrict
 }
 
 // E'[(e0 i0), (e1 i1)] = E[e0, e1] delta(i0, i1)  (apply.h:110-128)
-__global__ void k_expand_message(const double2 *__restrict__ E, double2 *__restrict__ En, int D,
-                                 int t) {
-  const int Dn = D * t;
-  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < Dn * Dn; e += gridDim.x * blockDim.x) {
+// E_e <- E_e (x) I_t for every message in one launch: block b expands
+// message jobs[b] (apply.h:110-128), element (e0 + D i0, e1 + D i1) of the
+// (D t) x (D t) result = E[e0, e1] if i0 == i1 else 0.
+struct ExpandJob {
+  int64_t src, dst;  // offsets (in elements) of the old and the new message
+  int D, t;
+};
+__global__ __launch_bounds__(256) void k_expand_messages(const double2 *__restrict__ E,
+                                                         double2 *__restrict__ En,
+                                                         const ExpandJob *__restrict__ jobs) {
+  const ExpandJob j = jobs[blockIdx.x];
+  const int D = j.D, Dn = j.D * j.t;
+  for (int e = threadIdx.x; e < Dn * Dn; e += blockDim.x) {
     const int r = e % Dn, c = e / Dn;
     const int e0 = r % D, i0 = r / D, e1 = c % D, i1 = c / D;
-    En[e] = (i0 == i1) ? E[e0 + D * e1] : make_double2(0.0, 0.0);
+    En[j.dst + e] = (i0 == i1) ? E[j.src + e0 + D * e1] : make_double2(0.0, 0.0);
+  }
+}
+
+// Copies of n-element blocks to two places each (after truncation both
+// directed messages of an edge are diag(Lambda)): block b does jobs[b].
+struct CopyJob {
+  int64_t src, dst0, dst1;
+  int n;
+};
+__global__ __launch_bounds__(256) void k_copy_twice(const double2 *__restrict__ src,
+                                                    double2 *__restrict__ dst,
+                                                    const CopyJob *__restrict__ jobs) {
+  const CopyJob j = jobs[blockIdx.x];
+  for (int e = threadIdx.x; e < j.n; e += blockDim.x) {
+    const double2 v = src[j.src + e];
+    dst[j.dst0 + e] = v;
+    dst[j.dst1 + e] = v;
   }
 }
 
@@ -1322,26 +1348,26 @@ static void apply_projectors(Ctx *ctx, Peps *p, const double2 *proj,
       maxn = n;
     }
     ensure_work(ctx, p, std::max(p->buf_bytes, maxn * p->elem_bytes()));
+    size_t n = p->phys[s];
+    for (int k = 0; k < z; ++k) n *= (size_t)keep[p->bonds[s][k]];
+    const size_t bytes = n * p->elem_bytes();
+    void *nv = ctx->alloc(bytes);
     const void *cur = site_data(ctx, p, s);
     int which = 0;
     for (int k = 0; k < z; ++k) {
       const int m = p->bonds[s][k];
-      const int D = old_dim[m], kp = keep[m];
-      void *dst = p->buf[which];
+      const int kp = keep[m];
+      // the last projector writes the new site tensor itself
+      void *dst = k + 1 == z ? nv : p->buf[which];
       if (p->ea[m] == s)
         mode_prod<S>(ctx, cur, proj + ej[m].pa_off, dst, sh, z + 1, k, kp, 0);
       else
         mode_prod<S>(ctx, cur, proj + ej[m].pb_off, dst, sh, z + 1, k, kp, 1);
-      (void)D;
       sh[k] = kp;
       cur = dst;
       which ^= 1;
     }
-    size_t n = p->phys[s];
-    for (int k = 0; k < z; ++k) n *= (size_t)sh[k];
-    const size_t bytes = n * p->elem_bytes();
-    void *nv = ctx->alloc(bytes);
-    HBP_HIP(hipMemcpyAsync(nv, cur, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    if (z == 0) HBP_HIP(hipMemcpyAsync(nv, cur, bytes, hipMemcpyDeviceToDevice, ctx->stream));
     // normalise the site (truncation.h:185, :216)
     double *dv = (double *)ctx->alloc(sizeof(double));
     device_norm2(ctx, nv, n, p->site_dtype(), dv);
@@ -1440,10 +1466,14 @@ static void truncate(Ctx *ctx, Peps *p, int chi, double sv_min, int32_t *dims_ou
   size_t nbytes;
   std::vector<int64_t> noff;
   layout_messages(ctx, p, &nE, &nbytes, &noff);
-  for (int m = 0; m < ne; ++m) {
-    const size_t b = (size_t)keep[m] * keep[m] * 16;
-    HBP_HIP(hipMemcpyAsync(nE + noff[m], proj + ej[m].e_off, b, hipMemcpyDeviceToDevice, ctx->stream));
-    HBP_HIP(hipMemcpyAsync(nE + noff[m + ne], proj + ej[m].e_off, b, hipMemcpyDeviceToDevice, ctx->stream));
+  if (ne > 0) {
+    std::vector<CopyJob> cj(ne);
+    for (int m = 0; m < ne; ++m) cj[m] = CopyJob{ej[m].e_off, noff[m], noff[m + ne], keep[m] * keep[m]};
+    CopyJob *dcj = (CopyJob *)ctx->alloc(ne * sizeof(CopyJob));
+    ctx->upload(dcj, cj.data(), ne * sizeof(CopyJob));
+    hipLaunchKernelGGL(k_copy_twice, dim3(ne), dim3(256), 0, ctx->stream, proj, nE, dcj);
+    HBP_HIP(hipGetLastError());
+    ctx->release(dcj, ne * sizeof(CopyJob));
   }
   ctx->release(p->E, p->E_bytes);
   p->E = nE;
@@ -2061,11 +2091,14 @@ hbp_status hbp_peps_absorb_tpo(hbp_ctx *ctx, hbp_peps *p, const double *const *o
   size_t nbytes;
   std::vector<int64_t> noff;
   layout_messages(ctx, p, &nE, &nbytes, &noff);
-  for (int m = 0; m < 2 * ne; ++m) {
-    const int D = old[m % ne], tt = t[m % ne];
-    hipLaunchKernelGGL(k_expand_message, dim3(gridn((size_t)D * tt * D * tt)), dim3(256), 0,
-                       ctx->stream, p->E + old_off[m], nE + noff[m], D, tt);
+  if (ne > 0) {
+    std::vector<ExpandJob> xj(2 * ne);
+    for (int m = 0; m < 2 * ne; ++m) xj[m] = ExpandJob{old_off[m], noff[m], old[m % ne], t[m % ne]};
+    ExpandJob *dxj = (ExpandJob *)ctx->alloc(2 * ne * sizeof(ExpandJob));
+    ctx->upload(dxj, xj.data(), 2 * ne * sizeof(ExpandJob));
+    hipLaunchKernelGGL(k_expand_messages, dim3(2 * ne), dim3(256), 0, ctx->stream, p->E, nE, dxj);
     HBP_HIP(hipGetLastError());
+    ctx->release(dxj, 2 * ne * sizeof(ExpandJob));
   }
   ctx->release(p->E, p->E_bytes);
   p->E = nE;

@@ -53,23 +53,29 @@ size_t shape_size(int order, const int32_t *shape) {
 namespace {
 constexpr size_t kBigBlock = (size_t)32 << 20;
 constexpr size_t kBlockStep = (size_t)2 << 20;
+constexpr size_t kSmallCacheCap = (size_t)256 << 20;
+// size class of a request: 2 MiB steps from 32 MiB, powers of two below
+size_t block_class(size_t bytes) {
+  if (bytes >= kBigBlock) return (bytes + kBlockStep - 1) / kBlockStep * kBlockStep;
+  size_t c = 256;
+  while (c < bytes) c <<= 1;
+  return c;
+}
 }  // namespace
 
 void *Ctx::alloc(size_t bytes) {
   if (bytes == 0) bytes = 16;
-  const bool big = bytes >= kBigBlock;
-  if (big) {
-    bytes = (bytes + kBlockStep - 1) / kBlockStep * kBlockStep;
-    auto it = free_blocks.find(bytes);
-    if (it != free_blocks.end()) {
-      void *p = it->second;
-      free_blocks.erase(it);
-      cached -= bytes;
-      big_sizes[p] = bytes;
-      in_use += bytes;
-      peak = std::max(peak, in_use);
-      return p;
-    }
+  bytes = block_class(bytes);
+  auto it = free_blocks.find(bytes);
+  if (it != free_blocks.end()) {
+    void *p = it->second;
+    free_blocks.erase(it);
+    cached -= bytes;
+    if (bytes < kBigBlock) cached_small -= bytes;
+    block_sizes[p] = bytes;
+    in_use += bytes;
+    peak = std::max(peak, in_use);
+    return p;
   }
   void *p = nullptr;
   hipError_t e = hipMallocFromPoolAsync(&p, bytes, pool, stream);
@@ -83,7 +89,7 @@ void *Ctx::alloc(size_t bytes) {
     throw Error(HBP_ERR_BAD_ALLOC, std::string("device allocation of ") + std::to_string(bytes) +
                                        " bytes failed: " + hipGetErrorString(e));
   }
-  if (big) big_sizes[p] = bytes;
+  block_sizes[p] = bytes;
   in_use += bytes;
   peak = std::max(peak, in_use);
   return p;
@@ -91,24 +97,25 @@ void *Ctx::alloc(size_t bytes) {
 
 void Ctx::release(void *p, size_t bytes) {
   if (!p) return;
-  auto it = big_sizes.find(p);
-  if (it != big_sizes.end()) {
-    // the size recorded at allocation decides the bucket, not the caller's
-    const size_t b = it->second;
-    big_sizes.erase(it);
+  auto it = block_sizes.find(p);
+  // the size recorded at allocation decides the class, not the caller's
+  const size_t b = it != block_sizes.end() ? it->second : (bytes == 0 ? (size_t)16 : bytes);
+  if (it != block_sizes.end()) block_sizes.erase(it);
+  in_use -= std::min(in_use, b);
+  if (b >= kBigBlock || cached_small + b <= kSmallCacheCap) {
     free_blocks.emplace(b, p);
     cached += b;
-    in_use -= std::min(in_use, b);
+    if (b < kBigBlock) cached_small += b;
     return;
   }
   HBP_HIP(hipFreeAsync(p, stream));
-  in_use -= std::min(in_use, bytes == 0 ? (size_t)16 : bytes);
 }
 
 void Ctx::flush_blocks() {
   for (auto &kv : free_blocks) (void)hipFreeAsync(kv.second, stream);
   free_blocks.clear();
   cached = 0;
+  cached_small = 0;
 }
 
 void *Ctx::get_scratch(size_t bytes) {

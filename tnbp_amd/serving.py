@@ -34,7 +34,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import circuits, pauli
-from ._lib import HBP_ABSORB_LAZY, HBP_PREC_FP64, HBP_PREC_MIXED
+from ._lib import HBP_ABSORB_EAGER, HBP_ABSORB_LAZY, HBP_PREC_FP64, HBP_PREC_MIXED
 from .peps import AbsorbTPO, InitTensorProductState, Truncation, bp_loop
 from .tci import Context, device_memory
 
@@ -65,12 +65,14 @@ def _degrees(edges):
     return deg
 
 
-def footprint_bytes(job: EstimatorJob) -> int:
-    """Device bytes a job holds at its peak: resident chi-sized sites, the
-    messages (three sets during a sweep), the work buffers of the per-site
-    sweep (four of the largest site at D = chi * t) or of the grouped sweep
-    (up to seven per site at D <= 32, at most 16 GiB), split-K partials, and
-    slack.  The plain mode stores the absorbed lattice."""
+# A job whose absorbed lattice (every bond at chi * t) takes at most this many
+# bytes absorbs each layer eagerly: its sites are then absorbed once per layer
+# instead of once per BP sweep, which at small bond dimensions saves more host
+# time (launches) than the memory is worth.
+EAGER_ABSORB_MAX_BYTES = 512 << 20
+
+
+def _sizes(job: EstimatorJob):
     elem = 16 if job.precision == HBP_PREC_FP64 else 8
     deg = _degrees(job.edges)
     t = 1
@@ -78,15 +80,32 @@ def footprint_bytes(job: EstimatorJob) -> int:
         for O in tpo:
             if O.ndim > 2:
                 t = max(t, max(O.shape[:-2]))
+    return elem, deg, job.chi * t
+
+
+def absorbed_bytes(job: EstimatorJob) -> int:
+    """Bytes of the lattice with every bond at D = chi * t."""
+    elem, deg, D = _sizes(job)
+    return sum(D ** z * 2 * elem for z in deg.values())
+
+
+def absorbs_eagerly(job: EstimatorJob) -> bool:
+    return job.mode == 1 or absorbed_bytes(job) <= EAGER_ABSORB_MAX_BYTES
+
+
+def footprint_bytes(job: EstimatorJob) -> int:
+    """Device bytes a job holds at its peak: resident chi-sized sites, the
+    absorbed lattice when it absorbs eagerly (the plain mode always does), the
+    messages (three sets during a sweep), the work buffers of the per-site
+    sweep (four of the largest site at D = chi * t), split-K partials, and
+    slack."""
+    elem, deg, D = _sizes(job)
     chi, d = job.chi, 2
-    D = chi * t
     resident = sum(chi ** z * d * elem for z in deg.values())
     zmax = max(deg.values())
     work = 4 * D ** zmax * d * elem
-    if D <= 32 and zmax <= 4:
-        work = max(work, min(16 << 30, sum(7 * D ** z * d * elem for z in deg.values())))
-    if job.mode == 1:
-        work += sum(D ** z * d * elem for z in deg.values())
+    if absorbs_eagerly(job):
+        work += absorbed_bytes(job)
     msgs = 3 * 2 * len(job.edges) * D * D * 16
     partials = 1024 * min(D, 128) ** 2 * 16 * 2
     return int(1.15 * (resident + work + msgs + partials)) + (64 << 20)
@@ -97,8 +116,9 @@ def run_job(ctx: Context, job: EstimatorJob):
     (:181-240, extended to every term and the coefficient-weighted sum)."""
     sites = sorted({s for e in job.edges for s in e})
     peps = InitTensorProductState(ctx, job.edges, [2] * len(sites), job.precision)
-    # the footprint model counts chi-sized resident sites: absorb lazily
-    peps.set_absorb_mode(HBP_ABSORB_LAZY)
+    # the footprint model counts chi-sized resident sites, plus the absorbed
+    # lattice when that is small enough to absorb eagerly
+    peps.set_absorb_mode(HBP_ABSORB_EAGER if absorbs_eagerly(job) else HBP_ABSORB_LAZY)
     peps.set_mode(job.mode)
     iters = []
     try:
