@@ -52,6 +52,9 @@ struct Ctx {
   // grow-only scratch for split-K partials and reductions
   void *scratch = nullptr;
   size_t scratch_bytes = 0;
+  // grow-only arena for partials whose reduction is deferred (batched)
+  void *arena = nullptr;
+  size_t arena_bytes = 0;
   // pinned host staging for small results
   double *host_small = nullptr;
 
@@ -75,6 +78,8 @@ struct Ctx {
   size_t cached = 0, cached_small = 0;
   void flush_blocks();
   void *get_scratch(size_t bytes);
+  // the arena with at least `bytes` (contents not kept when it grows)
+  void *get_arena(size_t bytes);
   void sync();
 
   // kernel timing with HIP events on this context's stream, per kernel class
@@ -156,6 +161,23 @@ int message_partials(hipStream_t st, const void *T, const void *V, double2 *part
 // (Da, Sb): column c of a row at L (c % Da) + Sb (c / Da); Da = 0 is the plain view.
 // w[i] = prod_b diag(E_b)[c_b] over the column-major split of i into dims[0..nb);
 // n may exceed prod(dims) by a trailing unit-weight factor.
+// Row weights of a diagonal environment: w[i] = prod_b E[diag_off[b] + c_b (dims[b] + 1)]
+// over the column-major digits c_b of i (dims[0], ..., dims[nb-1], then a unit
+// factor for indices that carry no message).  Up to WEIGHT_BATCH vectors
+// are built in one launch.
+struct WeightArgs {
+  int nb;
+  int64_t dims[HBP_MAX_ORDER];
+  int64_t diag_off[HBP_MAX_ORDER];  // offset of the message's (0,0) element
+};
+constexpr int WEIGHT_BATCH = 8;
+struct WeightBatch {
+  WeightArgs a[WEIGHT_BATCH];
+  int64_t n[WEIGHT_BATCH];
+  double2 *w[WEIGHT_BATCH];
+  int count;
+};
+void launch_build_weights_batch(hipStream_t st, const double2 *E, const WeightBatch &b);
 void launch_build_weights(hipStream_t st, const double2 *E, double2 *w, int64_t n, int nb,
                           const int64_t *dims, const int64_t *diag_off);
 int64_t message_num_chunks(int64_t Q);
@@ -184,6 +206,19 @@ void launch_swap_blocks(hipStream_t st, const void *X, void *Y, int64_t L, int64
                         int64_t R, int elem_bytes);
 // M = sum over chunks (fixed order), complex<double>; complex<float>
 // partials are summed in double.
+// up to RED_BATCH fixed-order reductions of partials (FP64, or FP32 stored as
+// float2) into FP64 messages, in one launch
+constexpr int RED_BATCH = 8;
+struct RedBatchJob {
+  const void *partials;
+  double2 *M;
+  int nch, n, f32;
+};
+struct RedBatch {
+  RedBatchJob j[RED_BATCH];
+  int count;
+};
+void launch_reduce_batch(hipStream_t st, const RedBatch &b);
 void launch_reduce_partials(hipStream_t st, const double2 *partials, int nchunks, int n,
                             double2 *M);
 void launch_reduce_partials(hipStream_t st, const float2 *partials, int nchunks, int n,

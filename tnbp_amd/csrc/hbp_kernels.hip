@@ -1106,27 +1106,34 @@ __global__ __launch_bounds__(256) void k_swap_blocks(const T *__restrict__ X, T 
 // w[i] = prod_b E_b[c_b, c_b] over the column-major decomposition of i into
 // (dims[0], ..., dims[nb-1], unit); the trailing unit factor covers indices
 // that carry no message (the physical bond).
-struct WeightArgs {
-  int nb;
-  int64_t dims[HBP_MAX_ORDER];
-  int64_t diag_off[HBP_MAX_ORDER];  // offset of the message's (0,0) element
-};
+__device__ __forceinline__ double2 weight_at(const double2 *__restrict__ E, const WeightArgs &a,
+                                             int64_t i) {
+  int64_t rem = i;
+  double re = 1.0, im = 0.0;
+  for (int b = 0; b < a.nb; ++b) {
+    const int64_t c = rem % a.dims[b];
+    rem /= a.dims[b];
+    const double2 e = E[a.diag_off[b] + c * (a.dims[b] + 1)];
+    const double r2 = re * e.x - im * e.y, i2 = re * e.y + im * e.x;
+    re = r2;
+    im = i2;
+  }
+  return make_double2(re, im);
+}
 __global__ void k_build_weights(const double2 *__restrict__ E, double2 *__restrict__ w, int64_t n,
                                 WeightArgs a) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    int64_t rem = i;
-    double re = 1.0, im = 0.0;
-    for (int b = 0; b < a.nb; ++b) {
-      const int64_t c = rem % a.dims[b];
-      rem /= a.dims[b];
-      const double2 e = E[a.diag_off[b] + c * (a.dims[b] + 1)];
-      const double r2 = re * e.x - im * e.y, i2 = re * e.y + im * e.x;
-      re = r2;
-      im = i2;
-    }
-    w[i] = make_double2(re, im);
-  }
+       i += (int64_t)gridDim.x * blockDim.x)
+    w[i] = weight_at(E, a, i);
+}
+// several weight vectors in one launch (blockIdx.y = vector)
+__global__ void k_build_weights_batch(const double2 *__restrict__ E, WeightBatch b) {
+  const int v = blockIdx.y;
+  const int64_t n = b.n[v];
+  double2 *w = b.w[v];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    w[i] = weight_at(E, b.a[v], i);
 }
 
 // M[e] = sum_c p[c n + e].  A block owns 16 entries; its G thread groups
@@ -1171,6 +1178,18 @@ __global__ __launch_bounds__(RED_G * RED_E) void k_reduce_partials(const PT *__r
                                                                     int nchunks, int n,
                                                                     double2 *__restrict__ M) {
   reduce_entries<PT, RED_G>(p, nchunks, n, M, blockIdx.x);
+}
+
+// Several reductions in one launch (blockIdx.y = job), the jobs passed by
+// value: the same reduce_entries as k_reduce_partials, so the sums are
+// bitwise those of separate launches.
+__global__ __launch_bounds__(RED_G * RED_E) void k_reduce_batch(RedBatch b) {
+  const RedBatchJob &j = b.j[blockIdx.y];
+  if ((int64_t)blockIdx.x * RED_E >= j.n) return;
+  if (j.f32)
+    reduce_entries<float2, RED_G>((const float2 *)j.partials, j.nch, j.n, j.M, blockIdx.x);
+  else
+    reduce_entries<double2, RED_G>((const double2 *)j.partials, j.nch, j.n, j.M, blockIdx.x);
 }
 
 __global__ __launch_bounds__(256) void k_reduce_grouped(const RedJob *__restrict__ jobs,
@@ -1951,6 +1970,23 @@ void launch_build_weights(hipStream_t st, const double2 *E, double2 *w, int64_t 
     a.diag_off[b] = diag_off[b];
   }
   hipLaunchKernelGGL(k_build_weights, dim3(grid_for((size_t)n)), dim3(256), 0, st, E, w, n, a);
+  HBP_HIP(hipGetLastError());
+}
+
+void launch_build_weights_batch(hipStream_t st, const double2 *E, const WeightBatch &b) {
+  if (b.count <= 0) return;
+  int64_t nmax = 1;
+  for (int i = 0; i < b.count; ++i) nmax = std::max(nmax, b.n[i]);
+  hipLaunchKernelGGL(k_build_weights_batch, dim3(grid_for((size_t)nmax), b.count), dim3(256), 0,
+                     st, E, b);
+  HBP_HIP(hipGetLastError());
+}
+
+void launch_reduce_batch(hipStream_t st, const RedBatch &b) {
+  if (b.count <= 0) return;
+  int gx = 1;
+  for (int i = 0; i < b.count; ++i) gx = std::max(gx, (b.j[i].n + RED_E - 1) / RED_E);
+  hipLaunchKernelGGL(k_reduce_batch, dim3(gx, b.count), dim3(RED_G * RED_E), 0, st, b);
   HBP_HIP(hipGetLastError());
 }
 

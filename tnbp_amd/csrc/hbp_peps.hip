@@ -680,12 +680,64 @@ static double2 *diag_weights(Ctx *ctx, Peps *p, int s, int j0, int j1, int64_t u
   return w;
 }
 
+// Deferred reductions: message kernels write their split-K partials into the
+// context's arena and the fixed-order reductions run together, up to
+// RED_BATCH per launch (at small bond dimensions a sweep is bound by the
+// number of launches, not by their work).  The arena is capped, so large
+// messages still reduce one or two at a time; flush() before anything reads
+// the messages.  The sums are bitwise those of one reduction per message.
+struct Reducer {
+  static constexpr size_t kCap = (size_t)64 << 20;
+  Ctx *ctx;
+  RedBatch b;
+  size_t used = 0;
+  explicit Reducer(Ctx *c) : ctx(c) { b.count = 0; }
+  // arena space for the partials of `jobs` messages (added next)
+  void *take(size_t bytes, int jobs = 1) {
+    bytes = (bytes + 255) & ~(size_t)255;
+    if (b.count + jobs > RED_BATCH ||
+        (used > 0 && (used + bytes > kCap || used + bytes > ctx->arena_bytes)))
+      flush();
+    // the arena grows only while no partials are pending in it
+    char *a = (char *)ctx->get_arena(used == 0 ? std::max(bytes, kCap) : used + bytes);
+    void *ptr = a + used;
+    used += bytes;
+    return ptr;
+  }
+  void add(const void *parts, bool f32, int nch, int n, double2 *M) {
+    b.j[b.count++] = RedBatchJob{parts, M, nch, n, f32 ? 1 : 0};
+  }
+  void flush() {
+    launch_reduce_batch(ctx->stream, b);
+    b.count = 0;
+    used = 0;
+  }
+};
+
 // M = sum over all bonds but k of (V x_{j != k} diag E_in(j))[.. i ..] conj V[.. j ..]:
 // every incoming message on the other bonds is diagonal, so the mode products
 // are row weights folded into the message kernel.  k == z gives the reduced
 // matrix over the physical index (measure.h:54-84).
+// Weight arguments of the bonds [j0, j1) of site s (empty: nb == 0) and the
+// vector length (times `unit`).
+static WeightArgs weight_args(const Peps *p, int s, int j0, int j1, int64_t unit, int64_t *n_out) {
+  WeightArgs a{};
+  int64_t n = unit;
+  for (int j = j0; j < j1; ++j) {
+    a.dims[a.nb] = p->bond_dim[p->bonds[s][j]];
+    a.diag_off[a.nb] = p->E_off[p->in_msg(s, j)];
+    n *= a.dims[a.nb];
+    ++a.nb;
+  }
+  *n_out = n;
+  return a;
+}
+
+// The message on bond k of site s from given row weights wL (bonds before k)
+// and wR (bonds after k, times the physical index); nullptr = all ones.
 template <typename S>
-static void weighted_message(Ctx *ctx, Peps *p, int s, int k, double2 *M) {
+static void weighted_message_w(Ctx *ctx, Peps *p, int s, int k, double2 *M, const double2 *wL,
+                               const double2 *wR, Reducer *red) {
   const int z = p->z(s);
   int32_t sh[HBP_MAX_ORDER];
   p->shape(s, sh);
@@ -693,20 +745,78 @@ static void weighted_message(Ctx *ctx, Peps *p, int s, int k, double2 *M) {
   for (int d = 0; d < k; ++d) L *= sh[d];
   for (int d = k + 1; d <= z; ++d) R *= sh[d];
   const int D = sh[k];
-  int64_t nl = 0, nr = 0;
-  double2 *wL = diag_weights(ctx, p, s, 0, k, 1, &nl);
-  double2 *wR = k < z ? diag_weights(ctx, p, s, k + 1, z, p->phys[s], &nr) : nullptr;
   const int64_t nch = message_num_chunks(L * R);
-  double2 *parts = (double2 *)ctx->get_scratch((size_t)nch * D * D * 16 + 64);
+  const size_t pb = (size_t)nch * D * D * 16 + 64;
+  double2 *parts = (double2 *)(red ? red->take(pb) : ctx->get_scratch(pb));
   const int pr = ctx->prof_begin(HBP_PROF_MESSAGE, 8.0 * L * R * D * D,
                                  2.0 * sizeof(S) * 2.0 * L * R * D);
   const void *Vs = site_data(ctx, p, s);
   const int n = message_partials<S>(ctx->stream, Vs, Vs, parts, L, D, D, R, L * D,
                                     L * D, nch, 1, wL, wR);
-  launch_reduce_partials(ctx->stream, parts, n, D * D, M);
+  if (red)
+    red->add(parts, false, n, D * D, M);
+  else
+    launch_reduce_partials(ctx->stream, parts, n, D * D, M);
   ctx->prof_end(pr);
+}
+
+template <typename S>
+static void weighted_message(Ctx *ctx, Peps *p, int s, int k, double2 *M,
+                             Reducer *red = nullptr) {
+  const int z = p->z(s);
+  int64_t nl = 0, nr = 0;
+  double2 *wL = diag_weights(ctx, p, s, 0, k, 1, &nl);
+  double2 *wR = k < z ? diag_weights(ctx, p, s, k + 1, z, p->phys[s], &nr) : nullptr;
+  weighted_message_w<S>(ctx, p, s, k, M, wL, wR, red);
   if (wL) ctx->release(wL, (size_t)nl * 16);
   if (wR) ctx->release(wR, (size_t)nr * 16);
+}
+
+// The separate weighted messages of site s on the bonds ks (diagonal
+// environment), with all their row weights built in one launch.
+template <typename S>
+static void weighted_messages_site(Ctx *ctx, Peps *p, int s, const std::vector<int> &ks,
+                                   double2 *raw, const std::vector<int64_t> &raw_off,
+                                   Reducer *red) {
+  const int z = p->z(s);
+  std::vector<WeightArgs> args;
+  std::vector<int64_t> len, off;
+  std::vector<int> slot(2 * ks.size(), -1);
+  int64_t total = 0;
+  for (size_t i = 0; i < ks.size(); ++i) {
+    const int k = ks[i];
+    for (int side = 0; side < 2; ++side) {
+      int64_t n = 0;
+      const WeightArgs a = side == 0 ? weight_args(p, s, 0, k, 1, &n)
+                                     : weight_args(p, s, k + 1, z, p->phys[s], &n);
+      if (a.nb == 0) continue;
+      slot[2 * i + side] = (int)args.size();
+      args.push_back(a);
+      len.push_back(n);
+      off.push_back(total);
+      total += n;
+    }
+  }
+  double2 *wbuf = total ? (double2 *)ctx->alloc((size_t)total * 16) : nullptr;
+  WeightBatch wb;
+  wb.count = 0;
+  for (size_t v = 0; v < args.size(); ++v) {
+    wb.a[wb.count] = args[v];
+    wb.n[wb.count] = len[v];
+    wb.w[wb.count] = wbuf + off[v];
+    if (++wb.count == WEIGHT_BATCH) {
+      launch_build_weights_batch(ctx->stream, p->E, wb);
+      wb.count = 0;
+    }
+  }
+  launch_build_weights_batch(ctx->stream, p->E, wb);
+  for (size_t i = 0; i < ks.size(); ++i) {
+    const int k = ks[i];
+    const double2 *wL = slot[2 * i] >= 0 ? wbuf + off[slot[2 * i]] : nullptr;
+    const double2 *wR = slot[2 * i + 1] >= 0 ? wbuf + off[slot[2 * i + 1]] : nullptr;
+    weighted_message_w<S>(ctx, p, s, k, raw + raw_off[p->out_msg(s, k)], wL, wR, red);
+  }
+  if (wbuf) ctx->release(wbuf, (size_t)total * 16);
 }
 
 // every message entering site s except on bond `skip` is diagonal
@@ -733,17 +843,21 @@ static void mode_prod(Ctx *ctx, const void *T, const double2 *E, void *Out, cons
 // raw M[i, j] = sum T[.. i ..] conj V[.. j ..] over all axes but k (both same shape)
 template <typename S>
 static void message_raw(Ctx *ctx, const void *T, const void *V, const int32_t *sh, int order,
-                        int k, double2 *M) {
+                        int k, double2 *M, Reducer *red = nullptr) {
   int64_t L = 1, R = 1;
   for (int d = 0; d < k; ++d) L *= sh[d];
   for (int d = k + 1; d < order; ++d) R *= sh[d];
   const int D = sh[k];
   const int64_t nch = message_num_chunks(L * R);
-  double2 *parts = (double2 *)ctx->get_scratch((size_t)nch * D * D * 16 + 64);
+  const size_t pb = (size_t)nch * D * D * 16 + 64;
+  double2 *parts = (double2 *)(red ? red->take(pb) : ctx->get_scratch(pb));
   const int pr = ctx->prof_begin(HBP_PROF_MESSAGE, 8.0 * L * R * D * D,
                                  2.0 * sizeof(S) * 2.0 * L * R * D);
   const int n = message_partials<S>(ctx->stream, T, V, parts, L, D, D, R, L * D, L * D, nch, 1);
-  launch_reduce_partials(ctx->stream, parts, n, D * D, M);
+  if (red)
+    red->add(parts, false, n, D * D, M);
+  else
+    launch_reduce_partials(ctx->stream, parts, n, D * D, M);
   ctx->prof_end(pr);
 }
 
@@ -757,16 +871,21 @@ static bool merge_ok(const Peps *p, const int32_t *sh, int k, int j) {
 // P, V viewed as (L, i, a, R) (k_first) or (L, a, i, R)
 template <typename S>
 static void product_message(Ctx *ctx, const void *P, const double2 *E, const void *V, int64_t L,
-                            int64_t R, int Dk, int Dj, int k_first, double2 *M) {
+                            int64_t R, int Dk, int Dj, int k_first, double2 *M,
+                            Reducer *red = nullptr) {
   constexpr int kMaxBlocks = 512;
   using C = std::conditional_t<sizeof(S) == 4, float2, double2>;
-  C *parts = (C *)ctx->get_scratch((size_t)kMaxBlocks * Dk * Dk * sizeof(C) + 64);
+  const size_t pb = (size_t)kMaxBlocks * Dk * Dk * sizeof(C) + 64;
+  C *parts = (C *)(red ? red->take(pb) : ctx->get_scratch(pb));
   const double n = (double)L * R * Dk * Dj;
   const int pr = ctx->prof_begin(HBP_PROF_PRODUCT_MESSAGE, 8.0 * n * (Dk + Dj),
                                  2.0 * sizeof(S) * 2.0 * n + 16.0 * Dj * Dj);
   const int nb = product_message_partials<S>(ctx->stream, P, V, E, parts, L, R, Dk, Dj, k_first,
                                              kMaxBlocks);
-  launch_reduce_partials(ctx->stream, parts, nb, Dk * Dk, M);
+  if (red)
+    red->add(parts, sizeof(C) == 8, nb, Dk * Dk, M);
+  else
+    launch_reduce_partials(ctx->stream, parts, nb, Dk * Dk, M);
   ctx->prof_end(pr);
 }
 
@@ -777,11 +896,11 @@ static void product_message(Ctx *ctx, const void *P, const double2 *E, const voi
 template <typename S>
 static void merged_pair(Ctx *ctx, Peps *p, int s, const void *P, const void *V,
                         const int32_t *sh, int t, int64_t R, double2 *raw,
-                        const std::vector<int64_t> &raw_off) {
+                        const std::vector<int64_t> &raw_off, Reducer *red) {
   for (int k = t; k <= t + 1; ++k) {
     const int j = k == t ? t + 1 : t;
     product_message<S>(ctx, P, p->E + p->E_off[p->in_msg(s, j)], V, 1, R, sh[k], sh[j],
-                       k == t ? 1 : 0, raw + raw_off[p->out_msg(s, k)]);
+                       k == t ? 1 : 0, raw + raw_off[p->out_msg(s, k)], red);
   }
 }
 
@@ -865,7 +984,7 @@ static const void *pair_front(Ctx *ctx, Peps *p, int s, const void *Vs, const in
 // that the remaining bonds form one block.
 template <typename S>
 static void pair_messages_diag(Ctx *ctx, Peps *p, int s, const int32_t *sh, int order, int t,
-                               double2 *Mx, double2 *My) {
+                               double2 *Mx, double2 *My, Reducer *red) {
   const int z = p->z(s);
   const void *X = pair_front<S>(ctx, p, s, site_data(ctx, p, s), sh, order, t);
   int64_t nw = 0;
@@ -875,7 +994,8 @@ static void pair_messages_diag(Ctx *ctx, Peps *p, int s, const int32_t *sh, int 
   const int64_t nO = (int64_t)(p->site_size(s) / ((size_t)Dx * Dy));
   constexpr int kMaxBlocks = 512;
   using C = std::conditional_t<sizeof(S) == 4, float2, double2>;
-  C *px = (C *)ctx->get_scratch((size_t)kMaxBlocks * (Dx * Dx + Dy * Dy) * sizeof(C) + 64);
+  const size_t pb = (size_t)kMaxBlocks * (Dx * Dx + Dy * Dy) * sizeof(C) + 64;
+  C *px = (C *)(red ? red->take(pb, 2) : ctx->get_scratch(pb));
   C *py = px + (size_t)kMaxBlocks * Dx * Dx;
   const double n = (double)nO * Dx * Dy;
   const int pr = ctx->prof_begin(HBP_PROF_MESSAGE, 8.0 * n * (Dx + Dy), sizeof(S) * 2.0 * n);
@@ -883,8 +1003,13 @@ static void pair_messages_diag(Ctx *ctx, Peps *p, int s, const int32_t *sh, int 
                                        p->E + p->E_off[p->in_msg(s, t)],
                                        p->E + p->E_off[p->in_msg(s, t + 1)], wo, px, py,
                                        kMaxBlocks);
-  launch_reduce_partials(ctx->stream, px, nb, Dx * Dx, Mx);
-  launch_reduce_partials(ctx->stream, py, nb, Dy * Dy, My);
+  if (red) {
+    red->add(px, sizeof(C) == 8, nb, Dx * Dx, Mx);
+    red->add(py, sizeof(C) == 8, nb, Dy * Dy, My);
+  } else {
+    launch_reduce_partials(ctx->stream, px, nb, Dx * Dx, Mx);
+    launch_reduce_partials(ctx->stream, py, nb, Dy * Dy, My);
+  }
   ctx->prof_end(pr);
   if (wo) ctx->release(wo, (size_t)nw * 16);
 }
@@ -894,6 +1019,7 @@ static void sweep_raw(Ctx *ctx, Peps *p, double2 *raw, const std::vector<int64_t
   size_t maxb = 16;
   for (int s = 0; s < p->n_sites; ++s) maxb = std::max(maxb, p->site_size(s) * p->elem_bytes());
   ensure_work(ctx, p, maxb);
+  Reducer red(ctx);
   int32_t sh[HBP_MAX_ORDER];
   for (int s = 0; s < p->n_sites; ++s) {
     const int z = p->z(s), order = z + 1;
@@ -907,18 +1033,21 @@ static void sweep_raw(Ctx *ctx, Peps *p, double2 *raw, const std::vector<int64_t
         for (int t : pairs) {
           if (t < 0 || !(t == 0 || t + 2 == z) || !merge_ok(p, sh, t, t + 1)) continue;
           pair_messages_diag<S>(ctx, p, s, sh, order, t, raw + raw_off[p->out_msg(s, t)],
-                                raw + raw_off[p->out_msg(s, t + 1)]);
+                                raw + raw_off[p->out_msg(s, t + 1)], &red);
           done[t] = done[t + 1] = true;
         }
       }
+      std::vector<int> ks;
       for (int k = 0; k < z; ++k)
-        if (!done[k]) weighted_message<S>(ctx, p, s, k, raw + raw_off[p->out_msg(s, k)]);
+        if (!done[k]) ks.push_back(k);
+      weighted_messages_site<S>(ctx, p, s, ks, raw, raw_off, &red);
       continue;
     }
     if (z <= 2) {
       for (int k = 0; k < z; ++k) {
         const void *T = absorb_env<S>(ctx, p, s, k, sh);
-        message_raw<S>(ctx, T, site_data(ctx, p, s), sh, order, k, raw + raw_off[p->out_msg(s, k)]);
+        message_raw<S>(ctx, T, site_data(ctx, p, s), sh, order, k, raw + raw_off[p->out_msg(s, k)],
+                       &red);
       }
       continue;
     }
@@ -956,7 +1085,7 @@ static void sweep_raw(Ctx *ctx, Peps *p, double2 *raw, const std::vector<int64_t
           cur = dst;
           w ^= 1;
         }
-        merged_pair<S>(ctx, p, s, cur, Vs, sh, t0, Lb * R, raw, raw_off);
+        merged_pair<S>(ctx, p, s, cur, Vs, sh, t0, Lb * R, raw, raw_off, &red);
         continue;
       }
       for (int j = a0; j < a1; ++j) {
@@ -976,10 +1105,11 @@ static void sweep_raw(Ctx *ctx, Peps *p, double2 *raw, const std::vector<int64_t
           T = dst;
           u = u == 1 ? 2 : 1;
         }
-        message_raw<S>(ctx, T, Vs, sh, order, k, raw + raw_off[p->out_msg(s, k)]);
+        message_raw<S>(ctx, T, Vs, sh, order, k, raw + raw_off[p->out_msg(s, k)], &red);
       }
     }
   }
+  red.flush();
 }
 
 // ---------------------------------------------------------- grouped sweep

@@ -129,6 +129,15 @@ void *Ctx::get_scratch(size_t bytes) {
   return scratch;
 }
 
+void *Ctx::get_arena(size_t bytes) {
+  if (bytes > arena_bytes) {
+    if (arena) release(arena, arena_bytes);
+    arena_bytes = (std::max(bytes, arena_bytes * 2) + 4095) & ~(size_t)4095;
+    arena = alloc(arena_bytes);
+  }
+  return arena;
+}
+
 void Ctx::sync() { HBP_HIP(hipStreamSynchronize(stream)); }
 
 size_t Ctx::free_bytes(size_t *total) {
@@ -593,6 +602,7 @@ hbp_status hbp_ctx_destroy(hbp_ctx *ctx) {
   if (!ctx) return HBP_OK;
   HBP_HIP(hipSetDevice(ctx->device));
   if (ctx->scratch) ctx->release(ctx->scratch, ctx->scratch_bytes);
+  if (ctx->arena) ctx->release(ctx->arena, ctx->arena_bytes);
   ctx->flush_blocks();
   HBP_HIP(hipStreamSynchronize(ctx->stream));
   ctx->prof_clear();
@@ -642,6 +652,11 @@ hbp_status hbp_ctx_trim(hbp_ctx *ctx) {
     ctx->release(ctx->scratch, ctx->scratch_bytes);
     ctx->scratch = nullptr;
     ctx->scratch_bytes = 0;
+  }
+  if (ctx->arena) {
+    ctx->release(ctx->arena, ctx->arena_bytes);
+    ctx->arena = nullptr;
+    ctx->arena_bytes = 0;
   }
   ctx->flush_blocks();
   ctx->sync();
