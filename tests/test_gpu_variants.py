@@ -52,9 +52,10 @@ print(json.dumps(out))
 """
 
 
-def _run(variant):
+def _run(variant, **extra):
     env = dict(os.environ)
     env["HBP_FORCE_VARIANT"] = str(variant)
+    env.update(extra)
     r = subprocess.run([sys.executable, "-c", SCRIPT, ROOT], capture_output=True, text=True,
                        env=env, timeout=240)
     assert r.returncode == 0, r.stderr[-3000:]
@@ -69,3 +70,12 @@ def test_forced_variants_bitwise_equal():
     for v in range(1, 7):
         got = _run(v)
         assert got == ref, (v, got, ref)
+
+
+def test_batched_reductions_bitwise_equal():
+    """A sweep defers its split-K reductions and runs up to eight per launch;
+    each keeps its fixed summation order, so the messages equal those of one
+    reduction per message bit for bit (same lattices: bonds 1 .. 64, degrees
+    2 .. 4, diagonal and dense environments, merged and separate messages)."""
+    announce("batched vs separate split-K reductions, bitwise comparison")
+    assert _run(0, HBP_REDUCE_BATCH="0") == _run(0)

@@ -1019,7 +1019,14 @@ static void sweep_raw(Ctx *ctx, Peps *p, double2 *raw, const std::vector<int64_t
   size_t maxb = 16;
   for (int s = 0; s < p->n_sites; ++s) maxb = std::max(maxb, p->site_size(s) * p->elem_bytes());
   ensure_work(ctx, p, maxb);
-  Reducer red(ctx);
+  // HBP_REDUCE_BATCH=0 in the environment reduces each message on its own
+  // (for tests that compare the two bitwise)
+  static const bool batch = [] {
+    const char *e = getenv("HBP_REDUCE_BATCH");
+    return !(e && e[0] == '0');
+  }();
+  Reducer reducer(ctx);
+  Reducer *red = batch ? &reducer : nullptr;
   int32_t sh[HBP_MAX_ORDER];
   for (int s = 0; s < p->n_sites; ++s) {
     const int z = p->z(s), order = z + 1;
@@ -1033,21 +1040,21 @@ static void sweep_raw(Ctx *ctx, Peps *p, double2 *raw, const std::vector<int64_t
         for (int t : pairs) {
           if (t < 0 || !(t == 0 || t + 2 == z) || !merge_ok(p, sh, t, t + 1)) continue;
           pair_messages_diag<S>(ctx, p, s, sh, order, t, raw + raw_off[p->out_msg(s, t)],
-                                raw + raw_off[p->out_msg(s, t + 1)], &red);
+                                raw + raw_off[p->out_msg(s, t + 1)], red);
           done[t] = done[t + 1] = true;
         }
       }
       std::vector<int> ks;
       for (int k = 0; k < z; ++k)
         if (!done[k]) ks.push_back(k);
-      weighted_messages_site<S>(ctx, p, s, ks, raw, raw_off, &red);
+      weighted_messages_site<S>(ctx, p, s, ks, raw, raw_off, red);
       continue;
     }
     if (z <= 2) {
       for (int k = 0; k < z; ++k) {
         const void *T = absorb_env<S>(ctx, p, s, k, sh);
         message_raw<S>(ctx, T, site_data(ctx, p, s), sh, order, k, raw + raw_off[p->out_msg(s, k)],
-                       &red);
+                       red);
       }
       continue;
     }
@@ -1085,7 +1092,7 @@ static void sweep_raw(Ctx *ctx, Peps *p, double2 *raw, const std::vector<int64_t
           cur = dst;
           w ^= 1;
         }
-        merged_pair<S>(ctx, p, s, cur, Vs, sh, t0, Lb * R, raw, raw_off, &red);
+        merged_pair<S>(ctx, p, s, cur, Vs, sh, t0, Lb * R, raw, raw_off, red);
         continue;
       }
       for (int j = a0; j < a1; ++j) {
@@ -1105,11 +1112,11 @@ static void sweep_raw(Ctx *ctx, Peps *p, double2 *raw, const std::vector<int64_t
           T = dst;
           u = u == 1 ? 2 : 1;
         }
-        message_raw<S>(ctx, T, Vs, sh, order, k, raw + raw_off[p->out_msg(s, k)], &red);
+        message_raw<S>(ctx, T, Vs, sh, order, k, raw + raw_off[p->out_msg(s, k)], red);
       }
     }
   }
-  red.flush();
+  if (red) red->flush();
 }
 
 // ---------------------------------------------------------- grouped sweep
