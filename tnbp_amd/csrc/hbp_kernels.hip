@@ -889,6 +889,10 @@ __global__ __launch_bounds__(256, OccPM<S>::v) void k_product_message(
       t_re[c] = v4{0, 0, 0, 0};
       t_im[c] = v4{0, 0, 0, 0};
     }
+    // the two products at raised wave priority, so that the MFMA phase of one
+    // workgroup is not held back by the other's loads and LDS staging on the
+    // same SIMD (measured 0.5-1.5 % less time for this kernel)
+    __builtin_amdgcn_s_setprio(1);
     // T_o^T = E^T P_o^T:  T = P E = (Pr Er - Pi Ei) + i (Pr Ei + Pi Er)
 #pragma unroll
     for (int s = 0; s < 16; ++s) {
@@ -922,6 +926,7 @@ __global__ __launch_bounds__(256, OccPM<S>::v) void k_product_message(
         }
       }
     }
+    __builtin_amdgcn_s_setprio(0);
   }
 
   // register v of lane (li, lk) of m[c2]: M[i][i'] with i' = 16 c2 + row(lk, v)
