@@ -1570,11 +1570,18 @@ static void mp_launch(hipStream_t st, const void *T, const double2 *E, void *Out
   const int64_t rows_per_tile = 4 * 16 * RB;
   const int64_t ntiles = (Q + rows_per_tile - 1) / rows_per_tile;
   // persistent tiles: E is staged once per workgroup and reused across tiles
-  // the buffer form runs one round of resident workgroups (OCC per CU), each
-  // walking its tiles with the next tile's loads in flight: 4.6 % less time
-  // than 2048 workgroups (2.7 rounds, the last one partly empty)
-  const int64_t cap = PFN == 18 ? (int64_t)OCC * device_cus() : 2048;
-  const int64_t nb = std::min<int64_t>(ntiles, cap);
+  // one round of resident workgroups (as many per CU as fit), each walking its
+  // tiles: against 2048 workgroups (2.7 or 4 rounds, the last one partly
+  // empty) 5 % less time for the FP32 buffer form and 3 % for FP64 at D = 64
+  static std::atomic<int> fit{0};
+  int per_cu = fit.load(std::memory_order_relaxed);
+  if (per_cu == 0) {
+    HBP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, reinterpret_cast<const void *>(&k_mode_product<S, NCB, RB, PFN, OCC>), 256, 0));
+    if (per_cu <= 0) per_cu = 1;
+    fit.store(per_cu, std::memory_order_relaxed);
+  }
+  const int64_t nb = std::min<int64_t>(ntiles, (int64_t)per_cu * device_cus());
   using C = typename Cplx<S>::type;
   timed_launch(k_mode_product<S, NCB, RB, PFN, OCC>, dim3((unsigned)nb), dim3(256), st,
                (const C *)T, E, (C *)Out, L, Din, Dout, Q, e_trans, (S)scale, parts);
