@@ -147,3 +147,43 @@ def test_trunc_svd_rule(ctx):
     assert abs(err - (1e-6 + 1e-18) / np.sum(s_true ** 2)) < 1e-12
     u, s, vd, err = T.trunc_svd(ctx, T.Tensor.from_numpy(ctx, m), 1, 10, 1e-8)
     assert s.to_numpy().size == 4
+
+
+def test_context_block_cache_accounting():
+    """Freed blocks are kept by size class and handed out again: hbp_ctx_memory
+    counts live blocks only (at their class size: powers of two from 256 B
+    below 32 MiB, 2 MiB steps above), the same sizes come back without
+    growing the live count, the data of a reused block is the new tensor's,
+    and trim returns everything while keeping live tensors intact."""
+    from tnbp_amd import tci as T
+    announce("context block cache accounting")
+    c = T.Context(0)
+    try:
+        base, _ = c.memory()
+        rng = np.random.default_rng(5)
+        sizes = [(3,), (100,), (1000, 3), (300, 1000), (2200, 1000)]  # up to 35 MB complex128
+        expect = 0
+        for shp in sizes:
+            n = int(np.prod(shp)) * 16
+            expect += 256 if n <= 256 else (1 << (n - 1).bit_length()) if n < (32 << 20) \
+                else -(-n // (2 << 20)) * (2 << 20)
+        for rnd in range(3):
+            ts = [T.Tensor.from_numpy(c, _rand(rng, s)) for s in sizes]
+            live, _ = c.memory()
+            assert live - base == expect, (rnd, live - base, expect)
+            # a reused block holds exactly what was uploaded into it
+            for s in sizes:
+                a = _rand(rng, s)
+                t2 = T.Tensor.from_numpy(c, a)
+                np.testing.assert_array_equal(t2.to_numpy(), a)
+                del t2
+            del ts
+            c.synchronize()
+            live, _ = c.memory()
+            assert live == base
+        keep = T.Tensor.from_numpy(c, np.arange(12.0).reshape(3, 4) + 0j)
+        c.trim()
+        np.testing.assert_array_equal(keep.to_numpy(), np.arange(12.0).reshape(3, 4) + 0j)
+        del keep
+    finally:
+        c.close()
