@@ -206,6 +206,12 @@ void launch_swap_blocks(hipStream_t st, const void *X, void *Y, int64_t L, int64
                         int64_t R, int elem_bytes);
 // M = sum over chunks (fixed order), complex<double>; complex<float>
 // partials are summed in double.
+// rho[p, p'] = sum_l w[l] T[l + L p] conj V[l + L p'] over the rows l < L for
+// a physical index of dimension 2 (w may be null): FP64 partials of 4 entries
+// per block into `partials`; returns the block count (<= max_blocks)
+template <typename S>
+int rho_partials(hipStream_t st, const void *T, const void *V, const double2 *w, int64_t L,
+                 double2 *partials, int max_blocks);
 // up to RED_BATCH fixed-order reductions of partials (FP64, or FP32 stored as
 // float2) into FP64 messages, in one launch
 constexpr int RED_BATCH = 8;

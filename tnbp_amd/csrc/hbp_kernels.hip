@@ -759,6 +759,60 @@ __global__ __launch_bounds__(256, 2) void k_message_grouped(const MsgGJob *__res
                         j.L * j.D2, 1, nullptr, nullptr, max(j.D1, j.D2), 0, tre, tim, vre, vim);
 }
 
+// ------------------------------------------------ physical-index message
+// rho[p, p'] = sum_l w[l] T[l + L p] conj V[l + L p'] for the physical index
+// (d = 2): the reduced matrix of a site measurement (measure.h:54-84), whose
+// contraction runs over all bonds.  The message kernel's 64 x 16 MFMA tiles
+// would be almost empty for a 2 x 2 result and its staged chunks
+// latency-bound (176 us for an interior site at D = 32); this streams the rows
+// instead, FP64 products and sums, each thread a fixed row set and the block
+// sums in a fixed order, so the partials are reproducible.
+template <typename S>
+__global__ __launch_bounds__(256) void k_rho_partials(const typename Cplx<S>::type *__restrict__ T,
+                                                      const typename Cplx<S>::type *__restrict__ V,
+                                                      const double2 *__restrict__ w, int64_t L,
+                                                      double2 *__restrict__ partials) {
+  __shared__ double sh[8];
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // (re, im) of rho[p + 2 p']
+  for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < L;
+       l += (int64_t)gridDim.x * blockDim.x) {
+    double tr[2], ti[2], vr[2], vi[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const auto t = T[l + L * q];
+      const auto v = V[l + L * q];
+      tr[q] = (double)t.x;
+      ti[q] = (double)t.y;
+      vr[q] = (double)v.x;
+      vi[q] = (double)v.y;
+    }
+    if (w) {
+      const double2 c = w[l];
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const double a = c.x * tr[q] - c.y * ti[q], b = c.x * ti[q] + c.y * tr[q];
+        tr[q] = a;
+        ti[q] = b;
+      }
+    }
+#pragma unroll
+    for (int pp = 0; pp < 2; ++pp)
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        // (tr + i ti)(vr - i vi)
+        acc[2 * (q + 2 * pp)] += tr[q] * vr[pp] + ti[q] * vi[pp];
+        acc[2 * (q + 2 * pp) + 1] += ti[q] * vr[pp] - tr[q] * vi[pp];
+      }
+  }
+  double2 *out = partials + (int64_t)blockIdx.x * 4;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const double re = block_sum(acc[2 * e], sh);
+    const double im = block_sum(acc[2 * e + 1], sh);
+    if (threadIdx.x == 0) out[e] = make_double2(re, im);
+  }
+}
+
 // ------------------------------------------------ product + message, merged
 // For two adjacent bonds k (index i) and j (index a) of column-major site
 // tensors P and V of the same shape, with o = (l, r) the remaining index,
@@ -1916,6 +1970,21 @@ template int message_partials<double>(hipStream_t, const void *, const void *, d
 template int message_partials<float>(hipStream_t, const void *, const void *, double2 *,
                                      int64_t, int, int, int64_t, int64_t, int64_t, int64_t,
                                      int, const double2 *, const double2 *, int, int64_t);
+
+template <typename S>
+int rho_partials(hipStream_t st, const void *T, const void *V, const double2 *w, int64_t L,
+                 double2 *partials, int max_blocks) {
+  using C = typename Cplx<S>::type;
+  const int64_t nb = std::max<int64_t>(1, std::min<int64_t>(max_blocks, (L + 255) / 256));
+  hipLaunchKernelGGL(k_rho_partials<S>, dim3((unsigned)nb), dim3(256), 0, st, (const C *)T,
+                     (const C *)V, w, L, partials);
+  HBP_HIP(hipGetLastError());
+  return (int)nb;
+}
+template int rho_partials<float>(hipStream_t, const void *, const void *, const double2 *, int64_t,
+                                 double2 *, int);
+template int rho_partials<double>(hipStream_t, const void *, const void *, const double2 *,
+                                  int64_t, double2 *, int);
 
 template <typename S>
 int product_message_partials(hipStream_t st, const void *P, const void *V, const double2 *E,

@@ -718,6 +718,21 @@ struct Reducer {
 // every incoming message on the other bonds is diagonal, so the mode products
 // are row weights folded into the message kernel.  k == z gives the reduced
 // matrix over the physical index (measure.h:54-84).
+// rho[p, p'] = sum_l w[l] T[l + L p] conj V[l + L p'] for a physical index of
+// dimension 2 (the last axis of T and V, every bond before it contracted)
+template <typename S>
+static void rho_message(Ctx *ctx, const void *T, const void *V, const double2 *w, int64_t L,
+                        double2 *M, Reducer *red) {
+  constexpr int kMaxBlocks = 1024;
+  const size_t pb = (size_t)kMaxBlocks * 4 * 16 + 64;
+  double2 *parts = (double2 *)(red ? red->take(pb) : ctx->get_scratch(pb));
+  const int n = rho_partials<S>(ctx->stream, T, V, w, L, parts, kMaxBlocks);
+  if (red)
+    red->add(parts, false, n, 4, M);
+  else
+    launch_reduce_partials(ctx->stream, parts, n, 4, M);
+}
+
 // Weight arguments of the bonds [j0, j1) of site s (empty: nb == 0) and the
 // vector length (times `unit`).
 static WeightArgs weight_args(const Peps *p, int s, int j0, int j1, int64_t unit, int64_t *n_out) {
@@ -745,12 +760,16 @@ static void weighted_message_w(Ctx *ctx, Peps *p, int s, int k, double2 *M, cons
   for (int d = 0; d < k; ++d) L *= sh[d];
   for (int d = k + 1; d <= z; ++d) R *= sh[d];
   const int D = sh[k];
+  const void *Vs = site_data(ctx, p, s);
+  if (k == z && D == 2 && !wR) {  // the reduced matrix over the physical index
+    rho_message<S>(ctx, Vs, Vs, wL, L, M, red);
+    return;
+  }
   const int64_t nch = message_num_chunks(L * R);
   const size_t pb = (size_t)nch * D * D * 16 + 64;
   double2 *parts = (double2 *)(red ? red->take(pb) : ctx->get_scratch(pb));
   const int pr = ctx->prof_begin(HBP_PROF_MESSAGE, 8.0 * L * R * D * D,
                                  2.0 * sizeof(S) * 2.0 * L * R * D);
-  const void *Vs = site_data(ctx, p, s);
   const int n = message_partials<S>(ctx->stream, Vs, Vs, parts, L, D, D, R, L * D,
                                     L * D, nch, 1, wL, wR);
   if (red)
@@ -848,6 +867,10 @@ static void message_raw(Ctx *ctx, const void *T, const void *V, const int32_t *s
   for (int d = 0; d < k; ++d) L *= sh[d];
   for (int d = k + 1; d < order; ++d) R *= sh[d];
   const int D = sh[k];
+  if (R == 1 && D == 2) {  // the reduced matrix over a last (physical) index
+    rho_message<S>(ctx, T, V, nullptr, L, M, red);
+    return;
+  }
   const int64_t nch = message_num_chunks(L * R);
   const size_t pb = (size_t)nch * D * D * 16 + 64;
   double2 *parts = (double2 *)(red ? red->take(pb) : ctx->get_scratch(pb));
