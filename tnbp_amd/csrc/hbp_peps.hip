@@ -2192,6 +2192,23 @@ hbp_status hbp_peps_absorb_tpo(hbp_ctx *ctx, hbp_peps *p, const double *const *o
       lazy = total > ctx->free_bytes() / 2;
     }
   }
+  // eager: every site's operator in one upload (a pageable copy per site
+  // would wait for the previous site's absorption each time)
+  std::vector<size_t> op_off(ns + 1, 0);
+  for (int s = 0; s < ns; ++s) {
+    const int32_t *osh = tpo_shapes + (size_t)s * HBP_MAX_ORDER;
+    size_t on = 1;
+    for (int k = 0; k <= p->z(s) + 1; ++k) on *= (size_t)osh[k];
+    op_off[s + 1] = op_off[s] + on;
+  }
+  double2 *dAll = nullptr;
+  if (!lazy) {
+    std::vector<double2> packed(std::max<size_t>(op_off[ns], 1));
+    for (int s = 0; s < ns; ++s)
+      std::memcpy(packed.data() + op_off[s], ops[s], (op_off[s + 1] - op_off[s]) * 16);
+    dAll = (double2 *)ctx->alloc(packed.size() * 16);
+    ctx->upload(dAll, packed.data(), packed.size() * 16);
+  }
   for (int s = 0; s < ns; ++s) {
     const int32_t *osh = tpo_shapes + (size_t)s * HBP_MAX_ORDER;
     const int z = p->z(s);
@@ -2219,10 +2236,10 @@ hbp_status hbp_peps_absorb_tpo(hbp_ctx *ctx, hbp_peps *p, const double *const *o
     a.ndim[z] = a.dout;
     for (int k = 0; k <= z + 1; ++k) on *= (size_t)osh[k];
     const size_t n = (size_t)ns_ * a.dout;
-    double2 *dO = (double2 *)ctx->alloc(on * 16);
-    HBP_HIP(hipMemcpyAsync(dO, ops[s], on * 16, hipMemcpyHostToDevice, ctx->stream));
     p->phys[s] = a.dout;
     if (lazy) {
+      double2 *dO = (double2 *)ctx->alloc(on * 16);
+      HBP_HIP(hipMemcpyAsync(dO, ops[s], on * 16, hipMemcpyHostToDevice, ctx->stream));
       p->pending[s] = 1;
       p->pend_O[s] = dO;
       p->pend_O_bytes[s] = on * 16;
@@ -2232,6 +2249,7 @@ hbp_status hbp_peps_absorb_tpo(hbp_ctx *ctx, hbp_peps *p, const double *const *o
     }
     const size_t bytes = n * p->elem_bytes();
     void *nv = ctx->alloc(bytes);
+    const double2 *dO = dAll + op_off[s];
     if (p->precision == HBP_PREC_FP64)
       hipLaunchKernelGGL(k_absorb_site<double2>, dim3(gridn(n)), dim3(256), 0, ctx->stream,
                          (const double2 *)p->V[s], dO, (double2 *)nv, n, a);
@@ -2239,11 +2257,11 @@ hbp_status hbp_peps_absorb_tpo(hbp_ctx *ctx, hbp_peps *p, const double *const *o
       hipLaunchKernelGGL(k_absorb_site<float2>, dim3(gridn(n)), dim3(256), 0, ctx->stream,
                          (const float2 *)p->V[s], dO, (float2 *)nv, n, a);
     HBP_HIP(hipGetLastError());
-    ctx->release(dO, on * 16);
     ctx->release(p->V[s], p->V_bytes[s]);
     p->V[s] = nv;
     p->V_bytes[s] = bytes;
   }
+  if (dAll) ctx->release(dAll, std::max<size_t>(op_off[ns], 1) * 16);
   std::vector<int> old = p->bond_dim;
   std::vector<int64_t> old_off = p->E_off;
   for (int m = 0; m < ne; ++m) p->bond_dim[m] = old[m] * t[m];
