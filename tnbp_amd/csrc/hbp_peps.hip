@@ -148,7 +148,27 @@ struct AbsorbArgs {
   int64_t nstride[HBP_MAX_ORDER];  // strides of V' (fused bonds, d_out)
   int ndim[HBP_MAX_ORDER];         // D_k t_k, then d_out
   int din, dout;
+  // n < 2^32: the index is split with 32-bit division by invariant integers
+  // (multiply-high and shift, magic m and shift s per divisor); 64-bit
+  // division had made this HBM-bound kernel instruction-bound (~4x slower)
+  int fast;
+  uint32_t nm[HBP_MAX_ORDER], ns[HBP_MAX_ORDER];  // for ndim
+  uint32_t vm[HBP_MAX_ORDER], vs[HBP_MAX_ORDER];  // for vdim
 };
+
+// Division of any 32-bit n by an invariant d >= 1 (Granlund and Montgomery,
+// "Division by invariant integers using multiplication", 1994):
+// s = ceil(log2 d), m = floor(2^32 (2^s - d) / d) + 1, n / d = (mulhi(n, m) + n) >> s
+// with the sum taken in 64 bits.
+static void fast_div_params(uint32_t d, uint32_t *m, uint32_t *s) {
+  uint32_t sh = 0;
+  while ((uint64_t(1) << sh) < d) ++sh;
+  *m = (uint32_t)(((uint64_t(1) << 32) * ((uint64_t(1) << sh) - d)) / d + 1);
+  *s = sh;
+}
+__device__ __forceinline__ uint32_t fast_div(uint32_t n, uint32_t m, uint32_t s) {
+  return (uint32_t)(((uint64_t)__umulhi(n, m) + n) >> s);
+}
 
 // V'[(v o)..., p'] = sum_p O[o..., p, p'] V[v..., p]  (apply.h:52-66)
 template <typename C>
@@ -156,16 +176,31 @@ __global__ void k_absorb_site(const C *__restrict__ V, const double2 *__restrict
                               C *__restrict__ Vn, size_t n, AbsorbArgs a) {
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
        e += (size_t)gridDim.x * blockDim.x) {
-    size_t rem = e;
     int64_t vb = 0, ob = 0;
-    for (int k = 0; k < a.z; ++k) {
-      const int f = (int)(rem % a.ndim[k]);
-      rem /= a.ndim[k];
-      const int v = f % a.vdim[k], o = f / a.vdim[k];
-      vb += v * a.vstride[k];
-      ob += o * a.ostride[k];
+    int pp;
+    if (a.fast) {
+      uint32_t rem = (uint32_t)e;
+      for (int k = 0; k < a.z; ++k) {
+        const uint32_t q = fast_div(rem, a.nm[k], a.ns[k]);
+        const uint32_t f = rem - q * (uint32_t)a.ndim[k];
+        rem = q;
+        const uint32_t o = fast_div(f, a.vm[k], a.vs[k]);
+        const uint32_t v = f - o * (uint32_t)a.vdim[k];
+        vb += (int64_t)v * a.vstride[k];
+        ob += (int64_t)o * a.ostride[k];
+      }
+      pp = (int)rem;
+    } else {
+      size_t rem = e;
+      for (int k = 0; k < a.z; ++k) {
+        const int f = (int)(rem % a.ndim[k]);
+        rem /= a.ndim[k];
+        const int v = f % a.vdim[k], o = f / a.vdim[k];
+        vb += v * a.vstride[k];
+        ob += o * a.ostride[k];
+      }
+      pp = (int)rem;
     }
-    const int pp = (int)rem;
     ob += pp * a.ostride[a.z + 1];
     double re = 0.0, im = 0.0;
     for (int p = 0; p < a.din; ++p) {
@@ -2234,6 +2269,11 @@ hbp_status hbp_peps_absorb_tpo(hbp_ctx *ctx, hbp_peps *p, const double *const *o
     a.ostride[z] = os;
     a.ostride[z + 1] = os * osh[z];
     a.ndim[z] = a.dout;
+    a.fast = (uint64_t)ns_ * (uint64_t)a.dout < (uint64_t(1) << 32) ? 1 : 0;
+    for (int k = 0; k < z; ++k) {
+      fast_div_params((uint32_t)a.ndim[k], &a.nm[k], &a.ns[k]);
+      fast_div_params((uint32_t)a.vdim[k], &a.vm[k], &a.vs[k]);
+    }
     for (int k = 0; k <= z + 1; ++k) on *= (size_t)osh[k];
     const size_t n = (size_t)ns_ * a.dout;
     p->phys[s] = a.dout;
