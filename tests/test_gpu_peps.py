@@ -136,6 +136,42 @@ def test_site_and_message_round_trip(ctx):
     p.close()
 
 
+def test_absorb_odd_bond_dims_matches_oracle(ctx):
+    """AbsorbTPO's kernel (lazy and eager) against the oracle's contraction
+    (apply.h:52-66) on a 3x3 lattice with bond dims and TPO bond dims that are
+    not powers of two (1..7 and 1..3), so that the index split by
+    multiply-high division meets odd divisors: sites bitwise-close (1e-13)."""
+    from tnbp_amd import circuits as pc
+    from tnbp_amd._lib import check, i32
+    from tnbp_amd.peps import HBP_ABSORB_EAGER, HBP_ABSORB_LAZY, AbsorbTPO, Peps
+    announce("absorb with odd bond dims vs oracle")
+    rng = np.random.default_rng(23)
+    edges = pc.square_lattice(3, 3)
+    dims = [int(x) for x in rng.choice([1, 2, 3, 5, 6, 7], size=len(edges))]
+    ts = [int(x) for x in rng.choice([1, 2, 3], size=len(edges))]
+    bonds = [sorted(m for m, e in enumerate(edges) if s in e) for s in range(9)]
+    sites = [rng.standard_normal([dims[m] for m in bonds[s]] + [2])
+             + 1j * rng.standard_normal([dims[m] for m in bonds[s]] + [2]) for s in range(9)]
+    ops = [rng.standard_normal([ts[m] for m in bonds[s]] + [2, 2])
+           + 1j * rng.standard_normal([ts[m] for m in bonds[s]] + [2, 2]) for s in range(9)]
+    st = ot.init_tensor_product_state(edges, [2] * 9)
+    st.V = [v.copy() for v in sites]
+    st.E = [np.zeros((dims[m % len(edges)],) * 2, dtype=complex) for m in range(2 * len(edges))]
+    ot.absorb_tpo(st, ops)
+    for mode in (HBP_ABSORB_LAZY, HBP_ABSORB_EAGER):
+        p = Peps(ctx, edges, [2] * 9, 0)
+        check(ctx.lib.hbp_peps_set_bond_dims(ctx.h, p.h, i32(dims), i32([2] * 9)))
+        for s in range(9):
+            p.set_site(s, sites[s])
+        p.set_absorb_mode(mode)
+        AbsorbTPO(ctx, ops, p)
+        for s in range(9):
+            got = p.get_site(s)
+            assert got.shape == st.V[s].shape, (mode, s)
+            np.testing.assert_allclose(got, st.V[s], rtol=0, atol=1e-13 * np.abs(st.V[s]).max())
+        p.close()
+
+
 def test_residual_matches_condition(ctx):
     """BeliefPropagationCondition and the fused sweep from random Hermitian PSD
     messages (far from the fixed point), against the oracle."""
