@@ -57,6 +57,31 @@ def test_config1_fp64_matches_golden(ctx, name):
                 assert abs(ra - rb) <= 1e-6 * rb
 
 
+@pytest.mark.parametrize("name", ["config1_kicked_4x4_chi4.json", "config1_fixed3_kicked_4x4_chi4.json"])
+def test_config1_plain_mode_matches_golden(ctx, name):
+    """PLAIN mode (the reference's op sequence one call at a time: a copy of V
+    per message, a norm and a scale after every bond, a conjugated copy of V,
+    a separate condition sweep, bp.h:128-175, :290-350) is the plain FP64 GPU
+    path the mixed path's speed is compared with: iteration counts, bond
+    dims, <X>, <Z>, <ZZ> equal the golden fixture (1e-10 relative, floor 1),
+    and so do its residuals, which keep the reference's phase-sensitive form."""
+    from tnbp_amd.peps import HBP_MODE_PLAIN
+    announce(f"config1 plain mode {name}")
+    g = _golden(name)
+    a = dict(g["args"])
+    a.pop("two_site_svd", None)
+    from tnbp_amd import driver
+    r = driver.run_kicked(ctx, mode=HBP_MODE_PLAIN, **a)
+    for x, y in zip(r["steps"], g["steps"]):
+        assert x["bp_iterations"] == y["bp_iterations"], (x["step"], x["residuals"], y["residuals"])
+        assert x["bond_dims"] == y["bond_dims"]
+        for key in ("x", "z", "zz"):
+            assert rel_close(x[key], y[key], 1e-10) <= 1.0, (key, x["step"])
+        for ra, rb in zip(x["residuals"], y["residuals"]):
+            if rb > 1e-12:
+                assert abs(ra - rb) <= 1e-6 * rb
+
+
 def test_config1_site11(ctx):
     g = _golden("config1_kicked_4x4_chi4.json")
     r = _run_gpu(ctx, g["args"])

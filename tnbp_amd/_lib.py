@@ -16,6 +16,7 @@ HBP_C128, HBP_F64, HBP_C64, HBP_F32 = 0, 1, 2, 3
 HBP_PREC_FP64, HBP_PREC_MIXED = 0, 1
 HBP_ABSORB_AUTO, HBP_ABSORB_LAZY, HBP_ABSORB_EAGER = 0, 1, 2
 HBP_SWEEP_AUTO, HBP_SWEEP_PER_SITE, HBP_SWEEP_GROUPED, HBP_SWEEP_MERGED = 0, 1, 2, 3
+HBP_MODE_FUSED, HBP_MODE_PLAIN = 0, 1
 
 STATUS = {
     1: ValueError,       # HBP_ERR_INVALID_ARGUMENT (std::invalid_argument)

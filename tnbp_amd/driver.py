@@ -14,7 +14,7 @@ import time
 import numpy as np
 
 from . import circuits
-from .peps import (HBP_PREC_FP64, AbsorbTPO, InitTensorProductState, Measure, MeasureEdges,
+from .peps import (HBP_MODE_FUSED, HBP_PREC_FP64, AbsorbTPO, InitTensorProductState, Measure, MeasureEdges,
                    Truncation, bp_loop)
 
 Z = circuits.PAULI["Z"]
@@ -30,12 +30,15 @@ def op2(u):
 def run_kicked(ctx, L, num_steps, Jz, hz, hx, dt, max_bp_iterations, bp_tolerance,
                max_bond_dim, sv_min, lattice="square", two_site_keep_zero=False,
                precision=HBP_PREC_FP64, measure="every", observables=("x", "z", "zz"),
-               log=None):
-    """Kicked-Ising Floquet evolution; returns per-step records like the oracle's."""
+               log=None, mode=HBP_MODE_FUSED):
+    """Kicked-Ising Floquet evolution; returns per-step records like the oracle's.
+    ``mode`` HBP_MODE_PLAIN runs the reference's op sequence one TCI-level
+    call at a time (bp.h:128-175, :290-350) instead of the fused kernels."""
     edges = circuits.square_lattice(L, L) if lattice == "square" else circuits.chain(L)
     sites = sorted({s for e in edges for s in e})
     tpo = circuits.kicked_ising_tpo(edges, Jz, hz, hx, dt * math.pi, two_site_keep_zero)
     peps = InitTensorProductState(ctx, edges, [2] * len(sites), precision)
+    peps.set_mode(mode)
     steps = []
     try:
         for step in range(num_steps):

@@ -14,13 +14,14 @@ import numpy as np
 
 from . import _lib
 from ._lib import (HBP_ABSORB_AUTO, HBP_ABSORB_EAGER, HBP_ABSORB_LAZY, HBP_MAX_ORDER,
+                   HBP_MODE_FUSED, HBP_MODE_PLAIN,
                    HBP_PREC_FP64, HBP_PREC_MIXED, HBP_SWEEP_AUTO, HBP_SWEEP_GROUPED,
                    HBP_SWEEP_MERGED, HBP_SWEEP_PER_SITE, check, i32)
 from .tci import Context
 
 __all__ = ["Peps", "InitTensorProductState", "AbsorbTPO", "BeliefPropagation",
            "BeliefPropagationCondition", "bp_loop", "Truncation", "Measure", "MeasureEdges",
-           "HBP_PREC_FP64", "HBP_PREC_MIXED"]
+           "HBP_PREC_FP64", "HBP_PREC_MIXED", "HBP_MODE_FUSED", "HBP_MODE_PLAIN"]
 
 
 class Peps:
