@@ -162,6 +162,28 @@ hbp_status hbp_trunc_svd(hbp_ctx *ctx, const hbp_tensor *a, int nrows, int chi_m
                          double s_min, hbp_tensor *u, hbp_tensor *s, hbp_tensor *vdag,
                          double *trunc_err);
 
+/* Allocation-free SVD for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference: it is hbp_svd's batched
+ * one-sided Jacobi kernel on caller-owned device buffers, with its work
+ * arrays in a fixed device scratch array of the library (16 MiB per device,
+ * held until the process ends) instead of context allocations.
+ * a: m x n complex128 column-major (device); u: m x n, s: n doubles, vdag:
+ * n x n, all device memory of the caller; singular values descending, the
+ * same bits as hbp_svd on the same matrix.  Needs m >= n, n <= 512 and
+ * (m n + n^2) * 16 bytes <= 16 MiB - 256; larger sizes are rejected with
+ * HBP_ERR_UNSUPPORTED.  Asynchronous on the context's stream (results are
+ * ready after hbp_ctx_synchronize or later work on that stream).  All calls
+ * on one device share its scratch, which keeps the last call's contents:
+ * calls that overlap on one device (two contexts' streams, two threads)
+ * race. */
+hbp_status hbp_svd_noalloc(hbp_ctx *ctx, const void *a, int m, int n, void *u, double *s,
+                           void *vdag);
+/* Device address of a tensor's column-major element array, valid until the
+ * tensor is resized or destroyed: lets callers hand the library's tensors
+ * to their own kernels or to hbp_svd_noalloc.  No reference counterpart. */
+hbp_status hbp_tensor_data(const hbp_tensor *t, void **ptr);
+
 /* ====================================================== PEPS layer ==== */
 /* tnbp::InitTensorProductState (init.h:17-95) on one rank: |0...0> with all
  * bonds of dimension 1 and 1x1 messages equal to 1.  edges holds n_edges

@@ -63,6 +63,8 @@ SIGNATURES = {
     "hbp_get_elem": [_vp, _vp, _i32p, _dp, _dp],
     "hbp_svd": [_vp, _vp, C.c_int, _vp, _vp, _vp],
     "hbp_trunc_svd": [_vp, _vp, C.c_int, C.c_int, C.c_double, _vp, _vp, _vp, _dp],
+    "hbp_svd_noalloc": [_vp, _vp, C.c_int, C.c_int, _vp, _dp, _vp],
+    "hbp_tensor_data": [_vp, C.POINTER(_vp)],
     "hbp_peps_create": [_vp, C.c_int, _i32p, _i32p, C.c_int, C.POINTER(_vp)],
     "hbp_peps_destroy": [_vp, _vp],
     "hbp_peps_set_mode": [_vp, C.c_int],

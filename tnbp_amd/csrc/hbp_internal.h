@@ -302,6 +302,8 @@ struct SvdJob {
 };
 void launch_batched_svd(hipStream_t st, const double2 *A, double2 *U, double *s, double2 *Vd,
                         double2 *work, const SvdJob *jobs_dev, int batch, int max_n);
+// one SvdJob written to device memory by a kernel (no host copy)
+void launch_store_svd_job(hipStream_t st, const SvdJob &job, SvdJob *dst);
 
 // Small batched complex GEMM: C[b] = op(A[b]) op(B[b]) for matrices up to a few
 // hundred on a side (truncation projectors and the generic TCI contract).

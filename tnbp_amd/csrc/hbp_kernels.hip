@@ -1560,6 +1560,13 @@ __global__ __launch_bounds__(256) void k_batched_svd(const double2 *__restrict__
   }
 }
 
+// Writes one job record into device memory from the kernel arguments: lets a
+// caller that may not allocate or copy from the host (hbp_svd_noalloc) place
+// the record in memory it already owns.
+__global__ void k_store_svd_job(SvdJob job, SvdJob *__restrict__ dst) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *dst = job;
+}
+
 // --------------------------------------------------------- batched GEMM
 __device__ inline double2 gemm_elem(const double2 *X, int ld, int op, int r, int c) {
   // element (r, c) of op(X)
@@ -2234,6 +2241,11 @@ void launch_batched_svd(hipStream_t st, const double2 *A, double2 *U, double *s,
   HBP_REQUIRE(max_n <= 512, HBP_ERR_UNSUPPORTED, "svd: more than 512 columns");
   if (batch <= 0) return;
   hipLaunchKernelGGL(k_batched_svd, dim3(batch), dim3(256), 0, st, A, U, s, Vd, work, jobs_dev);
+  HBP_HIP(hipGetLastError());
+}
+
+void launch_store_svd_job(hipStream_t st, const SvdJob &job, SvdJob *dst) {
+  hipLaunchKernelGGL(k_store_svd_job, dim3(1), dim3(64), 0, st, job, dst);
   HBP_HIP(hipGetLastError());
 }
 

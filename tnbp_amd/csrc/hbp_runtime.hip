@@ -23,6 +23,35 @@ namespace {
 thread_local std::string g_last_error;
 }
 
+// Device scratch of hbp_svd_noalloc: its job record (first 256 bytes) and the
+// one-sided Jacobi work arrays (a copy of A, m x n, and the rotation
+// accumulator, n x n, complex128).  Every device the library runs on holds
+// one copy until the process ends.  In host code `scratch` only names the
+// symbol: its device address on each device comes from hipGetSymbolAddress
+// (bind_noalloc_scratch), and only that address is passed to kernels.
+static constexpr size_t kNoAllocScratchBytes = (size_t)16 << 20;
+static constexpr size_t kNoAllocJobBytes = 256;
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {nullptr};
+
+// Looks up this device's copy of `scratch` once; the device must be current.
+void bind_noalloc_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || ptr == nullptr)
+    throw ::hbp::Error(HBP_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                        hipGetErrorString(e));
+  g_scratch_ptr[device] = ptr;
+}
+}  // namespace
+
 #define HBP_API_BEGIN try {
 #define HBP_API_END                                                    \
   }                                                                    \
@@ -583,6 +612,7 @@ hbp_status hbp_ctx_create(int device, hbp_ctx **out) {
   HBP_HIP(hipGetDeviceCount(&n));
   HBP_REQUIRE(device >= 0 && device < n, HBP_ERR_INVALID_ARGUMENT, "no such device");
   HBP_HIP(hipSetDevice(device));
+  bind_noalloc_scratch(device);
   hbp_ctx *c = new hbp_ctx();
   c->device = device;
   HBP_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -987,6 +1017,39 @@ hbp_status hbp_svd(hbp_ctx *ctx, const hbp_tensor *a, int nrows, hbp_tensor *u, 
                    hbp_tensor *vdag) {
   HBP_API_BEGIN
   svd_impl(ctx, a, nrows, u, s, vdag, -1, 0.0, nullptr);
+  HBP_API_END
+}
+
+hbp_status hbp_svd_noalloc(hbp_ctx *ctx, const void *a, int m, int n, void *u, double *s,
+                           void *vdag) {
+  HBP_API_BEGIN
+  HBP_REQUIRE(ctx && a && u && s && vdag, HBP_ERR_INVALID_ARGUMENT, "svd_noalloc: null argument");
+  HBP_REQUIRE(m >= 1 && n >= 1, HBP_ERR_INVALID_ARGUMENT, "svd_noalloc: empty matrix");
+  HBP_REQUIRE(m >= n, HBP_ERR_UNSUPPORTED,
+              "svd_noalloc: needs rows >= columns (pass the conjugate transpose and swap the "
+              "factors, or use hbp_svd)");
+  HBP_REQUIRE(n <= 512, HBP_ERR_UNSUPPORTED, "svd_noalloc: more than 512 columns");
+  const size_t work = ((size_t)m * n + (size_t)n * n) * sizeof(double2);
+  HBP_REQUIRE(work <= kNoAllocScratchBytes - kNoAllocJobBytes, HBP_ERR_UNSUPPORTED,
+              "svd_noalloc: (m n + n^2) complex128 work exceeds its 16 MiB device scratch "
+              "(m = " + std::to_string(m) + ", n = " + std::to_string(n) + ")");
+  HBP_REQUIRE(ctx->device >= 0 && ctx->device < kMaxDevices && g_scratch_ptr[ctx->device],
+              HBP_ERR_LOGIC, "svd_noalloc: device scratch not bound");
+  HBP_HIP(hipSetDevice(ctx->device));
+  char *base = (char *)g_scratch_ptr[ctx->device];
+  SvdJob *dj = (SvdJob *)base;
+  double2 *W = (double2 *)(base + kNoAllocJobBytes);
+  // the kernel initialises every work element it reads (A's copy, V = I)
+  launch_store_svd_job(ctx->stream, SvdJob{0, 0, 0, 0, 0, m, n}, dj);
+  launch_batched_svd(ctx->stream, (const double2 *)a, (double2 *)u, s, (double2 *)vdag, W, dj, 1,
+                     n);
+  HBP_API_END
+}
+
+hbp_status hbp_tensor_data(const hbp_tensor *t, void **ptr) {
+  HBP_API_BEGIN
+  HBP_REQUIRE(t && ptr, HBP_ERR_INVALID_ARGUMENT, "tensor_data: null argument");
+  *ptr = t->data;
   HBP_API_END
 }
 
