@@ -76,10 +76,16 @@ def test_parse_edges_and_slices():
     # a trailing barrier opens no slice
     assert len(qasm.qasm_to_tpo(p, qasm.edges_from_qasm(p))) == 2
     assert len(qasm.qasm_to_tpo(p, qasm.edges_from_qasm(p), num_gates=[3, 7])) == 2
-    for bad in ("qreg q[2]; cx q[0],q[0];", "qreg q[2]; measure q[0] -> c[0];",
-                "qreg q[2]; foo q[0];", "qreg q[2]; rx q[0];", "qreg q[2]; x q[5];"):
+    for bad in ("qreg q[2]; cx q[0],q[0];", "qreg q[2]; rx q[0];", "qreg q[2]; x q[5];",
+                "qreg q[2]; cx q[0];"):
         with pytest.raises(ValueError):
             qasm.parse(bad)
+    # parsed like the reference, but no operator on the TPO path
+    for bad in ("qreg q[2]; creg c[2]; cx q[0],q[1]; measure q[0] -> c[0];",
+                "qreg q[2]; cx q[0],q[1]; foo q[0];", "qreg q[2]; cx q[0],q[1]; reset q;"):
+        p = qasm.parse(bad)
+        with pytest.raises(ValueError, match="no operator"):
+            qasm.qasm_to_tpo(p, qasm.edges_from_qasm(p))
 
 
 def _full_operator(edges, O):
@@ -145,3 +151,58 @@ def test_opt_tpo_by_svd():
     assert [o.shape for o in ref] == [o.shape for o in opt]
     np.testing.assert_allclose(_full_operator(edges, ref), _full_operator(edges, first), atol=1e-12)
     _check_slices_against_statevector(p, edges, [circuits.opt_tpo_by_svd(edges, s) for s in slices])
+
+
+def _embed(u, qs, n):
+    """The n-qubit matrix of gate u on qubits qs (first listed = low bit)."""
+    k = len(qs)
+    t = np.asarray(u).reshape([2] * (2 * k))        # [out_{k-1}..out_0, in_{k-1}..in_0]
+    full = np.eye(2 ** n, dtype=complex).reshape([2] * (2 * n))
+    res = np.tensordot(t, full, axes=([2 * k - 1 - i for i in range(k)], [n - 1 - q for q in qs]))
+    res = np.moveaxis(res, list(range(k)), [n - 1 - qs[k - 1 - j] for j in range(k)])
+    return res.reshape(2 ** n, 2 ** n)
+
+
+def test_three_qubit_and_path_gates_build_the_exact_operator():
+    """CCX / CSWAP on qubits that are not neighbours, routed along the
+    shortest lattice path through the middle qubit (qasmtotpo.h:439-698), and
+    a two-qubit gate between non-neighbours (qasmtotpo.h:333-436): the slice's
+    operator equals the dense product of the gates, in order, on a chain and
+    on a 2x3 lattice."""
+    rng = np.random.default_rng(8)
+    u3 = lambda: qasm.gate_matrix("u3", list(rng.uniform(0, 6, 3)))
+    for edges, gates in (
+        (circuits.chain(5), [(u3(), (0,)), (qasm.gate_matrix("ccx"), (0, 2, 4)), (u3(), (2,)),
+                             (qasm.gate_matrix("cswap"), (3, 1, 2)), (qasm.gate_matrix("cx"), (4, 1))]),
+        (circuits.square_lattice(3, 2), [(u3(), (4,)), (qasm.gate_matrix("cswap"), (0, 5, 2)),
+                                         (qasm.gate_matrix("rzz", [0.7]), (1, 5)),
+                                         (qasm.gate_matrix("ccx"), (3, 4, 1))]),
+    ):
+        n = len({s for e in edges for s in e})
+        want = np.eye(2 ** n, dtype=complex)
+        for u, q in gates:
+            want = _embed(u, q, n) @ want
+        O = circuits.layer_tpo(edges, gates)
+        G = _full_operator(edges, O)                      # (in_0..in_{n-1}, out_0..out_{n-1})
+        got = np.transpose(G, list(range(2 * n - 1, n - 1, -1)) + list(range(n - 1, -1, -1)))
+        np.testing.assert_allclose(got.reshape(2 ** n, 2 ** n), want, atol=1e-13)
+
+
+def test_three_qubit_gate_matrices():
+    """CCX is the reference's matrix (qasmtotpo.h:117-126); CSWAP is the
+    unitary that swaps the last two listed qubits when the first is 1 (the
+    reference's table, qasmtotpo.h:127-136, puts its m[5][5] where m[4][4]
+    belongs)."""
+    ccx = qasm.gate_matrix("ccx")
+    ref = np.eye(8)
+    ref[3, 3] = ref[7, 7] = 0
+    ref[3, 7] = ref[7, 3] = 1
+    np.testing.assert_array_equal(ccx, ref)
+    cs = qasm.gate_matrix("cswap")
+    np.testing.assert_allclose(cs @ cs.conj().T, np.eye(8), atol=0)
+    for i in range(8):
+        c, a, b = i & 1, (i >> 1) & 1, (i >> 2) & 1
+        j = (c | (b << 1) | (a << 2)) if c else i
+        assert cs[j, i] == 1
+    p = qasm.parse("OPENQASM 3.0;\nqubit[3] q;\ntoffoli q[0], q[1], q[2];\ncswap q[2], q[0], q[1];\n")
+    assert [(i.op, i.qubits) for i in p.instructions] == [("ccx", [0, 1, 2]), ("cswap", [2, 0, 1])]

@@ -1,22 +1,23 @@
-"""OpenQASM 2 front end: program -> lattice edges -> TPO slices.
+"""OpenQASM 2 / 3 front end: program -> lattice edges -> TPO slices.
 
 The reference's estimator reads a QASM file into ``qasm::Program``
-(include/qasm/ir.h:10-40; its parser directories include/qasm/v2, v3 are not
-part of the reference tree), takes the lattice from the circuit's two-qubit
-gates (``EdgesFromQasm``, include/tnbp/parser/qasmtoedge.h:38-57), builds one
-TPO slice per barrier-separated block or per ``num_gates`` count
-(``QasmToTPO``, qasmtotpo.h:700-767) from the gate matrices of
-``InstructionMatrix`` (qasmtotpo.h:19-144), and optionally compresses each
-slice with ``OptTPObySVD`` (optsvdtpo.h:87-214).  This module does the same on
-the host (the operators are a few hundred numbers per site); the slices feed
-``AbsorbTPO``.
+(include/qasm/ir.h:10-40) with its own parsers (include/qasm/any.h picks
+v2/parser.h or v3/parser.h from the header), takes the lattice from the
+circuit's two-qubit gates (``EdgesFromQasm``, include/tnbp/parser/
+qasmtoedge.h:38-57), builds one TPO slice per barrier-separated block or per
+``num_gates`` count (``QasmToTPO``, qasmtotpo.h:700-767) from the gate
+matrices of ``InstructionMatrix`` (qasmtotpo.h:19-144), and optionally
+compresses each slice with ``OptTPObySVD`` (optsvdtpo.h:87-214).  This module
+does the same on the host (the operators are a few hundred numbers per site);
+the slices feed ``AbsorbTPO``.
 
-Parsed: ``OPENQASM``, ``include``, ``qreg``/``creg``, gate applications with
-parameter expressions (numbers, ``pi``, + - * / ^, unary minus, parentheses,
-sin cos tan exp ln sqrt) on indexed qubits, ``barrier``.  Gates: u3/u, u2,
-u1, rx, ry, rz, h, x, y, z, s, sdg, t, tdg, id, cx, cz, swap, rzz.  Like the
-reference, a gate without a matrix (measure, reset, custom gates, three-qubit
-gates on this TPO path) is an error.
+Parsed: the subset of the reference's parsers (see ``parse``), with parameter
+expressions of numbers, ``pi``, ``tau``, + - * / ^, unary signs, parentheses
+and sin cos tan exp ln sqrt.  Gates: u3/u, u2, u1/p, rx, ry, rz, h, x, y, z,
+s, sdg, t, tdg, id, cx/cnot, cz, swap, rzz, ccx/toffoli, cswap.  Two-qubit
+gates between non-neighbours and three-qubit gates act along the shortest
+lattice path (identities in between).  Like the reference, an instruction
+without a matrix (measure, reset, custom gates) is an error on the TPO path.
 """
 from __future__ import annotations
 
@@ -36,6 +37,7 @@ class Instruction:
     params: list
     qubits: list          # global qubit indices (registers laid out in declaration order)
     line: int = 0
+    name: str = ""        # the gate's name for "custom" instructions
 
 
 @dataclass
@@ -81,6 +83,21 @@ def gate_matrix(op, params=()):
     }
     if op in fixed:
         return np.array(fixed[op], dtype=c)
+    if op == "ccx":
+        # Toffoli, controls on the first two listed qubits (qasmtotpo.h:117-126)
+        m = np.eye(8, dtype=c)
+        m[3, 3] = m[7, 7] = 0
+        m[3, 7] = m[7, 3] = 1
+        return m
+    if op == "cswap":
+        # Fredkin: control on the first listed qubit, which swaps the other two:
+        # |011> <-> |101> in the first-qubit-lowest index (qasmtotpo.h:127-136
+        # sets the swap, but also m[5][5] = 1 where m[4][4] = 1 belongs, which
+        # loses |100>; this is the unitary its comment describes)
+        m = np.eye(8, dtype=c)
+        m[3, 3] = m[5, 5] = 0
+        m[3, 5] = m[5, 3] = 1
+        return m
     m = np.zeros((4, 4), dtype=c)
     if op == "cx":
         m[0, 0] = m[2, 2] = m[1, 3] = m[3, 1] = 1
@@ -133,8 +150,10 @@ def eval_expr(text):
             return v
         if k == "id":
             take()
-            if v == "pi":
+            if v.lower() == "pi":
                 return math.pi
+            if v.lower() == "tau":
+                return 2 * math.pi
             if v in _FUNCS:
                 take("op", "(")
                 x = expr()
@@ -188,13 +207,21 @@ def eval_expr(text):
 # ------------------------------------------------------------------ parser
 _STMT = re.compile(r"^([A-Za-z_]\w*)\s*(?:\((.*)\))?\s*(.*)$", re.S)
 
+# gate names of the reference's parsers (include/qasm/v2/parser.h:119-143,
+# v3/parser.h:324-349), case-insensitive
+GATE_NAMES = {"u3": "u3", "u": "u3", "u2": "u2", "u1": "u1", "p": "u1", "rx": "rx", "ry": "ry",
+              "rz": "rz", "h": "h", "x": "x", "y": "y", "z": "z", "s": "s", "sdg": "sdg", "t": "t",
+              "tdg": "tdg", "id": "id", "cx": "cx", "cnot": "cx", "cz": "cz", "swap": "swap",
+              "rzz": "rzz", "ccx": "ccx", "toffoli": "ccx", "cswap": "cswap"}
+THREE_QUBIT = {"ccx", "cswap"}
+
 
 def _split_top(s, sep=","):
     out, depth, cur = [], 0, []
     for ch in s:
-        if ch == "(":
+        if ch in "([":
             depth += 1
-        elif ch == ")":
+        elif ch in ")]":
             depth -= 1
         if ch == sep and depth == 0:
             out.append("".join(cur))
@@ -205,13 +232,63 @@ def _split_top(s, sep=","):
     return [x.strip() for x in out if x.strip()]
 
 
+def _version(text):
+    """3 when a '3' follows the OPENQASM header within 32 characters, else 2
+    (the header sniff of include/qasm/any.h:8-20)."""
+    pos = text.find("OPENQASM")
+    return 3 if pos >= 0 and "3" in text[pos:pos + 32] else 2
+
+
 def parse(text):
-    """Program of an OpenQASM 2 text (the subset above)."""
+    """Program of an OpenQASM 2 or 3 text (the subset the reference parses,
+    include/qasm/v2/parser.h, v3/parser.h): declarations (qreg/creg; qubit and
+    bit with or without a size in version 3), gate calls with parameter
+    expressions, register broadcast (`h q;` is one gate per qubit, a call's
+    qubit list is cut into groups of the gate's arity) and, in version 3,
+    slices q[lo:hi]; barrier, reset and measure (arrow form, and the
+    assignment forms of version 3).  Unknown gate names become "custom"
+    instructions; like the reference, those, measure and reset have no
+    operator on the TPO path (qasm_to_tpo rejects them).  Qubits are numbered
+    globally, registers in declaration order (with one register, the
+    reference's QArg index)."""
+    version = _version(text)
+    text = re.sub(r"/\*.*?\*/", lambda m: "\n" * m.group(0).count("\n"), text, flags=re.S)
     text = re.sub(r"//[^\n]*", "", text)
     prog = Program()
-    offset = {}
-    size = {}
+    offset, size = {}, {}
+    csize = {}
     line = 1
+
+    def qargs(spec, ln, many=True):
+        out = []
+        for a in _split_top(spec):
+            mm = re.match(r"^([A-Za-z_]\w*)\s*\[\s*(\d+)\s*(?::\s*(\d+)\s*)?\]$", a)
+            if mm:
+                reg, lo = mm.group(1), int(mm.group(2))
+                if reg not in offset:
+                    raise ValueError(f"QASM line {ln}: unknown qubit register {reg!r}")
+                if mm.group(3) is not None:
+                    hi = int(mm.group(3))
+                    if version != 3 or hi < lo or hi > size[reg]:
+                        raise ValueError(f"QASM line {ln}: bad slice {a!r}")
+                    out += [offset[reg] + k for k in range(lo, hi)]
+                else:
+                    if lo >= size[reg]:
+                        raise ValueError(f"QASM line {ln}: qubit {a!r} out of range")
+                    out.append(offset[reg] + lo)
+            elif a in offset:
+                out += list(range(offset[a], offset[a] + size[a]))
+            else:
+                raise ValueError(f"QASM line {ln}: bad qubit argument {a!r}")
+        if not many and len(out) != 1:
+            raise ValueError(f"QASM line {ln}: one qubit expected in {spec!r}")
+        return out
+
+    def declare_q(name, n):
+        offset[name] = sum(k for _, k in prog.qregs)
+        size[name] = n
+        prog.qregs.append((name, n))
+
     for raw in text.split(";"):
         stmt = raw.strip()
         nl = raw.count("\n")
@@ -225,50 +302,66 @@ def parse(text):
         m = re.match(r"^(qreg|creg)\s+([A-Za-z_]\w*)\s*\[\s*(\d+)\s*\]$", stmt)
         if m:
             if m.group(1) == "qreg":
-                offset[m.group(2)] = sum(n for _, n in prog.qregs)
-                size[m.group(2)] = int(m.group(3))
-                prog.qregs.append((m.group(2), int(m.group(3))))
+                declare_q(m.group(2), int(m.group(3)))
+            else:
+                csize[m.group(2)] = int(m.group(3))
+            continue
+        if version == 3:
+            m = re.match(r"^(qubit|bit)\s*(?:\[\s*(\d+)\s*\])?\s+([A-Za-z_]\w*)$", stmt)
+            if m:
+                n = int(m.group(2)) if m.group(2) else 1
+                if m.group(1) == "qubit":
+                    declare_q(m.group(3), n)
+                else:
+                    csize[m.group(3)] = n
+                continue
+            m = re.match(r"^(?:bit\s*(?:\[\s*\d+\s*\])?\s+)?[A-Za-z_]\w*\s*(?:\[\s*\d+\s*\])?\s*=\s*"
+                         r"measure\s+(.+)$", stmt, re.S)
+            if m:
+                for q in qargs(m.group(1), ln):
+                    prog.instructions.append(Instruction("measure", [], [q], ln))
+                continue
+        m = re.match(r"^measure\s+(.+?)\s*->\s*(.+)$", stmt, re.S)
+        if m:
+            prog.instructions.append(Instruction("measure", [], qargs(m.group(1), ln, False), ln))
             continue
         m = _STMT.match(stmt)
         if not m:
             raise ValueError(f"QASM parse error (line {ln}): {stmt!r}")
-        op, ptext, args = m.group(1).lower(), m.group(2), m.group(3)
-        if op == "u":
-            op = "u3"
-        if op in ("measure", "reset", "gate", "opaque", "if"):
-            raise ValueError(f"QASM line {ln}: {op!r} has no operator on the TPO path")
-        params = [eval_expr(p) for p in _split_top(ptext)] if ptext else []
-        qubits = []
-        for a in _split_top(args):
-            mm = re.match(r"^([A-Za-z_]\w*)\s*\[\s*(\d+)\s*\]$", a)
-            if mm:
-                reg, idx = mm.group(1), int(mm.group(2))
-                if reg not in offset or idx >= size[reg]:
-                    raise ValueError(f"QASM line {ln}: unknown qubit {a!r}")
-                qubits.append(offset[reg] + idx)
-            elif a in offset and op == "barrier":
-                qubits += list(range(offset[a], offset[a] + size[a]))
-            else:
-                raise ValueError(f"QASM line {ln}: bad argument {a!r}")
-        if op == "barrier":
-            prog.instructions.append(Instruction("barrier", [], qubits, ln))
+        word, ptext, args = m.group(1), m.group(2), m.group(3)
+        low = word.lower()
+        if low in ("ctrl", "inv", "pow", "gate", "opaque", "if"):
+            raise ValueError(f"QASM line {ln}: {word!r} is outside the parsed subset")
+        if low == "barrier":
+            prog.instructions.append(Instruction("barrier", [], qargs(args, ln), ln))
             continue
-        if op not in ONE_QUBIT and op not in TWO_QUBIT:
-            raise ValueError(f"QASM line {ln}: unsupported gate {op!r}")
-        want = 1 if op in ONE_QUBIT else 2
-        if len(qubits) != want or len(params) != NPARAMS.get(op, 0):
-            raise ValueError(f"QASM line {ln}: {op} takes {want} qubit(s) and "
-                             f"{NPARAMS.get(op, 0)} parameter(s)")
-        if want == 2 and qubits[0] == qubits[1]:
-            raise ValueError(f"QASM line {ln}: repeated qubit")
-        prog.instructions.append(Instruction(op, params, qubits, ln))
+        if low == "reset":
+            for q in qargs(args, ln):
+                prog.instructions.append(Instruction("reset", [], [q], ln))
+            continue
+        params = [eval_expr(x) for x in _split_top(ptext)] if ptext else []
+        qs = qargs(args, ln)
+        op = GATE_NAMES.get(low)
+        if op is None:
+            prog.instructions.append(Instruction("custom", params, qs, ln, name=word))
+            continue
+        ar = 1 if op in ONE_QUBIT else 2 if op in TWO_QUBIT else 3
+        if len(qs) % ar or not qs:
+            raise ValueError(f"QASM line {ln}: {op} expects a multiple of {ar} qubits")
+        if len(params) != NPARAMS.get(op, 0):
+            raise ValueError(f"QASM line {ln}: {op} takes {NPARAMS.get(op, 0)} parameter(s)")
+        for k in range(0, len(qs), ar):
+            group = qs[k:k + ar]
+            if len(set(group)) != ar:
+                raise ValueError(f"QASM line {ln}: repeated qubit in {op}")
+            prog.instructions.append(Instruction(op, list(params), group, ln))
     return prog
 
 
 # ------------------------------------------------------- program -> TPO
 def sites_from_qasm(prog):
     """``SitesFromQasm`` (qasmtoedge.h:19-32)."""
-    return sorted({q for ins in prog.instructions if ins.op != "barrier" for q in ins.qubits})
+    return sorted({q for ins in prog.instructions if ins.op not in ("barrier", "custom") for q in ins.qubits})
 
 
 def edges_from_qasm(prog):
@@ -297,5 +390,10 @@ def qasm_to_tpo(prog, edges, num_gates=None):
             else:
                 cur.append(ins)
         slices.append(cur)
+    for sl in slices:
+        for i in sl:
+            if i.op in ("measure", "reset", "custom"):
+                what = i.name or i.op
+                raise ValueError(f"QASM line {i.line}: {what!r} has no operator on the TPO path")
     return [circuits.layer_tpo(edges, [(gate_matrix(i.op, i.params), tuple(i.qubits)) for i in s])
             for s in slices]
