@@ -17,13 +17,17 @@ clock starts.
 
 `value` = 736 x jobs / wall seconds of the timed steps on the headline path
 (`--precision`, default mixed: FP32 MFMA contractions, FP64 messages,
-normalisation, residual (floor 1e-6), truncation and ratios).  During the
+normalisation, residual (floor 1e-6), truncation and ratios; `--schedule`,
+default checkerboard: the messages leaving one colour of the bipartite
+lattice are updated before those leaving the other, which reaches the same
+BP fixed point in about half the sweeps of the reference's Jacobi schedule).  During the
 first timed job the 736 values are also taken, untimed (the clock stops), at
 every step, to compare with the FP64 leg.
 
 After the headline line is printed (and flushed) further legs run and a
 second, complete line is printed:
-  * `fp64_fused`: the same job on the FP64 fused path within a time budget
+  * `fp64_fused`: the same job on the FP64 fused path (Jacobi, the
+    reference's schedule) within a time budget
     (`--fp64-budget` s): steps completed, per-step times and BP iterations,
     and the mixed path's error against it on all 736 values at every step
     both completed;
@@ -66,6 +70,9 @@ def parse():
     ap.add_argument("--chi", type=int, default=32)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
     ap.add_argument("--sweep", choices=["auto", "per-site", "merged"], default="auto")
+    ap.add_argument("--schedule", choices=["jacobi", "checkerboard"], default="checkerboard",
+                    help="BP update schedule of the headline path (the FP64 leg stays Jacobi, "
+                         "the reference's)")
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--max-bp-iterations", type=int, default=50,
                     help="the kicked driver's default (apps/kicked_ising_lattice/option.h:19)")
@@ -110,12 +117,13 @@ class Job:
 class Runner:
     """One PEPS on one context, stepping the job."""
 
-    def __init__(self, ctx, job, precision, sweep):
+    def __init__(self, ctx, job, precision, sweep, schedule="jacobi"):
         from tnbp_amd.peps import HBP_PREC_FP64, HBP_PREC_MIXED, InitTensorProductState
         self.ctx, self.job = ctx, job
         prec = HBP_PREC_FP64 if precision == "fp64" else HBP_PREC_MIXED
         self.peps = InitTensorProductState(ctx, job.edges, [2] * len(job.sites), prec)
         self.peps.set_sweep_kernel({"auto": 0, "per-site": 1, "merged": 3}[sweep])
+        self.peps.set_schedule({"jacobi": 0, "checkerboard": 1}[schedule])
         self.tag = precision
 
     def step(self, args):
@@ -165,7 +173,7 @@ def run_headline(ctx, job, args):
     """Warm-up, then `args.steps` timed Floquet steps as jobs of 20 steps."""
     from tnbp_amd._lib import check
     if args.warmup > 0:
-        w = Runner(ctx, job, args.precision, args.sweep)
+        w = Runner(ctx, job, args.precision, args.sweep, args.schedule)
         for k in range(args.warmup):
             rec = w.step(args)
             log({"warmup_step": k + 1, **rec})
@@ -180,7 +188,7 @@ def run_headline(ctx, job, args):
     while left > 0:
         n = min(JOB_STEPS, left)
         left -= n
-        r = Runner(ctx, job, args.precision, args.sweep)
+        r = Runner(ctx, job, args.precision, args.sweep, args.schedule)
         ctx.synchronize()
         t_job = time.perf_counter()
         paused = 0.0
@@ -432,7 +440,8 @@ def main():
                                f"BP tol {args.tol:g}{floor}, at most {args.max_bp_iterations} BP "
                                f"iterations, 256 <Z> + 480 <ZZ> at the end of the job; a bench step "
                                f"is one Floquet step",
-                   "precision": args.precision, "sweep": args.sweep, "L": job.L, "chi": job.chi,
+                   "precision": args.precision, "sweep": args.sweep, "schedule": args.schedule,
+                   "L": job.L, "chi": job.chi,
                    "jobs_timed": n_jobs, "values_per_job": job.n_values},
         "latency_s": {"per_job": {"n": n_jobs, "p50": float(np.percentile(job_lat, 50)),
                                   "p99": float(np.percentile(job_lat, 99)), "max": job_lat[-1]},

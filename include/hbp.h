@@ -183,6 +183,11 @@ hbp_status hbp_svd_noalloc(hbp_ctx *ctx, const void *a, int m, int n, void *u, d
  * tensor is resized or destroyed: lets callers hand the library's tensors
  * to their own kernels or to hbp_svd_noalloc.  No reference counterpart. */
 hbp_status hbp_tensor_data(const hbp_tensor *t, void **ptr);
+/* A value that changes with every write to the tensor through this C-ABI
+ * (process-wide unique), so that a caller can tell whether a tensor changed
+ * since it last looked (the fused TCI overloads cache PEPS state by it).  No
+ * reference counterpart. */
+hbp_status hbp_tensor_version(const hbp_tensor *t, uint64_t *version);
 
 /* ====================================================== PEPS layer ==== */
 /* tnbp::InitTensorProductState (init.h:17-95) on one rank: |0...0> with all
@@ -204,6 +209,26 @@ hbp_status hbp_peps_set_mode(hbp_peps *p, int mode);
 /* Mixed precision only: the BP loop also stops once the residual is below
  * this floor (default 1e-6), since FP32 messages cannot reach FP64 tolerances. */
 hbp_status hbp_peps_set_residual_floor(hbp_peps *p, double floor);
+/* FUSED mode: BP update schedule.  JACOBI (default) is the reference's: every
+ * message of a sweep from the previous sweep's messages (bp.h:65-261, E <- F
+ * after all layers, main.cc:138-141).  CHECKERBOARD, on a lattice whose graph
+ * is bipartite (square lattices, chains, trees; others fall back to JACOBI),
+ * first updates the messages leaving the sites of one colour, then those
+ * leaving the other colour from the new ones.  A message leaving a colour-c
+ * site depends only on messages leaving the other colour, so Jacobi sweeps
+ * advance two independent sequences one step each, and a checkerboard sweep
+ * (the same work) advances one of them two steps: about half the sweeps to
+ * the same fixed point.  Iteration counts then differ from the reference's;
+ * converged results agree within the BP tolerance.  No reference counterpart. */
+/* FUSED mode, merged sweep: whether the sites whose target pair sits behind
+ * other bonds are kept rearranged (pair first) for the whole BP loop, about
+ * one more absorbed lattice of memory.  -1 (default): when it fits with
+ * max(16 GiB, 15 %) of the device to spare; 0: never (each sweep rearranges
+ * into a work buffer); 1: always (for callers that have budgeted the memory,
+ * such as the serving layer's admission).  No reference counterpart. */
+hbp_status hbp_peps_set_pair_cache(hbp_peps *p, int policy);
+enum { HBP_SCHEDULE_JACOBI = 0, HBP_SCHEDULE_CHECKERBOARD = 1 };
+hbp_status hbp_peps_set_schedule(hbp_peps *p, int schedule);
 /* FUSED mode only (default on): fix each message's free scalar phase, trace
  * real and positive, and compare messages modulo that phase in the residual.
  * bp.h:321-350 compares phases too; rounding seeds them and each sweep
@@ -254,6 +279,27 @@ hbp_status hbp_peps_set_bond_dims(hbp_ctx *ctx, hbp_peps *p, const int32_t *bond
 hbp_status hbp_peps_get_message(hbp_ctx *ctx, hbp_peps *p, int m, int32_t *dim, double *host);
 hbp_status hbp_peps_set_message(hbp_ctx *ctx, hbp_peps *p, int m, int32_t dim,
                                 const double *host);
+
+/* Site tensors and messages held in a caller's tensors (the C++ TCI binding's
+ * fused overloads of tnbp's algorithm functions use these to run the PEPS
+ * layer on the drivers' own V and E vectors).  No reference counterpart.
+ * bind_sites: site s reads the caller's tensor sites[s] (bonds in ascending
+ * edge order, then the physical index; element type of the PEPS precision)
+ * without a copy; the PEPS never writes or frees a bound tensor, and a bound
+ * tensor must outlive its binding (until the site is replaced or bound again).
+ * Bond dims are taken from the tensors; messages are re-laid out (zeros)
+ * when they change.
+ * load_messages: copies the 2 n_e complex128 D x D message tensors into the
+ * PEPS (msgs[m] = E[m] in the reference's numbering, bp.h:170-174).
+ * store_messages: copies messages which[i] (or i when which is NULL) into
+ * out[i], resizing them.
+ * store_sites: writes every site tensor (absorbed form where pending) into
+ * out[s] as complex128, resizing it; in FP64 the site is then bound to out[s]. */
+hbp_status hbp_peps_bind_sites(hbp_ctx *ctx, hbp_peps *p, const hbp_tensor *const *sites);
+hbp_status hbp_peps_load_messages(hbp_ctx *ctx, hbp_peps *p, const hbp_tensor *const *msgs);
+hbp_status hbp_peps_store_messages(hbp_ctx *ctx, hbp_peps *p, int n, const int32_t *which,
+                                   hbp_tensor *const *out);
+hbp_status hbp_peps_store_sites(hbp_ctx *ctx, hbp_peps *p, hbp_tensor *const *out);
 
 /* tnbp::AbsorbTPO (apply.h:14-177): per site O_s(t_1..t_z, d_in, d_out) given
  * as complex128 column-major host arrays (ops[s] with shape tpo_shapes[s*8..]),

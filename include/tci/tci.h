@@ -31,5 +31,8 @@
 #include "tci/traits.h"
 #include "tci/hbp/tensor.h"
 #include "tci/hbp/functions.h"
+// tnbp's algorithm functions on the fused PEPS layer (overloads; off with
+// HBP_TCI_NO_FUSED or without mpi.h)
+#include "tci/hbp/fused.h"
 
 #endif

@@ -17,6 +17,7 @@ HBP_PREC_FP64, HBP_PREC_MIXED = 0, 1
 HBP_ABSORB_AUTO, HBP_ABSORB_LAZY, HBP_ABSORB_EAGER = 0, 1, 2
 HBP_SWEEP_AUTO, HBP_SWEEP_PER_SITE, HBP_SWEEP_GROUPED, HBP_SWEEP_MERGED = 0, 1, 2, 3
 HBP_MODE_FUSED, HBP_MODE_PLAIN = 0, 1
+HBP_SCHEDULE_JACOBI, HBP_SCHEDULE_CHECKERBOARD = 0, 1
 
 STATUS = {
     1: ValueError,       # HBP_ERR_INVALID_ARGUMENT (std::invalid_argument)
@@ -65,10 +66,17 @@ SIGNATURES = {
     "hbp_trunc_svd": [_vp, _vp, C.c_int, C.c_int, C.c_double, _vp, _vp, _vp, _dp],
     "hbp_svd_noalloc": [_vp, _vp, C.c_int, C.c_int, _vp, _dp, _vp],
     "hbp_tensor_data": [_vp, C.POINTER(_vp)],
+    "hbp_tensor_version": [_vp, C.POINTER(C.c_uint64)],
+    "hbp_peps_bind_sites": [_vp, _vp, C.POINTER(_vp)],
+    "hbp_peps_load_messages": [_vp, _vp, C.POINTER(_vp)],
+    "hbp_peps_store_messages": [_vp, _vp, C.c_int, _i32p, C.POINTER(_vp)],
+    "hbp_peps_store_sites": [_vp, _vp, C.POINTER(_vp)],
     "hbp_peps_create": [_vp, C.c_int, _i32p, _i32p, C.c_int, C.POINTER(_vp)],
     "hbp_peps_destroy": [_vp, _vp],
     "hbp_peps_set_mode": [_vp, C.c_int],
     "hbp_peps_set_residual_floor": [_vp, C.c_double],
+    "hbp_peps_set_schedule": [_vp, C.c_int],
+    "hbp_peps_set_pair_cache": [_vp, C.c_int],
     "hbp_peps_set_phase_gauge": [_vp, C.c_int],
     "hbp_peps_set_absorb_mode": [_vp, C.c_int],
     "hbp_peps_set_sweep_kernel": [_vp, C.c_int],

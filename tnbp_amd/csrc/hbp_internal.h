@@ -108,7 +108,12 @@ struct Tensor {
   size_t size = 1;
   void *data = nullptr;
   size_t bytes = 0;
+  // changes on every write through the C-ABI (process-wide unique values), so
+  // that a caller can tell whether a tensor changed since it last looked
+  uint64_t version = 0;
 };
+// a fresh version for a tensor that is being written
+void touch(Tensor *t);
 
 size_t shape_size(int order, const int32_t *shape);
 void tensor_resize(Ctx *ctx, Tensor *t, int dtype, int order, const int32_t *shape);

@@ -71,7 +71,8 @@ __global__ __launch_bounds__(256) void k_finalize_messages(const double2 *__rest
                                                             const double2 *__restrict__ old,
                                                             const MsgJob *__restrict__ jobs,
                                                             double *__restrict__ dist,
-                                                            int phase_gauge) {
+                                                            int phase_gauge,
+                                                            double2 *__restrict__ out2) {
   __shared__ double sh[8];
   const MsgJob j = jobs[blockIdx.x];
   const int n = j.D * j.D;
@@ -117,6 +118,7 @@ __global__ __launch_bounds__(256) void k_finalize_messages(const double2 *__rest
     const double2 v = raw[j.raw_off + e];
     const double2 m = make_double2(v.x * c.x - v.y * c.y, v.x * c.y + v.y * c.x);
     out[j.new_off + e] = m;
+    if (out2) out2[j.new_off + e] = m;
     if (old) {
       const double2 w = old[j.old_off + e];
       const double dx = m.x - (w.x * co.x - w.y * co.y), dy = m.y - (w.x * co.y + w.y * co.x);
@@ -127,6 +129,11 @@ __global__ __launch_bounds__(256) void k_finalize_messages(const double2 *__rest
     d = bsum(d, sh);
     if (threadIdx.x == 0) dist[blockIdx.x] = sqrt(d);
   }
+}
+
+__global__ void k_scatter_dist(const double *__restrict__ d, const int *__restrict__ pos, int n,
+                               double *__restrict__ dist) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) dist[pos[i]] = d[i];
 }
 
 __global__ void k_residual(const double *__restrict__ dist, int n, double vol,
@@ -539,6 +546,9 @@ struct Peps {
   std::vector<int> bond_dim;               // edge -> D
   std::vector<void *> V;                   // address -> device tensor (S precision)
   std::vector<size_t> V_bytes;
+  // site s's memory belongs to a caller's tensor (hbp_peps_bind_sites): never
+  // released or written by the PEPS; replacing the site only drops the flag
+  std::vector<char> V_borrowed;
   double2 *E = nullptr;                    // all 2 n_e messages
   size_t E_bytes = 0;
   std::vector<int64_t> E_off;              // message -> offset (complex<double>)
@@ -568,6 +578,8 @@ struct Peps {
   std::vector<void *> Vpair;
   std::vector<size_t> Vpair_bytes;
   int pair_cache = -1;  // -1 undecided (decided once per set of sites), 0 off, 1 on
+  // caller's choice (hbp_peps_set_pair_cache): -1 by free device memory, 0 never, 1 always
+  int pair_cache_policy = -1;
   int mode = HBP_MODE_FUSED;
   // mixed precision: BP stops once the residual is below this (FP32 messages
   // cannot reach the FP64 tolerances), or below the caller's tolerance
@@ -579,6 +591,13 @@ struct Peps {
   // without the fix the residual stalls near 4/pi once it is below ~1e-10 on
   // large lattices.  Expectation values and projectors do not depend on it.
   bool phase_gauge = true;
+  // BP update schedule of the fused mode: JACOBI computes every message from
+  // the previous sweep's (bp.h:65-261 with E <- F after all layers);
+  // CHECKERBOARD (bipartite lattices) first updates the messages leaving the
+  // sites of colour 0, then those leaving colour 1 from the new ones
+  int schedule = HBP_SCHEDULE_JACOBI;
+  std::vector<char> colour;  // site -> 0 / 1; empty until computed, or not bipartite
+  int bipartite = -1;        // -1 unknown, 0 no, 1 yes
 
   int elem_bytes() const { return precision == HBP_PREC_FP64 ? 16 : 8; }
   int site_dtype() const { return precision == HBP_PREC_FP64 ? HBP_C128 : HBP_C64; }
@@ -654,13 +673,22 @@ static const void *site_data(Ctx *ctx, Peps *p, int s) {
   return p->abs_buf;
 }
 
+// Give up site s's current memory (a borrowed site is only forgotten).
+static void release_site(Ctx *ctx, Peps *p, int s) {
+  if (!p->V_borrowed.empty() && p->V_borrowed[s]) {
+    p->V_borrowed[s] = 0;
+    return;
+  }
+  ctx->release(p->V[s], p->V_bytes[s]);
+}
+
 // Store the absorbed form of site s as its tensor and drop the operator.
 static void make_resident(Ctx *ctx, Peps *p, int s) {
   if (!p->pending[s]) return;
   const size_t bytes = p->site_size(s) * p->elem_bytes();
   void *nv = ctx->alloc(bytes);
   materialize(ctx, p, s, nv);
-  ctx->release(p->V[s], p->V_bytes[s]);
+  release_site(ctx, p, s);
   p->V[s] = nv;
   p->V_bytes[s] = bytes;
   ctx->release(p->pend_O[s], p->pend_O_bytes[s]);
@@ -683,7 +711,7 @@ static void drop_pair_cache(Ctx *ctx, Peps *p) {
 // any pending operator.
 static void set_resident(Ctx *ctx, Peps *p, int s, void *nv, size_t bytes) {
   drop_pair_cache(ctx, p);
-  ctx->release(p->V[s], p->V_bytes[s]);
+  release_site(ctx, p, s);
   p->V[s] = nv;
   p->V_bytes[s] = bytes;
   if (p->pending[s]) {
@@ -994,8 +1022,12 @@ static void plan_pair_cache(Ctx *ctx, Peps *p) {
     const char *e = getenv("HBP_PAIR_CACHE");
     return !(e && e[0] == '0');
   }();
-  if (!allowed) {
+  if (!allowed || p->pair_cache_policy == 0) {
     p->pair_cache = 0;
+    return;
+  }
+  if (p->pair_cache_policy == 1) {
+    p->pair_cache = 1;
     return;
   }
   int32_t sh[HBP_MAX_ORDER];
@@ -1073,7 +1105,8 @@ static void pair_messages_diag(Ctx *ctx, Peps *p, int s, const int32_t *sh, int 
 }
 
 template <typename S>
-static void sweep_raw(Ctx *ctx, Peps *p, double2 *raw, const std::vector<int64_t> &raw_off) {
+static void sweep_raw(Ctx *ctx, Peps *p, double2 *raw, const std::vector<int64_t> &raw_off,
+                      const std::vector<char> *only = nullptr, char colour = 0) {
   size_t maxb = 16;
   for (int s = 0; s < p->n_sites; ++s) maxb = std::max(maxb, p->site_size(s) * p->elem_bytes());
   ensure_work(ctx, p, maxb);
@@ -1087,6 +1120,7 @@ static void sweep_raw(Ctx *ctx, Peps *p, double2 *raw, const std::vector<int64_t
   Reducer *red = batch ? &reducer : nullptr;
   int32_t sh[HBP_MAX_ORDER];
   for (int s = 0; s < p->n_sites; ++s) {
+    if (only && (*only)[s] != colour) continue;
     const int z = p->z(s), order = z + 1;
     p->shape(s, sh);
     if (diag_env(p, s, -1)) {
@@ -1477,9 +1511,109 @@ static void sweep_raw_plain(Ctx *ctx, Peps *p, double2 *raw, const std::vector<i
   ctx->release(dv, sizeof(double));
 }
 
+// Two-colouring of the lattice graph (BFS); false when it has an odd cycle.
+static bool colour_sites(Peps *p) {
+  if (p->bipartite >= 0) return p->bipartite == 1;
+  std::vector<std::vector<int>> nb(p->n_sites);
+  for (int m = 0; m < p->n_edges; ++m) {
+    nb[p->ea[m]].push_back(p->eb[m]);
+    nb[p->eb[m]].push_back(p->ea[m]);
+  }
+  std::vector<int> c(p->n_sites, -1);
+  bool ok = true;
+  for (int s0 = 0; s0 < p->n_sites && ok; ++s0) {
+    if (c[s0] >= 0) continue;
+    c[s0] = 0;
+    std::vector<int> q{s0};
+    for (size_t i = 0; i < q.size() && ok; ++i)
+      for (int t : nb[q[i]]) {
+        if (c[t] < 0) {
+          c[t] = 1 - c[q[i]];
+          q.push_back(t);
+        } else if (c[t] == c[q[i]]) {
+          ok = false;
+        }
+      }
+  }
+  p->bipartite = ok ? 1 : 0;
+  if (ok) p->colour.assign(c.begin(), c.end());
+  return ok;
+}
+
+// One checkerboard BP sweep: the messages leaving colour-0 sites from E, then
+// those leaving colour-1 sites from E with the colour-0 ones already replaced.
+// On a bipartite lattice a message leaving a colour-c site depends only on
+// messages leaving colour 1 - c sites, so the Jacobi sweep advances two
+// independent sequences by one step each while this sweep advances one of
+// them by two: the fixed points are the same (both schedules stop on a
+// message set that one more update leaves within the tolerance).  newE gets
+// the new messages; the residual compares them with E like the Jacobi form.
+template <typename S>
+static void sweep_checkerboard(Ctx *ctx, Peps *p, double2 *raw, double2 *newE, double *dist,
+                               bool with_residual) {
+  const int nm = 2 * p->n_edges;
+  double2 *E_old = p->E;
+  double2 *mix = (double2 *)ctx->alloc(std::max<size_t>(p->E_bytes, 16));
+  HBP_HIP(hipMemcpyAsync(mix, E_old, p->E_bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  const std::vector<char> diag_saved = p->E_diag;
+  for (int c = 0; c < 2; ++c) {
+    std::vector<MsgJob> jobs;
+    std::vector<int> ms;
+    for (int m = 0; m < nm; ++m) {
+      const int src = m < p->n_edges ? p->ea[m] : p->eb[m - p->n_edges];  // E[m] leaves ea
+      if (p->colour[src] == c) ms.push_back(m);
+    }
+    if (ms.empty()) continue;
+    // colour 1 reads the colour-0 updates (dense from here on) through p->E
+    if (c == 1) p->E = mix;
+    sweep_raw<S>(ctx, p, raw, p->E_off, &p->colour, (char)c);
+    p->E = E_old;
+    for (int m : ms) jobs.push_back(MsgJob{p->E_off[m], p->E_off[m], p->E_off[m], p->msg_dim(m)});
+    MsgJob *dj = (MsgJob *)ctx->alloc(jobs.size() * sizeof(MsgJob));
+    ctx->upload(dj, jobs.data(), jobs.size() * sizeof(MsgJob));
+    double *d = (double *)ctx->alloc(jobs.size() * sizeof(double));
+    hipLaunchKernelGGL(k_finalize_messages, dim3((unsigned)jobs.size()), dim3(256), 0,
+                       ctx->stream, raw, newE, with_residual ? E_old : nullptr, dj, d,
+                       p->phase_gauge ? 1 : 0, c == 0 ? mix : nullptr);
+    HBP_HIP(hipGetLastError());
+    if (with_residual) {
+      // scatter the colour's distances to their message positions (tiny)
+      std::vector<int> pos(ms.begin(), ms.end());
+      int *dpos = (int *)ctx->alloc(pos.size() * sizeof(int));
+      ctx->upload(dpos, pos.data(), pos.size() * sizeof(int));
+      hipLaunchKernelGGL(k_scatter_dist, dim3(1), dim3(256), 0, ctx->stream, d, dpos,
+                         (int)pos.size(), dist);
+      HBP_HIP(hipGetLastError());
+      ctx->release(dpos, pos.size() * sizeof(int));
+    }
+    ctx->release(d, jobs.size() * sizeof(double));
+    ctx->release(dj, jobs.size() * sizeof(MsgJob));
+    if (c == 0)
+      for (int m : ms) p->E_diag[m] = 0;
+  }
+  p->E_diag = diag_saved;
+  ctx->release(mix, std::max<size_t>(p->E_bytes, 16));
+}
+
 // one BP sweep: newE = normalise(BP(E)); returns residual on device if dist != null
 static void sweep(Ctx *ctx, Peps *p, double2 *newE, bool with_residual, double *dev_res) {
   const int nm = 2 * p->n_edges;
+  if (p->mode == HBP_MODE_FUSED && p->schedule == HBP_SCHEDULE_CHECKERBOARD && colour_sites(p)) {
+    double2 *raw = (double2 *)ctx->alloc(std::max<size_t>(p->E_bytes, 16));
+    double *dist = (double *)ctx->alloc(nm * sizeof(double));
+    if (p->precision == HBP_PREC_FP64)
+      sweep_checkerboard<double>(ctx, p, raw, newE, dist, with_residual);
+    else
+      sweep_checkerboard<float>(ctx, p, raw, newE, dist, with_residual);
+    if (with_residual) {
+      hipLaunchKernelGGL(k_residual, dim3(1), dim3(256), 0, ctx->stream, dist, nm,
+                         1.0 / (2.0 * p->n_edges), dev_res);
+      HBP_HIP(hipGetLastError());
+    }
+    ctx->release(dist, nm * sizeof(double));
+    ctx->release(raw, std::max<size_t>(p->E_bytes, 16));
+    return;
+  }
   // job table first: its upload waits on the stream, which is then still short
   std::vector<MsgJob> jobs(nm);
   for (int m = 0; m < nm; ++m) jobs[m] = MsgJob{p->E_off[m], p->E_off[m], p->E_off[m], p->msg_dim(m)};
@@ -1513,7 +1647,7 @@ static void sweep(Ctx *ctx, Peps *p, double2 *newE, bool with_residual, double *
   double *dist = (double *)ctx->alloc(nm * sizeof(double));
   hipLaunchKernelGGL(k_finalize_messages, dim3(nm), dim3(256), 0, ctx->stream, raw, newE,
                      with_residual ? p->E : nullptr, dj, dist,
-                     p->mode == HBP_MODE_FUSED && p->phase_gauge ? 1 : 0);
+                     p->mode == HBP_MODE_FUSED && p->phase_gauge ? 1 : 0, nullptr);
   HBP_HIP(hipGetLastError());
   if (with_residual) {
     hipLaunchKernelGGL(k_residual, dim3(1), dim3(256), 0, ctx->stream, dist, nm,
@@ -2031,6 +2165,7 @@ hbp_status hbp_peps_create(hbp_ctx *ctx, int n_edges, const int32_t *edges,
   // |0>: bonds of dimension 1, V[0] = 1 (init.h:57-60)
   p->V.resize(p->n_sites);
   p->V_bytes.resize(p->n_sites);
+  p->V_borrowed.assign(p->n_sites, 0);
   p->pending.assign(p->n_sites, 0);
   p->pend_O.assign(p->n_sites, nullptr);
   p->pend_O_bytes.assign(p->n_sites, 0);
@@ -2055,7 +2190,7 @@ hbp_status hbp_peps_destroy(hbp_ctx *ctx, hbp_peps *p) {
   API_BEGIN
   if (!p) return HBP_OK;
   for (int s = 0; s < p->n_sites; ++s) {
-    ctx->release(p->V[s], p->V_bytes[s]);
+    release_site(ctx, p, s);
     if (p->pend_O[s]) ctx->release(p->pend_O[s], p->pend_O_bytes[s]);
   }
   if (p->abs_buf) ctx->release(p->abs_buf, p->abs_bytes);
@@ -2189,6 +2324,132 @@ hbp_status hbp_peps_set_message(hbp_ctx *ctx, hbp_peps *p, int m, int32_t dim,
   API_END
 }
 
+hbp_status hbp_peps_bind_sites(hbp_ctx *ctx, hbp_peps *p, const hbp_tensor *const *sites) {
+  API_BEGIN
+  HBP_REQUIRE(sites, HBP_ERR_INVALID_ARGUMENT, "bind_sites: null");
+  const int ns = p->n_sites, ne = p->n_edges;
+  std::vector<int> dims(ne, 0);
+  for (int s = 0; s < ns; ++s) {
+    const hbp_tensor *t = sites[s];
+    HBP_REQUIRE(t && t->data, HBP_ERR_INVALID_ARGUMENT, "bind_sites: empty site tensor");
+    HBP_REQUIRE(t->dtype == p->site_dtype(), HBP_ERR_INVALID_ARGUMENT,
+                "bind_sites: element type differs from the PEPS precision's");
+    HBP_REQUIRE(t->order == p->z(s) + 1 && t->shape[p->z(s)] == p->phys[s],
+                HBP_ERR_INVALID_ARGUMENT, "bind_sites: site order or physical dim");
+    for (int k = 0; k < p->z(s); ++k) {
+      const int m = p->bonds[s][k];
+      HBP_REQUIRE(dims[m] == 0 || dims[m] == t->shape[k], HBP_ERR_INVALID_ARGUMENT,
+                  "bind_sites: bond dims differ at the two ends of an edge");
+      dims[m] = t->shape[k];
+    }
+  }
+  drop_pair_cache(ctx, p);
+  p->abs_site = -1;
+  for (int s = 0; s < ns; ++s) {
+    if (p->pending[s]) {
+      ctx->release(p->pend_O[s], p->pend_O_bytes[s]);
+      p->pend_O[s] = nullptr;
+      p->pending[s] = 0;
+    }
+    if (p->V_borrowed[s] && p->V[s] == sites[s]->data) continue;
+    release_site(ctx, p, s);
+    p->V[s] = sites[s]->data;
+    p->V_bytes[s] = sites[s]->bytes;
+    p->V_borrowed[s] = 1;
+  }
+  if (dims != p->bond_dim) {
+    p->bond_dim = dims;
+    ctx->release(p->E, p->E_bytes);
+    layout_messages(ctx, p, &p->E, &p->E_bytes, &p->E_off);
+    launch_fill(ctx->stream, p->E, p->E_bytes / 16, HBP_C128, 0.0, 0.0);
+  }
+  p->E_diag.assign(2 * ne, 0);
+  API_END
+}
+
+hbp_status hbp_peps_load_messages(hbp_ctx *ctx, hbp_peps *p, const hbp_tensor *const *msgs) {
+  API_BEGIN
+  HBP_REQUIRE(msgs, HBP_ERR_INVALID_ARGUMENT, "load_messages: null");
+  for (int m = 0; m < 2 * p->n_edges; ++m) {
+    const hbp_tensor *t = msgs[m];
+    const int D = p->msg_dim(m);
+    HBP_REQUIRE(t && t->data && t->dtype == HBP_C128 && t->order == 2 && t->shape[0] == D &&
+                    t->shape[1] == D,
+                HBP_ERR_INVALID_ARGUMENT,
+                "load_messages: message " + std::to_string(m) + " is not a complex128 " +
+                    std::to_string(D) + " x " + std::to_string(D) + " matrix");
+  }
+  for (int m = 0; m < 2 * p->n_edges; ++m) {
+    const size_t D = (size_t)p->msg_dim(m);
+    HBP_HIP(hipMemcpyAsync(p->E + p->E_off[m], msgs[m]->data, D * D * 16,
+                           hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  // dense unless known otherwise (the diagonal fast paths only save time)
+  p->E_diag.assign(2 * p->n_edges, 0);
+  API_END
+}
+
+hbp_status hbp_peps_store_messages(hbp_ctx *ctx, hbp_peps *p, int n, const int32_t *which,
+                                   hbp_tensor *const *out) {
+  API_BEGIN
+  for (int i = 0; i < n; ++i) {
+    const int m = which ? which[i] : i;
+    HBP_REQUIRE(m >= 0 && m < 2 * p->n_edges, HBP_ERR_OUT_OF_RANGE, "store_messages: index");
+    HBP_REQUIRE(out[i], HBP_ERR_INVALID_ARGUMENT, "store_messages: null tensor");
+    const int32_t D = p->msg_dim(m);
+    const int32_t sh[2] = {D, D};
+    tensor_resize(ctx, out[i], HBP_C128, 2, sh);
+    HBP_HIP(hipMemcpyAsync(out[i]->data, p->E + p->E_off[m], (size_t)D * D * 16,
+                           hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  API_END
+}
+
+hbp_status hbp_peps_store_sites(hbp_ctx *ctx, hbp_peps *p, hbp_tensor *const *out) {
+  API_BEGIN
+  HBP_REQUIRE(out, HBP_ERR_INVALID_ARGUMENT, "store_sites: null");
+  drop_pair_cache(ctx, p);
+  for (int s = 0; s < p->n_sites; ++s) {
+    hbp_tensor *t = out[s];
+    HBP_REQUIRE(t, HBP_ERR_INVALID_ARGUMENT, "store_sites: null tensor");
+    if (p->V_borrowed[s] && !p->pending[s] && p->V[s] == t->data) continue;
+    int32_t sh[HBP_MAX_ORDER];
+    p->shape(s, sh);
+    const size_t n = p->site_size(s), bytes = n * 16;
+    // the new contents go to fresh memory first: the old tensor may still be
+    // this site's (borrowed) source
+    void *nv = ctx->alloc(bytes);
+    if (p->precision == HBP_PREC_FP64) {
+      if (p->pending[s]) materialize(ctx, p, s, nv);
+      else HBP_HIP(hipMemcpyAsync(nv, p->V[s], bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    } else {
+      launch_convert(ctx->stream, site_data(ctx, p, s), p->site_dtype(), nv, HBP_C128, n);
+    }
+    if (t->data) ctx->release(t->data, t->bytes);
+    t->data = nv;
+    t->bytes = bytes;
+    t->dtype = HBP_C128;
+    t->order = p->z(s) + 1;
+    t->size = n;
+    for (int d = 0; d < HBP_MAX_ORDER; ++d) t->shape[d] = d < t->order ? sh[d] : 0;
+    touch(t);
+    if (p->precision == HBP_PREC_FP64) {
+      // the PEPS now reads the caller's tensor
+      if (p->pending[s]) {
+        ctx->release(p->pend_O[s], p->pend_O_bytes[s]);
+        p->pend_O[s] = nullptr;
+        p->pending[s] = 0;
+      }
+      release_site(ctx, p, s);
+      p->V[s] = nv;
+      p->V_bytes[s] = t->bytes;
+      p->V_borrowed[s] = 1;
+    }
+  }
+  p->abs_site = -1;
+  API_END
+}
+
 hbp_status hbp_peps_absorb_tpo(hbp_ctx *ctx, hbp_peps *p, const double *const *ops,
                                const int32_t *tpo_shapes) {
   API_BEGIN
@@ -2297,7 +2558,7 @@ hbp_status hbp_peps_absorb_tpo(hbp_ctx *ctx, hbp_peps *p, const double *const *o
       hipLaunchKernelGGL(k_absorb_site<float2>, dim3(gridn(n)), dim3(256), 0, ctx->stream,
                          (const float2 *)p->V[s], dO, (float2 *)nv, n, a);
     HBP_HIP(hipGetLastError());
-    ctx->release(p->V[s], p->V_bytes[s]);
+    release_site(ctx, p, s);
     p->V[s] = nv;
     p->V_bytes[s] = bytes;
   }
@@ -2358,6 +2619,22 @@ hbp_status hbp_peps_set_absorb_mode(hbp_peps *p, int mode) {
 hbp_status hbp_peps_set_phase_gauge(hbp_peps *p, int on) {
   API_BEGIN
   p->phase_gauge = on != 0;
+  API_END
+}
+
+hbp_status hbp_peps_set_pair_cache(hbp_peps *p, int policy) {
+  API_BEGIN
+  HBP_REQUIRE(policy >= -1 && policy <= 1, HBP_ERR_INVALID_ARGUMENT, "peps: pair cache policy");
+  p->pair_cache_policy = policy;
+  p->pair_cache = -1;
+  API_END
+}
+
+hbp_status hbp_peps_set_schedule(hbp_peps *p, int schedule) {
+  API_BEGIN
+  HBP_REQUIRE(schedule == HBP_SCHEDULE_JACOBI || schedule == HBP_SCHEDULE_CHECKERBOARD,
+              HBP_ERR_INVALID_ARGUMENT, "peps: unknown BP schedule");
+  p->schedule = schedule;
   API_END
 }
 

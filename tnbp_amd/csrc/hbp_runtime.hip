@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -72,6 +73,11 @@ void bind_noalloc_scratch(int device) {
 namespace hbp {
 
 void set_last_error(const std::string &s) { g_last_error = s; }
+
+void touch(Tensor *t) {
+  static std::atomic<uint64_t> counter{0};
+  t->version = ++counter;
+}
 
 size_t shape_size(int order, const int32_t *shape) {
   size_t n = 1;
@@ -247,6 +253,7 @@ void tensor_resize(Ctx *ctx, Tensor *t, int dtype, int order, const int32_t *sha
   t->order = order;
   for (int d = 0; d < HBP_MAX_ORDER; ++d) t->shape[d] = d < order ? shape[d] : 0;
   t->size = n;
+  touch(t);
 }
 
 // ------------------------------------------------------------ reductions
@@ -791,6 +798,7 @@ hbp_status hbp_tensor_upload(hbp_ctx *ctx, hbp_tensor *t, const void *host) {
   HBP_HIP(hipMemcpyAsync(t->data, host, t->size * dtype_size(t->dtype), hipMemcpyHostToDevice,
                          ctx->stream));
   ctx->sync();
+  touch(t);
   HBP_API_END
 }
 
@@ -819,6 +827,7 @@ hbp_status hbp_tensor_reshape(hbp_ctx *ctx, hbp_tensor *t, int order, const int3
               "reshape must keep the number of elements");
   t->order = order;
   for (int d = 0; d < HBP_MAX_ORDER; ++d) t->shape[d] = d < order ? shape[d] : 0;
+  touch(t);
   HBP_API_END
 }
 
@@ -844,18 +853,21 @@ hbp_status hbp_transpose(hbp_ctx *ctx, const hbp_tensor *in, const int32_t *new_
   out->size = in->size;
   out->order = in->order;
   for (int d = 0; d < HBP_MAX_ORDER; ++d) out->shape[d] = d < in->order ? sh[d] : 0;
+  touch(out);
   HBP_API_END
 }
 
 hbp_status hbp_conj(hbp_ctx *ctx, hbp_tensor *t) {
   HBP_API_BEGIN
   launch_conj(ctx->stream, t->data, t->size, t->dtype);
+  touch(t);
   HBP_API_END
 }
 
 hbp_status hbp_fill(hbp_ctx *ctx, hbp_tensor *t, double re, double im) {
   HBP_API_BEGIN
   launch_fill(ctx->stream, t->data, t->size, t->dtype, re, im);
+  touch(t);
   HBP_API_END
 }
 
@@ -864,6 +876,7 @@ hbp_status hbp_contract(hbp_ctx *ctx, const hbp_tensor *a, const int32_t *la,
                         const int32_t *lc, int nc) {
   HBP_API_BEGIN
   contract_impl(ctx, a, la, b, lb, c, lc, nc);
+  touch(c);
   HBP_API_END
 }
 
@@ -885,12 +898,14 @@ hbp_status hbp_normalize(hbp_ctx *ctx, hbp_tensor *t, double *norm_out) {
     *norm_out = std::sqrt(h);
   }
   ctx->release(dv, sizeof(double));
+  touch(t);
   HBP_API_END
 }
 
 hbp_status hbp_scale(hbp_ctx *ctx, hbp_tensor *t, double re, double im) {
   HBP_API_BEGIN
   launch_scale(ctx->stream, t->data, t->size, t->dtype, re, im, nullptr);
+  touch(t);
   HBP_API_END
 }
 
@@ -919,6 +934,7 @@ hbp_status hbp_linear_combine(hbp_ctx *ctx, int n, const hbp_tensor *const *ins,
   out->order = order;
   out->size = ins[0]->size;
   for (int d = 0; d < HBP_MAX_ORDER; ++d) out->shape[d] = d < order ? sh[d] : 0;
+  touch(out);
   HBP_API_END
 }
 
@@ -948,6 +964,7 @@ hbp_status hbp_trace(hbp_ctx *ctx, const hbp_tensor *in, int npairs, const int32
   out->order = order;
   out->size = n;
   for (int d = 0; d < HBP_MAX_ORDER; ++d) out->shape[d] = d < order ? sh[d] : 0;
+  touch(out);
   HBP_API_END
 }
 
@@ -974,6 +991,7 @@ hbp_status hbp_diag(hbp_ctx *ctx, const hbp_tensor *vec, hbp_tensor *out) {
   out->order = 2;
   out->size = (size_t)n * n;
   for (int d = 0; d < HBP_MAX_ORDER; ++d) out->shape[d] = d < 2 ? sh[d] : 0;
+  touch(out);
   HBP_API_END
 }
 
@@ -1050,6 +1068,13 @@ hbp_status hbp_tensor_data(const hbp_tensor *t, void **ptr) {
   HBP_API_BEGIN
   HBP_REQUIRE(t && ptr, HBP_ERR_INVALID_ARGUMENT, "tensor_data: null argument");
   *ptr = t->data;
+  HBP_API_END
+}
+
+hbp_status hbp_tensor_version(const hbp_tensor *t, uint64_t *version) {
+  HBP_API_BEGIN
+  HBP_REQUIRE(t && version, HBP_ERR_INVALID_ARGUMENT, "tensor_version: null argument");
+  *version = t->version;
   HBP_API_END
 }
 

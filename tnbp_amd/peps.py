@@ -14,14 +14,16 @@ import numpy as np
 
 from . import _lib
 from ._lib import (HBP_ABSORB_AUTO, HBP_ABSORB_EAGER, HBP_ABSORB_LAZY, HBP_MAX_ORDER,
-                   HBP_MODE_FUSED, HBP_MODE_PLAIN,
+                   HBP_MODE_FUSED, HBP_MODE_PLAIN, HBP_SCHEDULE_CHECKERBOARD,
+                   HBP_SCHEDULE_JACOBI,
                    HBP_PREC_FP64, HBP_PREC_MIXED, HBP_SWEEP_AUTO, HBP_SWEEP_GROUPED,
                    HBP_SWEEP_MERGED, HBP_SWEEP_PER_SITE, check, i32)
 from .tci import Context
 
 __all__ = ["Peps", "InitTensorProductState", "AbsorbTPO", "BeliefPropagation",
            "BeliefPropagationCondition", "bp_loop", "Truncation", "Measure", "MeasureEdges",
-           "HBP_PREC_FP64", "HBP_PREC_MIXED", "HBP_MODE_FUSED", "HBP_MODE_PLAIN"]
+           "HBP_PREC_FP64", "HBP_PREC_MIXED", "HBP_MODE_FUSED", "HBP_MODE_PLAIN",
+           "HBP_SCHEDULE_JACOBI", "HBP_SCHEDULE_CHECKERBOARD"]
 
 
 class Peps:
@@ -72,6 +74,18 @@ class Peps:
         """HBP_SWEEP_AUTO (0), HBP_SWEEP_PER_SITE (1), HBP_SWEEP_GROUPED (2) or
         HBP_SWEEP_MERGED (3)."""
         check(self.ctx.lib.hbp_peps_set_sweep_kernel(self.h, int(kind)))
+
+    def set_schedule(self, schedule):
+        """HBP_SCHEDULE_JACOBI (0, the reference's) or HBP_SCHEDULE_CHECKERBOARD (1)."""
+        check(self.ctx.lib.hbp_peps_set_schedule(self.h, int(schedule)))
+
+    def set_pair_cache(self, policy):
+        """-1 by free memory (default), 0 never, 1 always (memory budgeted by the caller)."""
+        check(self.ctx.lib.hbp_peps_set_pair_cache(self.h, int(policy)))
+
+    def set_residual_floor(self, floor):
+        """Mixed precision: BP also stops once the residual is below this."""
+        check(self.ctx.lib.hbp_peps_set_residual_floor(self.h, float(floor)))
 
     def set_phase_gauge(self, on):
         check(self.ctx.lib.hbp_peps_set_phase_gauge(self.h, int(bool(on))))
