@@ -319,6 +319,21 @@ hbp_status hbp_peps_bp_condition(hbp_ctx *ctx, hbp_peps *p, double *residual);
  * residuals (may be NULL) receives max_iter values; iters the count run. */
 hbp_status hbp_peps_bp_run(hbp_ctx *ctx, hbp_peps *p, int max_iter, double tol, int *iters,
                            double *residuals);
+/* hbp_peps_bp_run for n PEPS of one context at once (no reference
+ * counterpart; each PEPS gets the iterations, residuals and state that
+ * hbp_peps_bp_run gives it alone).  PEPS in fused mode with the Jacobi
+ * schedule, site degree <= 4 and every bond <= 16 iterate in lockstep: each
+ * iteration is one persistent launch over every site of every such PEPS of
+ * a precision (one workgroup per site and stride), a finalisation and a
+ * residual launch, captured in a HIP graph after the first iteration and
+ * replayed (launch grid picked by the autotuner); a PEPS leaves the batch
+ * when it converges or at max_iter.  The others run hbp_peps_bp_run one
+ * after another.  residuals (may be NULL): n rows of max_iter values;
+ * iters, converged, batched (each may be NULL): n values, batched[i] = 1
+ * where PEPS i ran in the batch. */
+hbp_status hbp_peps_bp_run_batched(hbp_ctx *ctx, int n, hbp_peps *const *ps, int max_iter,
+                                   double tol, int *iters, double *residuals, int *converged,
+                                   int *batched);
 /* The same loop in pieces, for callers that interleave work or report
  * progress: up to n iterations; `first` starts a loop (main.cc:121), `last`
  * marks the piece that ends at max_bp_iterations.  converged = 1 once the

@@ -1,0 +1,190 @@
+"""BP loops of several PEPS in one batch (hbp_peps_bp_run_batched): every
+site of every PEPS with bonds <= 16 in one persistent launch per iteration,
+replayed from a HIP graph.
+
+* Config 1 (4x4 kicked Ising, chi = 4) with its BP loops run through the
+  batch: iteration counts and <X>, <Z>, <ZZ> equal the golden fixture (1e-10
+  relative, floor 1).
+* A batch of differently shaped lattices (3x3, 4x4, a chain, a tree) with
+  random circuits: each PEPS gets the iteration count of bp_loop alone and
+  residuals, messages and <Z> within 1e-12 (FP64; different kernels, so not
+  bitwise); a PEPS run in the batch alone equals itself run in the mix bit
+  for bit; in mixed precision within 1e-5 of FP64.
+* PEPS the batch does not take (a bond above 16, the checkerboard schedule)
+  run on their own and still get bp_loop's results.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from gpu_util import announce, rel_close
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from tnbp_amd.tci import Context
+    c = Context(0)
+    yield c
+    c.close()
+
+
+def test_config1_through_the_batch_matches_golden(ctx):
+    from tnbp_amd import circuits as pc
+    from tnbp_amd.peps import (AbsorbTPO, InitTensorProductState, Measure, MeasureEdges, Truncation,
+                               bp_loop_batched)
+    g = json.load(open(os.path.join(GOLDEN, "config1_kicked_4x4_chi4.json")))
+    a = g["args"]
+    announce("config 1 through the batched BP loop")
+    edges = pc.square_lattice(4, 4)
+    tpo = pc.kicked_ising_tpo(edges, a["Jz"], a["hz"], a["hx"], a["dt"] * math.pi)
+    p = InitTensorProductState(ctx, edges)
+    for step, want in enumerate(g["steps"]):
+        AbsorbTPO(ctx, tpo, p)
+        (it, res, conv, bat), = bp_loop_batched(ctx, [p], a["max_bp_iterations"], a["bp_tolerance"])
+        assert bat and it == want["bp_iterations"], (step, it, res, want["residuals"])
+        dims, _ = Truncation(ctx, p, a["max_bond_dim"], a["sv_min"])
+        assert dims == want["bond_dims"]
+        z = [v.real for v in Measure(ctx, p, list(range(16)), [pc.op_tensor(pc.PAULI["Z"])] * 16)]
+        x = [v.real for v in Measure(ctx, p, list(range(16)), [pc.op_tensor(pc.PAULI["X"])] * 16)]
+        zz = [v.real for v in MeasureEdges(ctx, p, edges, [pc.op_tensor(np.diag([1.0, -1, -1, 1]))] * len(edges))]
+        assert rel_close(z, want["z"], 1e-10) <= 1.0
+        assert rel_close(x, want["x"], 1e-10) <= 1.0
+        assert rel_close(zz, want["zz"], 1e-10) <= 1.0
+    p.close()
+
+
+def _lattices():
+    from tnbp_amd import circuits as pc
+    return [pc.square_lattice(3, 3), pc.square_lattice(4, 4), pc.chain(6),
+            [(0, 1), (1, 2), (1, 3), (3, 4), (5, 4), (4, 6)]]
+
+
+def _layer(rng, edges, n):
+    """RZZ(theta ~ U[0.3, 1.3]) on every edge, then random U3 on every site:
+    loops of the lattice carry correlations, so BP needs several iterations."""
+    from tnbp_amd import circuits as pc
+    gates = [(pc.rzz(rng.uniform(0.3, 1.3)), e) for e in edges]
+    gates += [(pc.u3(*rng.uniform(0, 2 * math.pi, 3)), (s,)) for s in range(n)]
+    return gates
+
+
+def _prepared(ctx, precision, seed=17, layers=6, chi=8):
+    """PEPS with random circuits absorbed (bonds <= 16), one BP-ready state each.
+    The first layer leaves |0...0> a product state (RZZ only adds a phase), the
+    next ones double the bonds: 2, 4, 8, then 16 cut to 8 (a lossy truncation,
+    after which BP on the loopy lattices needs several iterations), and the
+    last layer absorbs to bonds of 16."""
+    from tnbp_amd import circuits as pc
+    from tnbp_amd.peps import AbsorbTPO, InitTensorProductState, Truncation, bp_loop
+    rng = np.random.default_rng(seed)
+    out = []
+    for edges in _lattices():
+        n = len({s for e in edges for s in e})
+        p = InitTensorProductState(ctx, edges, [2] * n, precision)
+        tpos = [pc.layer_tpo(edges, _layer(rng, edges, n)) for _ in range(layers)]
+        for t in tpos[:-1]:
+            AbsorbTPO(ctx, t, p)
+            bp_loop(ctx, p, 100, 1e-12)
+            Truncation(ctx, p, chi, 1e-12)
+        AbsorbTPO(ctx, tpos[-1], p)
+        out.append((edges, p))
+    return out
+
+
+def _state(p):
+    return [p.get_message(m) for m in range(2 * p.n_edges)]
+
+
+def test_batch_equals_each_alone(ctx):
+    from tnbp_amd import circuits as pc
+    from tnbp_amd.peps import Measure, bp_loop, bp_loop_batched
+    announce("batch of four lattices vs bp_loop alone, FP64")
+    tol = 1e-11
+    ref = []
+    for edges, p in _prepared(ctx, 0):
+        it, res = bp_loop(ctx, p, 200, tol)
+        n = len({s for e in edges for s in e})
+        assert max(max(p.site_shape(x)[:-1]) for x in range(n)) <= 16
+        ref.append((it, res, _state(p), Measure(ctx, p, list(range(n)), [pc.op_tensor(pc.PAULI["Z"])] * n)))
+        p.close()
+    got = []
+    peps = _prepared(ctx, 0)
+    out = bp_loop_batched(ctx, [p for _, p in peps], 200, tol)
+    for (edges, p), (it, res, conv, bat) in zip(peps, out):
+        n = len({s for e in edges for s in e})
+        got.append((it, res, _state(p), Measure(ctx, p, list(range(n)), [pc.op_tensor(pc.PAULI["Z"])] * n), bat))
+    assert max(r[0] for r in ref) >= 5  # loopy lattices: real BP loops
+    assert max(max(p.site_shape(x)[:-1]) for x in range(9) for p in [peps[0][1]]) == 16
+    for (it0, r0, m0, z0), (it1, r1, m1, z1, bat) in zip(ref, got):
+        announce(f"  iterations alone {it0}, in the batch {it1}")
+        assert bat and it0 == it1
+        np.testing.assert_allclose(r1, r0, rtol=1e-8, atol=1e-14)
+        for a, b in zip(m1, m0):
+            np.testing.assert_allclose(a, b, atol=1e-12)
+        assert rel_close(z1, z0, 1e-12) <= 1.0
+    # the first PEPS alone in a batch equals itself in the mix, bit for bit
+    solo = _prepared(ctx, 0)
+    (it, res, conv, bat), = bp_loop_batched(ctx, [solo[0][1]], 200, tol)
+    assert it == got[0][0] and res == got[0][1]
+    for a, b in zip(_state(solo[0][1]), got[0][2]):
+        np.testing.assert_array_equal(a, b)
+    for _, p in peps + solo:
+        p.close()
+
+
+def test_mixed_batch_within_1e5_of_fp64(ctx):
+    from tnbp_amd import circuits as pc
+    from tnbp_amd.peps import Measure, bp_loop_batched
+    announce("mixed-precision batch vs FP64 batch")
+    vals = []
+    for prec in (0, 1):
+        peps = _prepared(ctx, prec)
+        out = bp_loop_batched(ctx, [p for _, p in peps], 200, 1e-12)
+        assert all(b for *_, b in out)
+        v = []
+        for edges, p in peps:
+            n = len({s for e in edges for s in e})
+            v += [x.real for x in Measure(ctx, p, list(range(n)), [pc.op_tensor(pc.PAULI["Z"])] * n)]
+            p.close()
+        vals.append(v)
+    assert rel_close(vals[1], vals[0], 1e-5) <= 1.0
+
+
+def _kicked_big(ctx):
+    """A 3x3 kicked-Ising PEPS absorbed to bonds of 32 (above the batch's 16)."""
+    from tnbp_amd import circuits as pc
+    from tnbp_amd.peps import AbsorbTPO, InitTensorProductState, Truncation, bp_loop
+    edges = pc.square_lattice(3, 3)
+    tpo = pc.kicked_ising_tpo(edges, 0.125, 0.0, 0.11, math.pi)
+    p = InitTensorProductState(ctx, edges)
+    for _ in range(6):
+        AbsorbTPO(ctx, tpo, p)
+        bp_loop(ctx, p, 100, 1e-12)
+        Truncation(ctx, p, 16, 1e-12)
+    AbsorbTPO(ctx, tpo, p)
+    assert max(max(p.site_shape(s)[:-1]) for s in range(9)) > 16
+    return p
+
+
+def test_ineligible_peps_run_on_their_own(ctx):
+    from tnbp_amd.peps import HBP_SCHEDULE_CHECKERBOARD, bp_loop, bp_loop_batched
+    announce("batch with PEPS it does not take")
+    tol = 1e-11
+    big, big_ref = _kicked_big(ctx), _kicked_big(ctx)
+    c = _prepared(ctx, 0, seed=5)
+    c[0][1].set_schedule(HBP_SCHEDULE_CHECKERBOARD)
+    d = _prepared(ctx, 0, seed=5)
+    d[0][1].set_schedule(HBP_SCHEDULE_CHECKERBOARD)
+    out = bp_loop_batched(ctx, [big, c[0][1], c[1][1]], 200, tol)
+    assert [bat for *_, bat in out] == [False, False, True]
+    for (it, res, conv, bat), q in zip(out[:2], [big_ref, d[0][1]]):
+        it0, res0 = bp_loop(ctx, q, 200, tol)
+        assert it == it0 and res == res0
+    for p in [big, big_ref] + [p for _, p in c + d]:
+        p.close()

@@ -71,3 +71,27 @@ def test_mixed_within_1e5_of_fp64():
             assert rel_close(rm["values"], rd["values"], 1e-5) <= 1.0, (jm.name,)
     finally:
         ctx.close()
+
+
+def test_footprint_covers_the_peak_of_a_merged_sweep_job():
+    """footprint_bytes (the admission model) is at least the peak device
+    memory the job's context reports, for a job whose bonds reach 64 (merged
+    sweep with pair-front copies, counted in the footprint) and one at 16."""
+    from tnbp_amd import serving
+    from tnbp_amd.tci import Context
+    rng = np.random.default_rng(31)
+    jobs = [serving.kicked_job(3, 32, 7, 0.7, precision=1, name="kicked-3x3-chi32"),
+            serving.random_circuit_job(rng, 5, 4, 8, 8, w_max=3, precision=1, name="random-5x5-chi8")]
+    assert serving.pair_cache_bytes(jobs[0]) > 0
+    for job in jobs:
+        ctx = Context(0)
+        try:
+            ctx.trim()
+            serving.run_job(ctx, job)
+            _, peak = ctx.memory()
+        finally:
+            ctx.close()
+        fp = serving.footprint_bytes(job)
+        announce(f"footprint {job.name}: model {fp / 2**20:.1f} MiB, peak {peak / 2**20:.1f} MiB, "
+                 f"pair cache {serving.pair_cache_bytes(job) / 2**20:.1f} MiB")
+        assert peak <= fp

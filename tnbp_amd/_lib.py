@@ -91,6 +91,8 @@ SIGNATURES = {
     "hbp_peps_bp_sweep": [_vp, _vp],
     "hbp_peps_bp_condition": [_vp, _vp, _dp],
     "hbp_peps_bp_run": [_vp, _vp, C.c_int, C.c_double, C.POINTER(C.c_int), _dp],
+    "hbp_peps_bp_run_batched": [_vp, C.c_int, C.POINTER(_vp), C.c_int, C.c_double,
+                                C.POINTER(C.c_int), _dp, C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "hbp_peps_bp_iterate": [_vp, _vp, C.c_int, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int), _dp,
                             C.POINTER(C.c_int)],
     "hbp_peps_truncate": [_vp, _vp, C.c_int, C.c_double, _i32p, _dp],

@@ -15,7 +15,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhbp.so")
-SOURCES = ["hbp_kernels.hip", "hbp_runtime.hip", "hbp_peps.hip"]
+SOURCES = ["hbp_kernels.hip", "hbp_runtime.hip", "hbp_peps.hip", "hbp_batched.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

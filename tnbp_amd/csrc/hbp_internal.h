@@ -91,8 +91,16 @@ struct Ctx {
   };
   bool prof = false;
   std::vector<ProfRec> prof_recs;
+  // totals of records already folded (their events destroyed), per kind:
+  // launches, ms, flops, bytes
+  double prof_acc[3][4] = {{0}};
   hipEvent_t prof_event();
   void prof_clear();  // destroys the events of every open record
+  // folds the older half of the open records into prof_acc once more than
+  // kProfMaxLive are open, so that a long profiled run keeps a bounded
+  // number of events (each holds a completion signal and pins its dispatch)
+  static constexpr size_t kProfMaxLive = 4096;
+  void prof_fold();
   // returns the index of the open record, or -1 when profiling is off
   int prof_begin(int kind, double flops, double bytes);
   void prof_end(int idx);
@@ -265,6 +273,35 @@ struct RedJob {
   int nch, n;
   int64_t first_block;
 };
+// Batched small-bond sweep (hbp_batched.hip): one work item per site of a
+// PEPS in the batch; all of the site's outgoing raw messages (unnormalised,
+// complex double, D x D column-major) from its tensor V (bonds then the
+// physical index) and the incoming messages E + in_off[k] (D x D each).
+constexpr int SMALL_D_MAX = 16;
+struct SmallSiteJob {
+  const void *V;
+  const double2 *E;
+  double2 *raw;
+  int64_t in_off[4];
+  int64_t out_off[4];
+  int dims[5];  // bond dims, then the physical dim at index z
+  int z;
+};
+template <typename S>
+void launch_sweep_small(hipStream_t st, const SmallSiteJob *jobs_dev, int njobs, void *scratch,
+                        int64_t slab_elems, int grid);
+// Grids the batched sweep is launched with: GRID_MULT[v] workgroups per CU
+// (variant v), at most one per work item; scratch for the largest.
+constexpr int SMALL_GRID_VARIANTS = 3;
+int small_sweep_grid(int variant, int njobs);
+// The variant for (precision, njobs class, slab class): timed on first use by
+// running every variant on the real work (each computes the same messages;
+// the chosen one runs last), remembered for the process.  Must not run while
+// the stream is being captured.
+template <typename S>
+int launch_sweep_small_tuned(hipStream_t st, const SmallSiteJob *jobs_dev, int njobs,
+                             void *scratch, int64_t slab_elems);
+
 // the autotuner's choices so far, one line per shape class
 std::string autotune_report();
 int mode_product_col_blocks(int Dout);
@@ -305,8 +342,12 @@ struct SvdJob {
   int64_t a_off, u_off, s_off, vd_off, work_off;
   int m, n;
 };
+// Matrices with m n + n^2 <= SVD_LDS_ELEMS (64 x 64 and smaller) run with
+// their working columns in LDS (max_m: the largest m of the batch, 0 when
+// unknown); larger ones in `work` (job.work_off, m n + n^2 elements each).
+constexpr int64_t SVD_LDS_ELEMS = 8192;
 void launch_batched_svd(hipStream_t st, const double2 *A, double2 *U, double *s, double2 *Vd,
-                        double2 *work, const SvdJob *jobs_dev, int batch, int max_n);
+                        double2 *work, const SvdJob *jobs_dev, int batch, int max_n, int max_m);
 // one SvdJob written to device memory by a kernel (no host copy)
 void launch_store_svd_job(hipStream_t st, const SvdJob &job, SvdJob *dst);
 

@@ -1459,17 +1459,24 @@ __device__ inline double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
 // columns (p, q) are rotated until mutually orthogonal; the rotations are
 // accumulated in V so that A V = [sigma_j u_j].  Round-robin pairing puts
 // n'/2 disjoint pairs in each round; each wave takes pairs w, w+4, ...
+// IN_LDS: the working columns W (m x n) and V (n x n) live in the block's
+// LDS (m n + n^2 <= SVD_LDS_ELEMS, i.e. up to 64 x 64): every round is a
+// chain of dependent dot products and rotations, so their latency decides
+// the time; the same operations in the same order as the global-memory form,
+// so the results are bitwise those of the global form.
+template <bool IN_LDS>
 __global__ __launch_bounds__(256) void k_batched_svd(const double2 *__restrict__ Ain,
                                                       double2 *__restrict__ U,
                                                       double *__restrict__ s,
                                                       double2 *__restrict__ Vd,
                                                       double2 *__restrict__ work,
                                                       const SvdJob *__restrict__ jobs) {
+  extern __shared__ double2 svd_lds[];
   const SvdJob job = jobs[blockIdx.x];
   const int m = job.m, n = job.n;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  double2 *W = work + job.work_off;              // m x n
-  double2 *Vm = W + (int64_t)m * n;              // n x n
+  double2 *W = IN_LDS ? svd_lds : work + job.work_off;  // m x n
+  double2 *Vm = W + (int64_t)m * n;                     // n x n
   const double2 *A = Ain + job.a_off;
   for (int64_t e = threadIdx.x; e < (int64_t)m * n; e += blockDim.x) W[e] = A[e];
   for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += blockDim.x) {
@@ -1751,12 +1758,34 @@ void tuned_launch(hipStream_t st, const TuneKey &key, int nv, F run) {
 }
 }  // namespace
 
+int small_sweep_grid(int variant, int njobs) {
+  static const int mult[SMALL_GRID_VARIANTS] = {1, 2, 4};
+  const int g = device_cus() * mult[variant < 0 || variant >= SMALL_GRID_VARIANTS ? 0 : variant];
+  return std::max(1, std::min(g, njobs));
+}
+
+template <typename S>
+int launch_sweep_small_tuned(hipStream_t st, const SmallSiteJob *jobs_dev, int njobs,
+                             void *scratch, int64_t slab_elems) {
+  const TuneKey key{2, (int)sizeof(S), 0, ilog2(std::max(1, njobs)), ilog2(std::max<int64_t>(1, slab_elems))};
+  int chosen = 0;
+  tuned_launch(st, key, SMALL_GRID_VARIANTS, [&](int v) {
+    chosen = v;
+    launch_sweep_small<S>(st, jobs_dev, njobs, scratch, slab_elems, small_sweep_grid(v, njobs));
+  });
+  return chosen;
+}
+template int launch_sweep_small_tuned<double>(hipStream_t, const SmallSiteJob *, int, void *,
+                                              int64_t);
+template int launch_sweep_small_tuned<float>(hipStream_t, const SmallSiteJob *, int, void *,
+                                             int64_t);
+
 std::string autotune_report() {
   std::lock_guard<std::mutex> g(g_tune_mu);
   std::string out;
   for (const auto &kv : g_tune) {
     const TuneKey &k = kv.first;
-    out += std::string(k.kind == 0 ? "mode_product" : "message") + " elem=" +
+    out += std::string(k.kind == 0 ? "mode_product" : k.kind == 1 ? "message" : "small_sweep") + " elem=" +
            std::to_string(k.elem) + " ncb=" + std::to_string(k.ncb) + " log2Q=" +
            std::to_string(k.qlog) + " aux=" + std::to_string(k.aux) + " -> variant " +
            std::to_string(kv.second) + "\n";
@@ -2237,10 +2266,24 @@ void launch_trace(hipStream_t st, const double2 *in, double2 *out, int order,
 }
 
 void launch_batched_svd(hipStream_t st, const double2 *A, double2 *U, double *s, double2 *Vd,
-                        double2 *work, const SvdJob *jobs_dev, int batch, int max_n) {
+                        double2 *work, const SvdJob *jobs_dev, int batch, int max_n, int max_m) {
   HBP_REQUIRE(max_n <= 512, HBP_ERR_UNSUPPORTED, "svd: more than 512 columns");
   if (batch <= 0) return;
-  hipLaunchKernelGGL(k_batched_svd, dim3(batch), dim3(256), 0, st, A, U, s, Vd, work, jobs_dev);
+  const int64_t elems = (int64_t)max_m * max_n + (int64_t)max_n * max_n;
+  if (max_m > 0 && elems <= SVD_LDS_ELEMS) {
+    const unsigned bytes = (unsigned)(elems * sizeof(double2));
+    static std::once_flag once;
+    std::call_once(once, [] {
+      (void)hipFuncSetAttribute((const void *)k_batched_svd<true>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)(SVD_LDS_ELEMS * sizeof(double2)));
+    });
+    hipLaunchKernelGGL(k_batched_svd<true>, dim3(batch), dim3(256), bytes, st, A, U, s, Vd, work,
+                       jobs_dev);
+  } else {
+    hipLaunchKernelGGL(k_batched_svd<false>, dim3(batch), dim3(256), 0, st, A, U, s, Vd, work,
+                       jobs_dev);
+  }
   HBP_HIP(hipGetLastError());
 }
 

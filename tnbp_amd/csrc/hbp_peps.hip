@@ -66,22 +66,21 @@ __device__ inline double2 phase_of_trace(double re, double im) {
 // distance between it and the normalised stored message (bp.h:321-350).
 // phase_gauge != 0 also fixes each message's free scalar phase (trace real
 // and positive) on both sides of the distance.
-__global__ __launch_bounds__(256) void k_finalize_messages(const double2 *__restrict__ raw,
-                                                            double2 *__restrict__ out,
-                                                            const double2 *__restrict__ old,
-                                                            const MsgJob *__restrict__ jobs,
-                                                            double *__restrict__ dist,
-                                                            int phase_gauge,
-                                                            double2 *__restrict__ out2) {
-  __shared__ double sh[8];
-  const MsgJob j = jobs[blockIdx.x];
-  const int n = j.D * j.D;
+// Normalise one raw message (FP64) into out (and out2 when given) and, when
+// old != null, return the Frobenius distance between it and the normalised
+// stored message (bp.h:321-350); phase_gauge != 0 also fixes each message's
+// free scalar phase (trace real and positive) on both sides of the distance.
+// Called by every thread of a block; the result is valid in thread 0.
+__device__ double finalize_one(const double2 *__restrict__ raw, double2 *__restrict__ out,
+                               const double2 *__restrict__ old, int D, int phase_gauge,
+                               double2 *__restrict__ out2, double *sh) {
+  const int n = D * D;
   double s = 0.0, so = 0.0;
   for (int e = threadIdx.x; e < n; e += blockDim.x) {
-    const double2 v = raw[j.raw_off + e];
+    const double2 v = raw[e];
     s += v.x * v.x + v.y * v.y;
     if (old) {
-      const double2 w = old[j.old_off + e];
+      const double2 w = old[e];
       so += w.x * w.x + w.y * w.y;
     }
   }
@@ -90,12 +89,12 @@ __global__ __launch_bounds__(256) void k_finalize_messages(const double2 *__rest
   double2 c = make_double2(1.0, 0.0), co = make_double2(1.0, 0.0);
   if (phase_gauge) {
     double tr = 0.0, ti = 0.0, tor = 0.0, toi = 0.0;
-    for (int i = threadIdx.x; i < j.D; i += blockDim.x) {
-      const double2 v = raw[j.raw_off + (int64_t)i * (j.D + 1)];
+    for (int i = threadIdx.x; i < D; i += blockDim.x) {
+      const double2 v = raw[(int64_t)i * (D + 1)];
       tr += v.x;
       ti += v.y;
       if (old) {
-        const double2 w = old[j.old_off + (int64_t)i * (j.D + 1)];
+        const double2 w = old[(int64_t)i * (D + 1)];
         tor += w.x;
         toi += w.y;
       }
@@ -115,20 +114,60 @@ __global__ __launch_bounds__(256) void k_finalize_messages(const double2 *__rest
   co = make_double2(co.x * invo, co.y * invo);
   double d = 0.0;
   for (int e = threadIdx.x; e < n; e += blockDim.x) {
-    const double2 v = raw[j.raw_off + e];
+    const double2 v = raw[e];
     const double2 m = make_double2(v.x * c.x - v.y * c.y, v.x * c.y + v.y * c.x);
-    out[j.new_off + e] = m;
-    if (out2) out2[j.new_off + e] = m;
+    out[e] = m;
+    if (out2) out2[e] = m;
     if (old) {
-      const double2 w = old[j.old_off + e];
+      const double2 w = old[e];
       const double dx = m.x - (w.x * co.x - w.y * co.y), dy = m.y - (w.x * co.y + w.y * co.x);
       d += dx * dx + dy * dy;
     }
   }
-  if (old) {
-    d = bsum(d, sh);
-    if (threadIdx.x == 0) dist[blockIdx.x] = sqrt(d);
-  }
+  if (old) d = bsum(d, sh);
+  return sqrt(d);
+}
+
+// normalise every raw message of one PEPS (offsets into the raw, out and old
+// buffers per job) and the distance of each to its stored message
+__global__ __launch_bounds__(256) void k_finalize_messages(const double2 *__restrict__ raw,
+                                                            double2 *__restrict__ out,
+                                                            const double2 *__restrict__ old,
+                                                            const MsgJob *__restrict__ jobs,
+                                                            double *__restrict__ dist,
+                                                            int phase_gauge,
+                                                            double2 *__restrict__ out2) {
+  __shared__ double sh[8];
+  const MsgJob j = jobs[blockIdx.x];
+  const double d = finalize_one(raw + j.raw_off, out + j.new_off, old ? old + j.old_off : nullptr,
+                                j.D, phase_gauge, out2 ? out2 + j.new_off : nullptr, sh);
+  if (old && threadIdx.x == 0) dist[blockIdx.x] = d;
+}
+
+// The same for the messages of several PEPS (a job per message, with its
+// own buffers), each PEPS's residual its distances' mean (bp.h:342-350).
+struct MsgJobB {
+  const double2 *raw;
+  double2 *out;
+  const double2 *old;
+  int D, phase_gauge;
+};
+__global__ __launch_bounds__(256) void k_finalize_batched(const MsgJobB *__restrict__ jobs,
+                                                           double *__restrict__ dist) {
+  __shared__ double sh[8];
+  const MsgJobB j = jobs[blockIdx.x];
+  const double d = finalize_one(j.raw, j.out, j.old, j.D, j.phase_gauge, nullptr, sh);
+  if (j.old && threadIdx.x == 0) dist[blockIdx.x] = d;
+}
+// res[p] = vol[p] * sum of dist[first[p] .. first[p + 1]) in a fixed order
+__global__ void k_residual_batched(const double *__restrict__ dist, const int *__restrict__ first,
+                                   const double *__restrict__ vol, double *__restrict__ res) {
+  __shared__ double sh[8];
+  const int p = blockIdx.x;
+  double s = 0.0;
+  for (int i = first[p] + threadIdx.x; i < first[p + 1]; i += blockDim.x) s += dist[i] * vol[p];
+  s = bsum(s, sh);
+  if (threadIdx.x == 0) res[p] = s;
 }
 
 __global__ void k_scatter_dist(const double *__restrict__ d, const int *__restrict__ pos, int n,
@@ -1726,7 +1765,7 @@ static void truncate(Ctx *ctx, Peps *p, int chi, double sv_min, int32_t *dims_ou
   }
   SvdJob *dsj = (SvdJob *)ctx->alloc(nm * sizeof(SvdJob));
   ctx->upload(dsj, sj.data(), nm * sizeof(SvdJob));
-  launch_batched_svd(ctx->stream, p->E, U, sv, Vd, W, dsj, nm, maxD);
+  launch_batched_svd(ctx->stream, p->E, U, sv, Vd, W, dsj, nm, maxD, maxD);
   // 2. square roots and pseudo-inverse roots
   double2 *roots = (double2 *)ctx->alloc(2 * tot * 16);
   std::vector<RootJob> rj(nm);
@@ -1771,7 +1810,7 @@ static void truncate(Ctx *ctx, Peps *p, int chi, double sv_min, int32_t *dims_ou
     tj[m] = SvdJob{ej[m].t_off, ej[m].u_off, ej[m].s_off, ej[m].vd_off, 2 * ej[m].t_off, D, D};
   }
   ctx->upload(dsj, tj.data(), ne * sizeof(SvdJob));
-  launch_batched_svd(ctx->stream, T, U, sv, Vd, W, dsj, ne, maxD);
+  launch_batched_svd(ctx->stream, T, U, sv, Vd, W, dsj, ne, maxD, maxD);
   // 5. selection and projectors
   double2 *proj = (double2 *)ctx->alloc(std::max<int64_t>(poff, 1) * 16);
   int *dkeep = (int *)ctx->alloc(ne * sizeof(int));
@@ -2728,6 +2767,267 @@ static void bp_iterate(Ctx *ctx, Peps *p, int n, double tol, int first, int last
   if (iters) *iters = std::min(it, n);
   if (converged) *converged = conv;
   ctx->release(dres, sizeof(double));
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------ batched BP loops
+// PEPS the batched small-bond sweep takes: fused mode with the reference's
+// Jacobi schedule, degree 1..4, every bond <= SMALL_D_MAX.
+static bool small_eligible(const Peps *p) {
+  if (p->mode != HBP_MODE_FUSED || p->schedule != HBP_SCHEDULE_JACOBI) return false;
+  for (int s = 0; s < p->n_sites; ++s)
+    if (p->z(s) < 1 || p->z(s) > 4) return false;
+  for (int m = 0; m < p->n_edges; ++m)
+    if (p->bond_dim[m] > SMALL_D_MAX) return false;
+  return true;
+}
+
+namespace {
+// one PEPS in a batched BP loop
+struct BatchMember {
+  Peps *p = nullptr;
+  double2 *G = nullptr;    // the other message set (E <-> G swap per iteration)
+  double2 *raw = nullptr;  // raw messages of the sweep
+  int index = 0;           // position in the caller's arrays
+  double stop = 0.0;
+  int iters = 0, conv = 0;
+  bool done = false;
+  std::vector<double> history;  // residual of every iteration
+};
+
+// The work of one sweep of every member (E -> raw -> G), uploaded to the
+// device: site jobs for the batched kernel, message jobs for the batched
+// finalisation, each member's message range for its residual.
+struct BatchTables {
+  SmallSiteJob *sites = nullptr;
+  MsgJobB *msgs = nullptr;
+  int *first = nullptr;
+  double *vol = nullptr;
+  int nsites = 0, nmsgs = 0, nmem = 0;
+  size_t bytes_sites = 0, bytes_msgs = 0, bytes_first = 0, bytes_vol = 0;
+  void release(Ctx *ctx) {
+    if (sites) ctx->release(sites, bytes_sites);
+    if (msgs) ctx->release(msgs, bytes_msgs);
+    if (first) ctx->release(first, bytes_first);
+    if (vol) ctx->release(vol, bytes_vol);
+    *this = BatchTables{};
+  }
+};
+
+BatchTables build_tables(Ctx *ctx, const std::vector<BatchMember *> &mem, bool residual) {
+  std::vector<SmallSiteJob> sj;
+  std::vector<MsgJobB> mj;
+  std::vector<int> first{0};
+  std::vector<double> vol;
+  for (BatchMember *b : mem) {
+    Peps *p = b->p;
+    for (int s = 0; s < p->n_sites; ++s) {
+      SmallSiteJob j{};
+      j.V = p->V[s];
+      j.E = p->E;
+      j.raw = b->raw;
+      j.z = p->z(s);
+      for (int k = 0; k < j.z; ++k) {
+        j.dims[k] = p->bond_dim[p->bonds[s][k]];
+        j.in_off[k] = p->E_off[p->in_msg(s, k)];
+        j.out_off[k] = p->E_off[p->out_msg(s, k)];
+      }
+      j.dims[j.z] = p->phys[s];
+      sj.push_back(j);
+    }
+    for (int m = 0; m < 2 * p->n_edges; ++m)
+      mj.push_back(MsgJobB{b->raw + p->E_off[m], b->G + p->E_off[m],
+                           residual ? p->E + p->E_off[m] : nullptr, p->msg_dim(m),
+                           p->phase_gauge ? 1 : 0});
+    first.push_back((int)mj.size());
+    vol.push_back(1.0 / (2.0 * p->n_edges));
+  }
+  BatchTables t;
+  t.nsites = (int)sj.size();
+  t.nmsgs = (int)mj.size();
+  t.nmem = (int)mem.size();
+  t.bytes_sites = std::max<size_t>(sj.size(), 1) * sizeof(SmallSiteJob);
+  t.bytes_msgs = std::max<size_t>(mj.size(), 1) * sizeof(MsgJobB);
+  t.bytes_first = first.size() * sizeof(int);
+  t.bytes_vol = std::max<size_t>(vol.size(), 1) * sizeof(double);
+  t.sites = (SmallSiteJob *)ctx->alloc(t.bytes_sites);
+  t.msgs = (MsgJobB *)ctx->alloc(t.bytes_msgs);
+  t.first = (int *)ctx->alloc(t.bytes_first);
+  t.vol = (double *)ctx->alloc(t.bytes_vol);
+  ctx->upload(t.sites, sj.data(), sj.size() * sizeof(SmallSiteJob));
+  ctx->upload(t.msgs, mj.data(), mj.size() * sizeof(MsgJobB));
+  ctx->upload(t.first, first.data(), first.size() * sizeof(int));
+  if (!vol.empty()) ctx->upload(t.vol, vol.data(), vol.size() * sizeof(double));
+  return t;
+}
+
+// one sweep of every member from the tables: the batched site kernel with
+// the grid `variant`, finalisation, residuals (when the tables carry the old
+// messages); nothing here synchronises, so it can be captured
+template <typename S>
+void launch_batch_sweep(Ctx *ctx, const BatchTables &t, void *scratch, int64_t slab, int variant,
+                        double *dist, double *res, bool residual) {
+  launch_sweep_small<S>(ctx->stream, t.sites, t.nsites, scratch, slab,
+                        small_sweep_grid(variant, t.nsites));
+  hipLaunchKernelGGL(k_finalize_batched, dim3((unsigned)t.nmsgs), dim3(256), 0, ctx->stream,
+                     t.msgs, dist);
+  HBP_HIP(hipGetLastError());
+  if (residual) {
+    hipLaunchKernelGGL(k_residual_batched, dim3((unsigned)t.nmem), dim3(256), 0, ctx->stream,
+                       dist, t.first, t.vol, res);
+    HBP_HIP(hipGetLastError());
+  }
+}
+}  // namespace
+
+// The driver's BP loop (as bp_iterate with first = last = 1) for several
+// PEPS of one precision in lockstep: every iteration is one batched sweep of
+// all members still iterating; a member leaves when its residual is below
+// its stop value or at max_iter, keeping E like the reference.  After the
+// first iteration (which also times the launch variants) the launches of an
+// iteration are captured in a HIP graph per message-set parity and replayed.
+template <typename S>
+static void bp_run_batched_impl(Ctx *ctx, std::vector<BatchMember> &mem, int max_iter) {
+  int64_t slab = 1;
+  for (auto &b : mem) {
+    Peps *p = b.p;
+    drop_pair_cache(ctx, p);
+    for (int s = 0; s < p->n_sites; ++s) make_resident(ctx, p, s);
+    for (int s = 0; s < p->n_sites; ++s) slab = std::max<int64_t>(slab, (int64_t)p->site_size(s));
+    b.G = (double2 *)ctx->alloc(std::max<size_t>(p->E_bytes, 16));
+    b.raw = (double2 *)ctx->alloc(std::max<size_t>(p->E_bytes, 16));
+  }
+  int nsites_all = 0, nmsgs_all = 0;
+  for (auto &b : mem) {
+    nsites_all += b.p->n_sites;
+    nmsgs_all += 2 * b.p->n_edges;
+  }
+  const int grid_max = small_sweep_grid(SMALL_GRID_VARIANTS - 1, nsites_all);
+  const size_t scratch_bytes = (size_t)grid_max * 2 * (size_t)slab * 2 * sizeof(S);
+  void *scratch = ctx->alloc(scratch_bytes);
+  double *dist = (double *)ctx->alloc(std::max(nmsgs_all, 1) * sizeof(double));
+  double *res = (double *)ctx->alloc(std::max<size_t>(mem.size(), 1) * sizeof(double));
+  auto swap_sets = [&](BatchMember &b) {
+    std::swap(b.p->E, b.G);
+    b.p->E_diag.assign(2 * b.p->n_edges, 0);
+  };
+  // the loop's first sweep E_1 = BP(E_0), no residual (main.cc:121-141)
+  std::vector<BatchMember *> act;
+  for (auto &b : mem) act.push_back(&b);
+  int variant = 0;
+  {
+    BatchTables t = build_tables(ctx, act, false);
+    variant = launch_sweep_small_tuned<S>(ctx->stream, t.sites, t.nsites, scratch, slab);
+    hipLaunchKernelGGL(k_finalize_batched, dim3((unsigned)t.nmsgs), dim3(256), 0, ctx->stream,
+                       t.msgs, dist);
+    HBP_HIP(hipGetLastError());
+    ctx->sync();
+    t.release(ctx);
+    for (auto *b : act) swap_sets(*b);
+  }
+  std::vector<double> hres(mem.size());
+  // graphs of the current member set, one per parity of the E/G swap
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  BatchTables tabs[2];
+  bool have[2] = {false, false};
+  auto drop_graphs = [&]() {
+    for (int q = 0; q < 2; ++q) {
+      if (exec[q]) (void)hipGraphExecDestroy(exec[q]);
+      exec[q] = nullptr;
+      if (have[q]) tabs[q].release(ctx);
+      have[q] = false;
+    }
+  };
+  int parity = 0;
+  for (int it = 1; it <= max_iter && !act.empty(); ++it) {
+    if (!have[parity]) {
+      tabs[parity] = build_tables(ctx, act, true);
+      have[parity] = true;
+      hipGraph_t g = nullptr;
+      HBP_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      launch_batch_sweep<S>(ctx, tabs[parity], scratch, slab, variant, dist, res, true);
+      HBP_HIP(hipStreamEndCapture(ctx->stream, &g));
+      HBP_HIP(hipGraphInstantiate(&exec[parity], g, nullptr, nullptr, 0));
+      HBP_HIP(hipGraphDestroy(g));
+    }
+    HBP_HIP(hipGraphLaunch(exec[parity], ctx->stream));
+    HBP_HIP(hipMemcpyAsync(hres.data(), res, act.size() * sizeof(double), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    ctx->sync();
+    bool left = false;
+    std::vector<BatchMember *> next;
+    for (size_t i = 0; i < act.size(); ++i) {
+      BatchMember *b = act[i];
+      b->iters = it;
+      b->history.push_back(hres[i]);
+      b->conv = hres[i] < b->stop ? 1 : 0;
+      b->done = b->conv || it == max_iter;
+      if (b->done) {
+        left = true;  // keeps E, like the reference at its stop (main.cc:138-158)
+      } else {
+        swap_sets(*b);
+        next.push_back(b);
+      }
+    }
+    if (left) drop_graphs();
+    act.swap(next);
+    parity ^= 1;
+  }
+  drop_graphs();
+  ctx->release(res, std::max<size_t>(mem.size(), 1) * sizeof(double));
+  ctx->release(dist, std::max(nmsgs_all, 1) * sizeof(double));
+  ctx->release(scratch, scratch_bytes);
+  for (auto &b : mem) {
+    ctx->release(b.G, std::max<size_t>(b.p->E_bytes, 16));
+    ctx->release(b.raw, std::max<size_t>(b.p->E_bytes, 16));
+  }
+}
+
+extern "C" {
+
+hbp_status hbp_peps_bp_run_batched(hbp_ctx *ctx, int n, hbp_peps *const *ps, int max_iter,
+                                   double tol, int *iters, double *residuals, int *converged,
+                                   int *batched) {
+  API_BEGIN
+  HBP_REQUIRE(n >= 0 && (n == 0 || ps), HBP_ERR_INVALID_ARGUMENT, "bp batched: no PEPS");
+  HBP_REQUIRE(max_iter >= 1, HBP_ERR_INVALID_ARGUMENT, "bp: iteration count < 1");
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < i; ++j)
+      HBP_REQUIRE(ps[i] != ps[j], HBP_ERR_INVALID_ARGUMENT, "bp batched: a PEPS appears twice");
+  std::vector<BatchMember> mem[2];
+  std::vector<int> alone;
+  for (int i = 0; i < n; ++i) {
+    Peps *p = ps[i];
+    if (small_eligible(p)) {
+      BatchMember b;
+      b.p = p;
+      b.index = i;
+      b.stop = p->precision == HBP_PREC_MIXED ? std::max(tol, p->mixed_floor) : tol;
+      mem[p->precision == HBP_PREC_MIXED ? 1 : 0].push_back(b);
+    } else {
+      alone.push_back(i);
+    }
+  }
+  auto report = [&](int i, int it, int conv, const double *hist, int nh, int in_batch) {
+    if (iters) iters[i] = it;
+    if (converged) converged[i] = conv;
+    if (batched) batched[i] = in_batch;
+    if (residuals)
+      for (int k = 0; k < nh && k < max_iter; ++k) residuals[(size_t)i * max_iter + k] = hist[k];
+  };
+  if (!mem[0].empty()) bp_run_batched_impl<double>(ctx, mem[0], max_iter);
+  if (!mem[1].empty()) bp_run_batched_impl<float>(ctx, mem[1], max_iter);
+  for (int q = 0; q < 2; ++q)
+    for (auto &b : mem[q])
+      report(b.index, b.iters, b.conv, b.history.data(), (int)b.history.size(), 1);
+  for (int i : alone) {
+    int it = 0, conv = 0;
+    std::vector<double> h(max_iter, 0.0);
+    bp_iterate(ctx, ps[i], max_iter, tol, 1, 1, &it, h.data(), &conv);
+    report(i, it, conv, h.data(), it, 0);
+  }
+  API_END
 }
 
 hbp_status hbp_peps_bp_run(hbp_ctx *ctx, hbp_peps *p, int max_iter, double tol, int *iters,

@@ -200,6 +200,27 @@ void Ctx::prof_clear() {
     (void)hipEventDestroy(r.b);
   }
   prof_recs.clear();
+  for (auto &k : prof_acc)
+    for (double &v : k) v = 0.0;
+}
+
+void Ctx::prof_fold() {
+  const size_t n = prof_recs.size() / 2;
+  for (size_t i = 0; i < n; ++i) {
+    ProfRec &r = prof_recs[i];
+    if (r.timed && r.kind >= 0 && r.kind < 3) {
+      HBP_HIP(hipEventSynchronize(r.b));
+      float t = 0.0f;
+      HBP_HIP(hipEventElapsedTime(&t, r.a, r.b));
+      prof_acc[r.kind][0] += 1.0;
+      prof_acc[r.kind][1] += t;
+      prof_acc[r.kind][2] += r.flops;
+      prof_acc[r.kind][3] += r.bytes;
+    }
+    HBP_HIP(hipEventDestroy(r.a));
+    HBP_HIP(hipEventDestroy(r.b));
+  }
+  prof_recs.erase(prof_recs.begin(), prof_recs.begin() + (std::ptrdiff_t)n);
 }
 
 LaunchTiming &launch_timing() {
@@ -213,6 +234,7 @@ bool &launch_tuning() {
 
 int Ctx::prof_begin(int kind, double flops, double bytes) {
   if (!prof) return -1;
+  if (prof_recs.size() >= kProfMaxLive) prof_fold();
   ProfRec r{prof_event(), prof_event(), kind, flops, bytes, false};
   LaunchTiming &t = launch_timing();
   t.start = r.a;
@@ -560,7 +582,7 @@ void svd_impl(Ctx *ctx, const Tensor *a, int nrows, Tensor *u, Tensor *s, Tensor
   SvdJob job{0, 0, 0, 0, 0, (int)m, (int)n};
   SvdJob *dj = (SvdJob *)ctx->alloc(sizeof(SvdJob));
   ctx->upload(dj, &job, sizeof(job));
-  launch_batched_svd(ctx->stream, (const double2 *)a->data, U, S, Vd, W, dj, 1, (int)n);
+  launch_batched_svd(ctx->stream, (const double2 *)a->data, U, S, Vd, W, dj, 1, (int)n, (int)m);
   std::vector<double> hs(k);
   HBP_HIP(hipMemcpyAsync(hs.data(), S, k * 8, hipMemcpyDeviceToHost, ctx->stream));
   ctx->sync();
@@ -751,6 +773,14 @@ hbp_status hbp_ctx_profile_read(hbp_ctx *ctx, int kind, int64_t *count, double *
     HBP_HIP(hipEventDestroy(r.b));
   }
   ctx->prof_recs.swap(keep);
+  if (kind >= 0 && kind < 3) {
+    double *acc = ctx->prof_acc[kind];
+    n += (int64_t)acc[0];
+    ms += acc[1];
+    fl += acc[2];
+    by += acc[3];
+    for (int i = 0; i < 4; ++i) acc[i] = 0.0;
+  }
   if (count) *count = n;
   if (total_ms) *total_ms = ms;
   if (flops) *flops = fl;
@@ -1060,7 +1090,7 @@ hbp_status hbp_svd_noalloc(hbp_ctx *ctx, const void *a, int m, int n, void *u, d
   // the kernel initialises every work element it reads (A's copy, V = I)
   launch_store_svd_job(ctx->stream, SvdJob{0, 0, 0, 0, 0, m, n}, dj);
   launch_batched_svd(ctx->stream, (const double2 *)a, (double2 *)u, s, (double2 *)vdag, W, dj, 1,
-                     n);
+                     n, m);
   HBP_API_END
 }
 

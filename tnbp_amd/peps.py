@@ -21,7 +21,7 @@ from ._lib import (HBP_ABSORB_AUTO, HBP_ABSORB_EAGER, HBP_ABSORB_LAZY, HBP_MAX_O
 from .tci import Context
 
 __all__ = ["Peps", "InitTensorProductState", "AbsorbTPO", "BeliefPropagation",
-           "BeliefPropagationCondition", "bp_loop", "Truncation", "Measure", "MeasureEdges",
+           "BeliefPropagationCondition", "bp_loop", "bp_loop_batched", "Truncation", "Measure", "MeasureEdges",
            "HBP_PREC_FP64", "HBP_PREC_MIXED", "HBP_MODE_FUSED", "HBP_MODE_PLAIN",
            "HBP_SCHEDULE_JACOBI", "HBP_SCHEDULE_CHECKERBOARD"]
 
@@ -232,6 +232,25 @@ def bp_loop(ctx, peps: Peps, max_iter: int, tol: float, progress=None):
         if conv:
             break
     return len(residuals), residuals
+
+
+def bp_loop_batched(ctx, peps_list, max_iter: int, tol: float):
+    """The BP loop of several PEPS of one context at once
+    (``hbp_peps_bp_run_batched``): PEPS with bonds <= 16 iterate in lockstep,
+    one graph-replayed persistent launch per iteration for all of them; each
+    gets what ``bp_loop`` gives it alone.  Returns a list of (iterations,
+    residuals, converged, batched) per PEPS."""
+    n = len(peps_list)
+    if n == 0:
+        return []
+    hs = (C.c_void_p * n)(*[p.h.value if isinstance(p.h, C.c_void_p) else p.h for p in peps_list])
+    it = (C.c_int * n)()
+    conv = (C.c_int * n)()
+    bat = (C.c_int * n)()
+    res = np.zeros((n, max_iter), dtype=np.float64)
+    check(ctx.lib.hbp_peps_bp_run_batched(ctx.h, n, hs, max_iter, tol, it, res.ctypes.data_as(
+        C.POINTER(C.c_double)), conv, bat))
+    return [(it[i], list(res[i, : it[i]]), bool(conv[i]), bool(bat[i])) for i in range(n)]
 
 
 def BPIterate(ctx, peps: Peps, n: int, tol: float, first: bool, last: bool):

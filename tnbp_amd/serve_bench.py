@@ -6,10 +6,13 @@ observable (weight 1..w_max on contiguous row runs, coefficients ~ N(0, 1)),
 drawn from a seed.  Jobs arrive with exponential inter-arrival times at each
 swept rate; the server (continuous batching, footprint admission) reports
 p50/p99 latency (arrival -> result) and expectation values/s.  One JSON line
-per (path, rate).
+per (path, server, rate).  ``--server`` picks the threaded server (one job
+per worker thread and context), the layer-batched one (every resident job's
+BP loop in one batched call per circuit layer) or both on the same stream.
 
     python -m tnbp_amd.serve_bench [--jobs 24] [--rates 2,8,32] [--L 6,10]
         [--chi 8,16] [--depth 3,6] [--terms 1,64] [--wmax 6] [--slots 4] [--plain 1]
+        [--server threaded|batched|both]
 """
 from __future__ import annotations
 
@@ -39,10 +42,11 @@ def make_jobs(seed, n, Ls, chis, depths, terms, wmax, precision, mode=0):
     return jobs
 
 
-def run_stream(jobs, rate, slots, seed):
+def run_stream(jobs, rate, slots, seed, server="threaded"):
     rng = np.random.default_rng(seed + 1)
     t0 = time.perf_counter()
-    with serving.Server(slots=slots) as srv:
+    make = serving.BatchedServer if server == "batched" else (lambda: serving.Server(slots=slots))
+    with make() as srv:
         futs = serving.poisson_submit(srv, jobs, rate, rng)
         for f in futs:  # one progress line per finished job
             f.add_done_callback(lambda f: print(json.dumps(
@@ -53,6 +57,9 @@ def run_stream(jobs, rate, slots, seed):
         st = srv.stats()
     st["wall_s"] = time.perf_counter() - t0
     st["rate_jobs_per_s"] = rate
+    if server == "batched":
+        st["rounds"] = len(srv.rounds)
+        st["mean_jobs_per_round"] = float(np.mean([r["jobs"] for r in srv.rounds]))
     return st, res
 
 
@@ -68,22 +75,26 @@ def main():
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--plain", type=int, default=1, help="also run the plain FP64 path at the lowest rate")
+    ap.add_argument("--server", default="threaded", choices=["threaded", "batched", "both"])
     a = ap.parse_args()
+    servers = ["threaded", "batched"] if a.server == "both" else [a.server]
     two = lambda s: [int(x) for x in s.split(",")]
     Ls, chis, depths, terms = two(a.L), two(a.chi), two(a.depth), two(a.terms)
     rates = [float(x) for x in a.rates.split(",")]
     ref = None
     for rate in rates:
-        jobs = make_jobs(a.seed, a.jobs, Ls, chis, depths, terms, a.wmax, HBP_PREC_MIXED)
-        st, res = run_stream(jobs, rate, a.slots, a.seed)
-        if ref is None:
-            ref = res
-        print(json.dumps({"path": "mixed/fused", **st}), flush=True)
+        for server in servers:
+            jobs = make_jobs(a.seed, a.jobs, Ls, chis, depths, terms, a.wmax, HBP_PREC_MIXED)
+            st, res = run_stream(jobs, rate, a.slots, a.seed, server)
+            if ref is None:
+                ref = res
+            print(json.dumps({"path": "mixed/fused", "server": server, **st}), flush=True)
     if a.plain:
         jobs = make_jobs(a.seed, a.jobs, Ls, chis, depths, terms, a.wmax, HBP_PREC_FP64, mode=1)
         st, res = run_stream(jobs, rates[0], a.slots, a.seed)
         err = max(abs(x["value"] - y["value"]) / max(1.0, abs(y["value"])) for x, y in zip(ref, res))
-        print(json.dumps({"path": "fp64/plain", **st, "max_rel_err_mixed_vs_plain": err}), flush=True)
+        print(json.dumps({"path": "fp64/plain", "server": "threaded", **st,
+                          "max_rel_err_mixed_vs_plain": err}), flush=True)
 
 
 if __name__ == "__main__":

@@ -93,19 +93,29 @@ def absorbs_eagerly(job: EstimatorJob) -> bool:
     return job.mode == 1 or absorbed_bytes(job) <= EAGER_ABSORB_MAX_BYTES
 
 
+def pair_cache_bytes(job: EstimatorJob) -> int:
+    """Bytes of the merged sweep's pair-front site copies (sites of degree 3
+    and 4 at D = chi * t, when the merged kernels take D, i.e. 32 < D <= 64,
+    and the job absorbs eagerly; 0 otherwise)."""
+    elem, deg, D = _sizes(job)
+    if job.mode == 1 or not absorbs_eagerly(job) or not 32 < D <= 64:
+        return 0
+    return sum(D ** z * 2 * elem for z in deg.values() if z in (3, 4))
+
+
 def footprint_bytes(job: EstimatorJob) -> int:
     """Device bytes a job holds at its peak: resident chi-sized sites, the
-    absorbed lattice when it absorbs eagerly (the plain mode always does), the
-    messages (three sets during a sweep), the work buffers of the per-site
-    sweep (four of the largest site at D = chi * t), split-K partials, and
-    slack."""
+    absorbed lattice when it absorbs eagerly (the plain mode always does) and
+    then also the pair-front copies of the merged sweep, the messages (three
+    sets during a sweep), the work buffers of the per-site sweep (four of the
+    largest site at D = chi * t), split-K partials, and slack."""
     elem, deg, D = _sizes(job)
     chi, d = job.chi, 2
     resident = sum(chi ** z * d * elem for z in deg.values())
     zmax = max(deg.values())
     work = 4 * D ** zmax * d * elem
     if absorbs_eagerly(job):
-        work += absorbed_bytes(job)
+        work += absorbed_bytes(job) + pair_cache_bytes(job)
     msgs = 3 * 2 * len(job.edges) * D * D * 16
     partials = 1024 * min(D, 128) ** 2 * 16 * 2
     return int(1.15 * (resident + work + msgs + partials)) + (64 << 20)
@@ -120,6 +130,9 @@ def run_job(ctx: Context, job: EstimatorJob):
     # lattice when that is small enough to absorb eagerly
     peps.set_absorb_mode(HBP_ABSORB_EAGER if absorbs_eagerly(job) else HBP_ABSORB_LAZY)
     peps.set_mode(job.mode)
+    # the pair-front copies only where the admitted footprint counted them:
+    # the device's free memory is shared with the other admitted jobs
+    peps.set_pair_cache(1 if pair_cache_bytes(job) > 0 else 0)
     iters = []
     try:
         for tpo in job.tpos:
@@ -269,6 +282,209 @@ class Server:
                 fut.set_result(res)
         finally:
             ctx.close()
+
+
+class BatchedServer:
+    """Continuous batching at the granularity of circuit layers, on one
+    context (no reference counterpart).
+
+    One scheduler thread keeps every admitted job's PEPS resident and runs
+    rounds: each resident job absorbs its next TPO slice, then the BP loops of
+    all resident jobs run as one ``hbp_peps_bp_run_batched`` call (jobs whose
+    bonds are <= 16 iterate together: one persistent launch over every site
+    of every such job per BP iteration, replayed from a HIP graph; larger
+    ones run their own fused sweeps in the same call), then each job
+    truncates; a job whose last slice is done is measured and leaves.  Queued
+    jobs are admitted by footprint at every round boundary, so a job that
+    arrives while others run joins the next round.  A job's result does not
+    depend on the others in the batch (every site's messages are computed by
+    one workgroup in a fixed order).  Same interface as ``Server``.
+    """
+
+    # device scratch of the batched sweep (per precision: up to 4 workgroups
+    # per CU, two site-sized buffers each at bond 16), kept out of the budget
+    SCRATCH_RESERVE = 6 << 30
+
+    def __init__(self, device=0, memory_budget=None, max_bypass=8, make_context=None):
+        self.device = device
+        self._make_context = make_context or (lambda: Context(device))
+        if memory_budget is None:
+            free, _ = device_memory(device)
+            memory_budget = int(0.9 * free) - self.SCRATCH_RESERVE
+        self.budget = memory_budget
+        self.max_bypass = max_bypass
+        self._pending = deque()
+        self._cv = threading.Condition()
+        self._in_use = 0
+        self._closing = False
+        self.records = []
+        self.rounds = []
+        self._thread = threading.Thread(target=self._loop, name="hbp-batch", daemon=True)
+        self._thread.start()
+
+    submit = Server.submit
+    stats = Server.stats
+
+    @property
+    def slots(self):
+        return 1
+
+    def close(self):
+        with self._cv:
+            self._closing = True
+            self._cv.notify_all()
+        self._thread.join()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _admit(self, block):
+        """Jobs that fit now (FIFO with bounded bypass, like Server._take);
+        waits for arrivals when `block` and nothing is resident."""
+        out = []
+        with self._cv:
+            while True:
+                i = 0
+                while i < len(self._pending):
+                    job, fp, fut = self._pending[i]
+                    if fp <= self.budget - self._in_use:
+                        del self._pending[i]
+                        for j in range(i):
+                            self._pending[j][0].bypassed += 1
+                        self._in_use += fp
+                        out.append((job, fp, fut))
+                        continue
+                    if job.bypassed >= self.max_bypass:
+                        break
+                    i += 1
+                if out or not block:
+                    return out
+                if self._closing and not self._pending:
+                    return None
+                self._cv.wait()
+
+    def _loop(self):
+        from .peps import bp_loop_batched
+        try:
+            ctx = self._make_context()
+        except BaseException as e:
+            with self._cv:
+                self._dead, self._dead_reason = 1, e
+                while self._pending:
+                    self._pending.popleft()[2].set_exception(e)
+            return
+        active = []
+        try:
+            while True:
+                new = self._admit(block=not active)
+                if new is None:
+                    return
+                for job, fp, fut in new:
+                    start = time.perf_counter()
+                    try:
+                        sites = sorted({x for e in job.edges for x in e})
+                        peps = InitTensorProductState(ctx, job.edges, [2] * len(sites), job.precision)
+                        peps.set_absorb_mode(HBP_ABSORB_EAGER if absorbs_eagerly(job) else HBP_ABSORB_LAZY)
+                        peps.set_mode(job.mode)
+                        peps.set_pair_cache(1 if pair_cache_bytes(job) > 0 else 0)
+                    except BaseException as e:
+                        self._finish_fp(fp)
+                        fut.set_exception(e)
+                        continue
+                    active.append({"job": job, "fp": fp, "fut": fut, "peps": peps, "layer": 0,
+                                   "start": start, "iters": []})
+                if not active:
+                    continue
+                t0 = time.perf_counter()
+                for r in active:
+                    AbsorbTPO(ctx, r["job"].tpos[r["layer"]], r["peps"])
+                groups = {}
+                for r in active:
+                    groups.setdefault((r["job"].max_bp_iterations, r["job"].bp_tolerance), []).append(r)
+                nb = 0
+                for (mi, tol), rs in groups.items():
+                    out = bp_loop_batched(ctx, [r["peps"] for r in rs], mi, tol)
+                    for r, (it, res, conv, bat) in zip(rs, out):
+                        r["iters"].append(it)
+                        nb += int(bat)
+                for r in active:
+                    Truncation(ctx, r["peps"], r["job"].chi, r["job"].sv_min)
+                    r["layer"] += 1
+                self.rounds.append({"jobs": len(active), "batched": nb,
+                                    "seconds": time.perf_counter() - t0})
+                still = []
+                for r in active:
+                    if r["layer"] < len(r["job"].tpos):
+                        still.append(r)
+                        continue
+                    self._retire(ctx, r, len(active))
+                active = still
+        except BaseException as e:  # fail every resident, queued and later job
+            for r in active:
+                if not r["fut"].done():
+                    r["fut"].set_exception(e)
+            with self._cv:
+                self._dead, self._dead_reason = 1, e
+                while self._pending:
+                    self._pending.popleft()[2].set_exception(e)
+        finally:
+            for r in active:
+                try:
+                    r["peps"].close()
+                except Exception:
+                    pass
+            ctx.close()
+
+    def _finish_fp(self, fp):
+        with self._cv:
+            self._in_use -= fp
+            self._cv.notify_all()
+
+    def _retire(self, ctx, r, concurrent):
+        job = r["job"]
+        try:
+            total, vals = pauli.estimate(ctx, r["peps"], job.edges, job.terms)
+        except BaseException as e:
+            r["peps"].close()
+            self._finish_fp(r["fp"])
+            r["fut"].set_exception(e)
+            return
+        r["peps"].close()
+        ctx.trim()
+        done = time.perf_counter()
+        self._finish_fp(r["fp"])
+        res = {"value": total, "values": vals, "bp_iterations": r["iters"], "n_values": len(vals)}
+        rec = {"name": job.name, "arrival": job.arrival, "start": r["start"], "done": done,
+               "latency": done - job.arrival, "wait": r["start"] - job.arrival,
+               "n_values": len(vals), "concurrent": concurrent, "footprint": r["fp"]}
+        res.update(rec)
+        self.records.append(rec)
+        r["fut"].set_result(res)
+
+
+def run_job_batched(ctx, job: EstimatorJob):
+    """``run_job`` through the batched BP loop (a batch of one): the result a
+    job gets from ``BatchedServer`` alone."""
+    from .peps import bp_loop_batched
+    sites = sorted({s for e in job.edges for s in e})
+    peps = InitTensorProductState(ctx, job.edges, [2] * len(sites), job.precision)
+    peps.set_absorb_mode(HBP_ABSORB_EAGER if absorbs_eagerly(job) else HBP_ABSORB_LAZY)
+    peps.set_mode(job.mode)
+    peps.set_pair_cache(1 if pair_cache_bytes(job) > 0 else 0)
+    iters = []
+    try:
+        for tpo in job.tpos:
+            AbsorbTPO(ctx, tpo, peps)
+            (it, _, _, _), = bp_loop_batched(ctx, [peps], job.max_bp_iterations, job.bp_tolerance)
+            iters.append(it)
+            Truncation(ctx, peps, job.chi, job.sv_min)
+        total, vals = pauli.estimate(ctx, peps, job.edges, job.terms)
+    finally:
+        peps.close()
+    return {"value": total, "values": vals, "bp_iterations": iters, "n_values": len(vals)}
 
 
 # ------------------------------------------------------------- workloads
