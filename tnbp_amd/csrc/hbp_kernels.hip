@@ -52,6 +52,56 @@ template <> struct Mfma<float> {
   __device__ static inline int row(int lk, int v) { return 4 * lk + v; }
 };
 
+// Complex multiply-accumulate of 16x16 blocks on the real 16x16x4 MFMA, the
+// small matrix E as the A operand and the streamed tensor T as B.
+//   FP64: four real products, Re += Er Tr - Ei Ti, Im += Ei Tr + Er Ti.
+//   FP32: Gauss's three real products, P1 += Er Tr, P2 += Ei Ti,
+//         P3 += (Er + Ei)(Tr + Ti); Re = P1 - P2, Im = (P3 - P1) - P2: a
+//         quarter fewer MFMA cycles for two adds per step, with an error of
+//         the same order as the four-product form's (a few FP32 ulps of
+//         |E| |T| per term, against the 1e-5 budget of the mixed path).
+// Every mode-product form of one precision goes through this, so they stay
+// bitwise equal to each other.
+template <typename S> struct CAcc;
+template <> struct CAcc<double> {
+  using M = Mfma<double>;
+  struct B { double r, i, ni; };
+  __device__ static inline B prep(double r, double i) { return B{r, i, -i}; }
+  M::v4 re, im;
+  __device__ inline void zero() {
+    re = M::v4{0, 0, 0, 0};
+    im = M::v4{0, 0, 0, 0};
+  }
+  // (-Ei) Ti = Ei (-Ti) exactly: the negation once per B operand
+  __device__ inline void mac(double ar, double ai, const B &b) {
+    re = M::op(ar, b.r, re);
+    re = M::op(ai, b.ni, re);
+    im = M::op(ai, b.r, im);
+    im = M::op(ar, b.i, im);
+  }
+  __device__ inline double out_re(int v) const { return re[v]; }
+  __device__ inline double out_im(int v) const { return im[v]; }
+};
+template <> struct CAcc<float> {
+  using M = Mfma<float>;
+  struct B { float r, i, s; };
+  __device__ static inline B prep(float r, float i) { return B{r, i, r + i}; }
+  M::v4 p1, p2, p3;
+  __device__ inline void zero() {
+    p1 = M::v4{0, 0, 0, 0};
+    p2 = M::v4{0, 0, 0, 0};
+    p3 = M::v4{0, 0, 0, 0};
+  }
+  __device__ inline void mac(float ar, float ai, const B &b) {
+    const float as = ar + ai;
+    p1 = M::op(ar, b.r, p1);
+    p2 = M::op(ai, b.i, p2);
+    p3 = M::op(as, b.s, p3);
+  }
+  __device__ inline float out_re(int v) const { return p1[v] - p2[v]; }
+  __device__ inline float out_im(int v) const { return (p3[v] - p1[v]) - p2[v]; }
+};
+
 // Workgroups per CU the mode-product kernel asks the compiler to fit: two for
 // FP64 (its LDS tile takes 66 KB), three for FP32 (33 KB; at four its widest
 // variant spills registers), so that FP32 has more waves per SIMD to hide HBM
@@ -134,14 +184,11 @@ __device__ __forceinline__ void mode_product_tiles(
       const int64_t l = qq % L, r = qq / L;
       abase[rb] = T + l + L * (int64_t)Din * r;
     }
-    v4 acc_re[RB][NCB], acc_im[RB][NCB];
+    CAcc<S> acc[RB][NCB];
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-      for (int cb = 0; cb < NCB; ++cb) {
-        acc_re[rb][cb] = v4{0, 0, 0, 0};
-        acc_im[rb][cb] = v4{0, 0, 0, 0};
-      }
+      for (int cb = 0; cb < NCB; ++cb) acc[rb][cb].zero();
 
     for (int kc = 0; kc < Din; kc += MP_KC) {
       if (!single) {
@@ -163,10 +210,9 @@ __device__ __forceinline__ void mode_product_tiles(
         }
       };
       auto step = [&](int k4, const S *are, const S *aim) {
-        // (-Ei) Ti = Ei (-Ti) exactly: the negation once per row block
-        S naim[RB];
+        typename CAcc<S>::B bt[RB];
 #pragma unroll
-        for (int rb = 0; rb < RB; ++rb) naim[rb] = -aim[rb];
+        for (int rb = 0; rb < RB; ++rb) bt[rb] = CAcc<S>::prep(are[rb], aim[rb]);
 #pragma unroll
         for (int cb = 0; cb < NCB; ++cb) {
           const S bre = er[k4 + lk][cb * 16 + li];
@@ -175,12 +221,7 @@ __device__ __forceinline__ void mode_product_tiles(
           // transposed (lane li holds output row li), so that each store
           // instruction writes 16 consecutive rows of 4 output columns
 #pragma unroll
-          for (int rb = 0; rb < RB; ++rb) {
-            acc_re[rb][cb] = M::op(bre, are[rb], acc_re[rb][cb]);
-            acc_re[rb][cb] = M::op(bim, naim[rb], acc_re[rb][cb]);
-            acc_im[rb][cb] = M::op(bim, are[rb], acc_im[rb][cb]);
-            acc_im[rb][cb] = M::op(bre, aim[rb], acc_im[rb][cb]);
-          }
+          for (int rb = 0; rb < RB; ++rb) acc[rb][cb].mac(bre, bim, bt[rb]);
         }
       };
       // a ring of PF k-steps in flight: the loads for k-step s + PF are issued
@@ -226,8 +267,8 @@ __device__ __forceinline__ void mode_product_tiles(
           const int j = cb * 16 + M::row(lk, v);
           if (j >= Dout) continue;
           C o;
-          o.x = acc_re[rb][cb][v] * scale;
-          o.y = acc_im[rb][cb][v] * scale;
+          o.x = acc[rb][cb].out_re(v) * scale;
+          o.y = acc[rb][cb].out_im(v) * scale;
           ob[L * (int64_t)j] = o;
           nrm += (double)o.x * (double)o.x + (double)o.y * (double)o.y;
         }
@@ -301,31 +342,23 @@ __device__ __forceinline__ void mode_product_tiles_pf(
   };
   double nrm = 0.0;
   auto compute_store = [&](int64_t tile, S (*re)[RB], S (*im)[RB]) {
-    v4 acc_re[RB][NCB], acc_im[RB][NCB];
+    CAcc<S> acc[RB][NCB];
 #pragma unroll
     for (int rb = 0; rb < RB; ++rb)
 #pragma unroll
-      for (int cb = 0; cb < NCB; ++cb) {
-        acc_re[rb][cb] = v4{0, 0, 0, 0};
-        acc_im[rb][cb] = v4{0, 0, 0, 0};
-      }
+      for (int cb = 0; cb < NCB; ++cb) acc[rb][cb].zero();
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
       if (!FULL && 4 * s >= Din) break;
-      S nim[RB];
+      typename CAcc<S>::B bt[RB];
 #pragma unroll
-      for (int rb = 0; rb < RB; ++rb) nim[rb] = -im[s][rb];
+      for (int rb = 0; rb < RB; ++rb) bt[rb] = CAcc<S>::prep(re[s][rb], im[s][rb]);
 #pragma unroll
       for (int cb = 0; cb < NCB; ++cb) {
         const S bre = er[4 * s + lk][cb * 16 + li];
         const S bim = ei[4 * s + lk][cb * 16 + li];
 #pragma unroll
-        for (int rb = 0; rb < RB; ++rb) {
-          acc_re[rb][cb] = M::op(bre, re[s][rb], acc_re[rb][cb]);
-          acc_re[rb][cb] = M::op(bim, nim[rb], acc_re[rb][cb]);
-          acc_im[rb][cb] = M::op(bim, re[s][rb], acc_im[rb][cb]);
-          acc_im[rb][cb] = M::op(bre, im[s][rb], acc_im[rb][cb]);
-        }
+        for (int rb = 0; rb < RB; ++rb) acc[rb][cb].mac(bre, bim, bt[rb]);
       }
     }
     const int64_t row0 = tile * ROWS + (int64_t)wave * 16 * RB;
@@ -341,8 +374,8 @@ __device__ __forceinline__ void mode_product_tiles_pf(
           const int j = cb * 16 + M::row(lk, v);
           if (!FULL && j >= Dout) continue;
           C o;
-          o.x = acc_re[rb][cb][v] * scale;
-          o.y = acc_im[rb][cb][v] * scale;
+          o.x = acc[rb][cb].out_re(v) * scale;
+          o.y = acc[rb][cb].out_im(v) * scale;
           ob[L * (int64_t)j] = o;
           if (norm_out) nrm += (double)o.x * (double)o.x + (double)o.y * (double)o.y;
         }
@@ -448,24 +481,17 @@ __device__ __forceinline__ void mode_product_tiles_buf(
     }
   };
   auto compute_store = [&](int64_t tile, const S *re, const S *im) {
-    v4 acc_re[4], acc_im[4];
+    CAcc<S> acc[4];
 #pragma unroll
-    for (int cb = 0; cb < 4; ++cb) {
-      acc_re[cb] = v4{0, 0, 0, 0};
-      acc_im[cb] = v4{0, 0, 0, 0};
-    }
+    for (int cb = 0; cb < 4; ++cb) acc[cb].zero();
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
-      // (-Ei) Ti = Ei (-Ti) exactly: one negation per step instead of per block
-      const S nim = -im[s];
+      const typename CAcc<S>::B bt = CAcc<S>::prep(re[s], im[s]);
 #pragma unroll
       for (int cb = 0; cb < 4; ++cb) {
         const S bre = er[4 * s + lk][cb * 16 + li];
         const S bim = ei[4 * s + lk][cb * 16 + li];
-        acc_re[cb] = M::op(bre, re[s], acc_re[cb]);
-        acc_re[cb] = M::op(bim, nim, acc_re[cb]);
-        acc_im[cb] = M::op(bim, re[s], acc_im[cb]);
-        acc_im[cb] = M::op(bre, im[s], acc_im[cb]);
+        acc[cb].mac(bre, bim, bt);
       }
     }
     // register v of lane (li, lk): column j = 16 cb + row(lk, v); the lane
@@ -477,8 +503,8 @@ __device__ __forceinline__ void mode_product_tiles_buf(
 #pragma unroll
       for (int cb = 0; cb < 4; ++cb) {
         C o;
-        o.x = acc_re[cb][v];
-        o.y = acc_im[cb][v];
+        o.x = acc[cb].out_re(v);
+        o.y = acc[cb].out_im(v);
         buf_store<S>(o, rO, vo, Lb * (unsigned)(cb * 16 + (sizeof(S) == 4 ? v : 4 * v)));
       }
     }
@@ -921,11 +947,17 @@ __global__ __launch_bounds__(256, OccPM<S>::v) void k_product_message(
     }
   };
 
-  v4 m_re[4], m_im[4];
+  // FP64: m_re / m_im are Re and Im of M^T, four real products per complex
+  // one.  FP32: Gauss's three products (as in CAcc<float>), m_re = Q1 =
+  // sum Vr Tr, m_im = Q2 = sum Vi Ti, m_s = Q3 = sum (Vr - Vi)(Tr + Ti), with
+  // Re = Q1 + Q2 and Im = (Q3 - Q1) + Q2 for conj(V) T.
+  constexpr bool G3 = sizeof(S) == 4;
+  v4 m_re[4], m_im[4], m_s[G3 ? 4 : 1];
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     m_re[c] = v4{0, 0, 0, 0};
     m_im[c] = v4{0, 0, 0, 0};
+    if constexpr (G3) m_s[c] = v4{0, 0, 0, 0};
   }
   int64_t o = n0;
   if (o < nO) {
@@ -938,11 +970,15 @@ __global__ __launch_bounds__(256, OccPM<S>::v) void k_product_message(
     commit_v();
     __syncthreads();
     if (more) load_v(o + nb);  // in flight until the next commit
-    v4 t_re[4], t_im[4];
+    // FP64: t_re / t_im are Re and Im of T_o^T.  FP32: P1 = sum Er Pr (t_re),
+    // P2 = sum Ei Pi (t_im), P3 = sum (Er + Ei)(Pr + Pi) (t_s); Re = P1 - P2,
+    // Im = (P3 - P1) - P2.
+    v4 t_re[4], t_im[4], t_s[G3 ? 4 : 1];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       t_re[c] = v4{0, 0, 0, 0};
       t_im[c] = v4{0, 0, 0, 0};
+      if constexpr (G3) t_s[c] = v4{0, 0, 0, 0};
     }
     // the two products at raised wave priority, so that the MFMA phase of one
     // workgroup is not held back by the other's loads and LDS staging on the
@@ -952,14 +988,25 @@ __global__ __launch_bounds__(256, OccPM<S>::v) void k_product_message(
 #pragma unroll
     for (int s = 0; s < 16; ++s) {
       const int a = pm_a(s, lk);
-      const S pr = pc[s].x, pi = pc[s].y, npi = -pc[s].y;
+      if constexpr (G3) {
+        const S pr = pc[s].x, pi = pc[s].y, ps = pc[s].x + pc[s].y;
 #pragma unroll
-      for (int cb = 0; cb < 4; ++cb) {
-        const C e = Es[a][cb * 16 + li];
-        t_re[cb] = M::op(e.x, pr, t_re[cb]);
-        t_im[cb] = M::op(e.x, pi, t_im[cb]);
-        t_re[cb] = M::op(e.y, npi, t_re[cb]);
-        t_im[cb] = M::op(e.y, pr, t_im[cb]);
+        for (int cb = 0; cb < 4; ++cb) {
+          const C e = Es[a][cb * 16 + li];
+          t_re[cb] = M::op(e.x, pr, t_re[cb]);
+          t_im[cb] = M::op(e.y, pi, t_im[cb]);
+          t_s[cb] = M::op(e.x + e.y, ps, t_s[cb]);
+        }
+      } else {
+        const S pr = pc[s].x, pi = pc[s].y, npi = -pc[s].y;
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) {
+          const C e = Es[a][cb * 16 + li];
+          t_re[cb] = M::op(e.x, pr, t_re[cb]);
+          t_im[cb] = M::op(e.x, pi, t_im[cb]);
+          t_re[cb] = M::op(e.y, npi, t_re[cb]);
+          t_im[cb] = M::op(e.y, pr, t_im[cb]);
+        }
       }
     }
     if (more) load_p(o + nb);  // registers free again; in flight during the second product
@@ -970,14 +1017,26 @@ __global__ __launch_bounds__(256, OccPM<S>::v) void k_product_message(
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int a2 = cb * 16 + M::row(lk, v);
-        const S tr = t_re[cb][v], ti = t_im[cb][v], ntr = -t_re[cb][v];
+        if constexpr (G3) {
+          const S p1 = t_re[cb][v], p2 = t_im[cb][v];
+          const S tr = p1 - p2, ti = (t_s[cb][v] - p1) - p2, ts = tr + ti;
 #pragma unroll
-        for (int c2 = 0; c2 < 4; ++c2) {
-          const C x = Vs[a2][c2 * 16 + li];
-          m_re[c2] = M::op(x.x, tr, m_re[c2]);
-          m_im[c2] = M::op(x.x, ti, m_im[c2]);
-          m_re[c2] = M::op(x.y, ti, m_re[c2]);
-          m_im[c2] = M::op(x.y, ntr, m_im[c2]);
+          for (int c2 = 0; c2 < 4; ++c2) {
+            const C x = Vs[a2][c2 * 16 + li];
+            m_re[c2] = M::op(x.x, tr, m_re[c2]);
+            m_im[c2] = M::op(x.y, ti, m_im[c2]);
+            m_s[c2] = M::op(x.x - x.y, ts, m_s[c2]);
+          }
+        } else {
+          const S tr = t_re[cb][v], ti = t_im[cb][v], ntr = -t_re[cb][v];
+#pragma unroll
+          for (int c2 = 0; c2 < 4; ++c2) {
+            const C x = Vs[a2][c2 * 16 + li];
+            m_re[c2] = M::op(x.x, tr, m_re[c2]);
+            m_im[c2] = M::op(x.x, ti, m_im[c2]);
+            m_re[c2] = M::op(x.y, ti, m_re[c2]);
+            m_im[c2] = M::op(x.y, ntr, m_im[c2]);
+          }
         }
       }
     }
@@ -995,8 +1054,14 @@ __global__ __launch_bounds__(256, OccPM<S>::v) void k_product_message(
         const int i2 = c2 * 16 + M::row(lk, v);
         if (!FULL && i2 >= Dk) continue;
         C x;
-        x.x = m_re[c2][v];
-        x.y = m_im[c2][v];
+        if constexpr (G3) {
+          const S q1 = m_re[c2][v], q2 = m_im[c2][v];
+          x.x = q1 + q2;
+          x.y = (m_s[c2][v] - q1) + q2;
+        } else {
+          x.x = m_re[c2][v];
+          x.y = m_im[c2][v];
+        }
         out[i + (int64_t)Dk * i2] = x;
       }
     }
