@@ -251,14 +251,12 @@ hbp_status hbp_peps_set_absorb_mode(hbp_peps *p, int mode);
  * ~64); MERGED is PER_SITE with the last mode product of each outgoing
  * message computed inside the message kernel (bonds in (32, 64] on both of
  * the two bonds involved; other messages as PER_SITE), which saves writing
- * and re-reading that product; GROUPED runs each level of the sweep for all sites (or a group of
- * sites that fits a workspace of min(free/4, 16 GiB)) as one launch per
- * column-block count - five to ten launches per sweep instead of ~12 per
- * site; needs site degree <= 4 and bond dims <= 32.  AUTO (default) is
- * MERGED (which falls back to PER_SITE launches wherever bonds are outside
- * (32, 64]).  GROUPED is disabled: the call returns HBP_ERR_UNSUPPORTED (its
- * first GPU run left the device unusable and the cause is not found).  No
- * reference counterpart. */
+ * and re-reading that product.  AUTO (default) is MERGED (which falls back
+ * to PER_SITE launches wherever bonds are outside (32, 64]).  GROUPED (one
+ * launch per sweep level for all sites) is reserved and rejected with
+ * HBP_ERR_UNSUPPORTED: its first GPU run left the device unusable, the cause
+ * was not found and its code was taken out; hbp_peps_bp_run_batched covers
+ * the launch-bound small bonds instead.  No reference counterpart. */
 enum { HBP_SWEEP_AUTO = 0, HBP_SWEEP_PER_SITE = 1, HBP_SWEEP_GROUPED = 2, HBP_SWEEP_MERGED = 3 };
 hbp_status hbp_peps_set_sweep_kernel(hbp_peps *p, int kind);
 hbp_status hbp_peps_num_sites(const hbp_peps *p, int *n_sites, int *n_edges);

@@ -326,9 +326,9 @@ def test_lazy_and_eager_agree_on_merged_sweeps(ctx, precision):
 
 
 def test_grouped_sweep_is_rejected(ctx):
-    """The grouped sweep is switched off at the API (its first GPU run left the
-    device unusable and the cause is not found): asking for it is an error and
-    the PEPS keeps its sweep kernel."""
+    """The grouped sweep is switched off at the API and its code removed (its
+    first GPU run left the device unusable and the cause was not found):
+    asking for it is an error and the PEPS keeps its sweep kernel."""
     from tnbp_amd import circuits as pc
     from tnbp_amd.peps import HBP_SWEEP_GROUPED, InitTensorProductState
     announce("grouped sweep rejected")
@@ -338,8 +338,8 @@ def test_grouped_sweep_is_rejected(ctx):
     p.close()
 
 
-@pytest.mark.skip(reason="grouped sweep disabled: its first GPU run left the device unusable "
-                         "and the cause is not yet found")
+@pytest.mark.skip(reason="grouped sweep disabled and its code removed: its first GPU run left "
+                         "the device unusable and the cause was not found")
 @pytest.mark.parametrize("precision,tol", [(0, 1e-12), (1, 1e-6)])
 def test_grouped_sweep_matches_per_site(ctx, precision, tol):
     """The grouped sweep (one launch per level for all sites) against the

@@ -243,36 +243,6 @@ void launch_reduce_partials(hipStream_t st, const double2 *partials, int nchunks
 void launch_reduce_partials(hipStream_t st, const float2 *partials, int nchunks, int n,
                             double2 *M);
 
-// ---- grouped launches: many independent products in one kernel launch.
-// Jobs of one launch sit contiguously in a device table, ordered by
-// first_block (relative to the launch), and all have the same column-block
-// count (ceil(Dout / 16) for mode products, ceil(D2 / 16) for messages).
-// Mode product Out = T x E on a (L, Din, R) view, Q = L R rows, E(i, j) =
-// E[i + Din j], scale 1; the job walks its tiles with nblocks workgroups.
-struct MpJob {
-  const void *T;
-  const double2 *E;
-  void *Out;
-  int64_t L, Q;
-  int Din, Dout;
-  int64_t first_block, nblocks;
-};
-// Message partials M_c[i, j] = sum over rows of chunk c of T[q, i] conj V[q, j]
-// for (L, D1, R) / (L, D2, R) views; blocks = nch * ceil(D1 / 64).
-struct MsgGJob {
-  const void *T, *V;
-  double2 *partials;
-  int64_t L, Q, chunk, nch;
-  int D1, D2;
-  int64_t first_block;
-};
-// M = sum of nch partials of n entries; blocks = ceil(n / 16).
-struct RedJob {
-  const double2 *partials;
-  double2 *M;
-  int nch, n;
-  int64_t first_block;
-};
 // Batched small-bond sweep (hbp_batched.hip): one work item per site of a
 // PEPS in the batch; all of the site's outgoing raw messages (unnormalised,
 // complex double, D x D column-major) from its tensor V (bonds then the
@@ -305,14 +275,6 @@ int launch_sweep_small_tuned(hipStream_t st, const SmallSiteJob *jobs_dev, int n
 // the autotuner's choices so far, one line per shape class
 std::string autotune_report();
 int mode_product_col_blocks(int Dout);
-int64_t mode_product_tiles_of(int Dout, int64_t Q);
-template <typename S>
-void launch_mode_product_grouped(hipStream_t st, const MpJob *jobs_dev, int njobs, int ncb,
-                                 int64_t blocks);
-template <typename S>
-void launch_message_grouped(hipStream_t st, const MsgGJob *jobs_dev, int njobs, int ncb,
-                            int64_t blocks);
-void launch_reduce_grouped(hipStream_t st, const RedJob *jobs_dev, int njobs, int64_t blocks);
 constexpr int MSG_ROWS_PER_STAGE = 32;
 constexpr int MSG_OUT_ROWS = 64;
 constexpr int RED_ENTRIES = 16;

@@ -549,26 +549,6 @@ __global__ __launch_bounds__(256, OCC) void k_mode_product(
                                         blockIdx.x, gridDim.x, er, ei, red);
 }
 
-// Many independent mode products in one launch: job j owns blocks
-// [first_block, first_block + nblocks) and walks its own tiles with them.
-template <typename S, int NCB, int RB>
-__global__ __launch_bounds__(256, Occ<S>::v) void k_mode_product_grouped(
-    const MpJob *__restrict__ jobs, int njobs) {
-  __shared__ S er[MP_KC][16 * NCB + 1];
-  __shared__ S ei[MP_KC][16 * NCB + 1];
-  __shared__ double red[8];
-  int lo = 0, hi = njobs - 1;
-  while (lo < hi) {  // last job whose first_block <= blockIdx.x
-    const int mid = (lo + hi + 1) >> 1;
-    if (jobs[mid].first_block <= (int64_t)blockIdx.x) lo = mid; else hi = mid - 1;
-  }
-  const MpJob j = jobs[lo];
-  using C = typename Cplx<S>::type;
-  mode_product_tiles<S, NCB, RB, 0>((const C *)j.T, j.E, (C *)j.Out, j.L, j.Din, j.Dout, j.Q, 0,
-                                 (S)1, nullptr, (int64_t)blockIdx.x - j.first_block, j.nblocks,
-                                 er, ei, red);
-}
-
 // --------------------------------------------------------------- message
 constexpr int MSG_QS = MSG_ROWS_PER_STAGE;  // rows per LDS stage
 constexpr int MSG_IB = MSG_OUT_ROWS;        // output rows (i) per block (4 waves x 16)
@@ -759,30 +739,6 @@ __global__ __launch_bounds__(256, OCC) void k_message_partials(
   message_chunk<S, NCB>(T, V, partials + (int64_t)blockIdx.x * D1 * D2, L, D1, D2, qbeg,
                         min(Q, qbeg + chunk), blockIdx.y * MSG_IB, ldT_R, ldV_R, conjV, wL, wR,
                         Da, Sb, tre, tim, vre, vim);
-}
-
-// Many messages in one launch (conj V, no weights): job j owns blocks
-// [first_block, first_block + nch * nib), block b of it takes row chunk
-// b % nch and output rows 64 (b / nch) ...
-template <typename S, int NCB>
-__global__ __launch_bounds__(256, 2) void k_message_grouped(const MsgGJob *__restrict__ jobs,
-                                                            int njobs) {
-  __shared__ S tre[MSG_QS][MSG_IB + 1], tim[MSG_QS][MSG_IB + 1];
-  __shared__ S vre[MSG_QS][16 * NCB + 1], vim[MSG_QS][16 * NCB + 1];
-  int lo = 0, hi = njobs - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (jobs[mid].first_block <= (int64_t)blockIdx.x) lo = mid; else hi = mid - 1;
-  }
-  const MsgGJob j = jobs[lo];
-  const int64_t b = (int64_t)blockIdx.x - j.first_block;
-  const int64_t c = b % j.nch;
-  const int ib = (int)(b / j.nch);
-  const int64_t qbeg = c * j.chunk;
-  using C = typename Cplx<S>::type;
-  message_chunk<S, NCB>((const C *)j.T, (const C *)j.V, j.partials + c * j.D1 * j.D2, j.L, j.D1,
-                        j.D2, qbeg, min(j.Q, qbeg + j.chunk), ib * MSG_IB, j.L * j.D1,
-                        j.L * j.D2, 1, nullptr, nullptr, max(j.D1, j.D2), 0, tre, tim, vre, vim);
 }
 
 // ------------------------------------------------ physical-index message
@@ -1322,17 +1278,6 @@ __global__ __launch_bounds__(RED_G * RED_E) void k_reduce_batch(RedBatch b) {
     reduce_entries<float2, RED_G>((const float2 *)j.partials, j.nch, j.n, j.M, blockIdx.x);
   else
     reduce_entries<double2, RED_G>((const double2 *)j.partials, j.nch, j.n, j.M, blockIdx.x);
-}
-
-__global__ __launch_bounds__(256) void k_reduce_grouped(const RedJob *__restrict__ jobs,
-                                                         int njobs) {
-  int lo = 0, hi = njobs - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (jobs[mid].first_block <= (int64_t)blockIdx.x) lo = mid; else hi = mid - 1;
-  }
-  const RedJob j = jobs[lo];
-  reduce_entries<double2, 16>(j.partials, j.nch, j.n, j.M, (int64_t)blockIdx.x - j.first_block);
 }
 
 // ------------------------------------------------------------ elementwise
@@ -1950,59 +1895,6 @@ template void launch_mode_product<float>(hipStream_t, const void *, const double
                                          int *);
 
 int mode_product_col_blocks(int Dout) { return (Dout + 15) / 16; }
-int64_t mode_product_tiles_of(int Dout, int64_t Q) {
-  const int64_t rows = mode_product_col_blocks(Dout) <= 4 ? 4 * 16 * 2 : 4 * 16;
-  return (Q + rows - 1) / rows;
-}
-
-template <typename S>
-void launch_mode_product_grouped(hipStream_t st, const MpJob *jobs, int njobs, int ncb,
-                                 int64_t blocks) {
-  if (njobs <= 0 || blocks <= 0) return;
-  HBP_REQUIRE(ncb >= 1 && ncb <= 8, HBP_ERR_UNSUPPORTED, "grouped mode product: output bond > 128");
-  const dim3 g((unsigned)blocks), b(256);
-  switch (ncb) {
-    case 1: hipLaunchKernelGGL((k_mode_product_grouped<S, 1, 2>), g, b, 0, st, jobs, njobs); break;
-    case 2: hipLaunchKernelGGL((k_mode_product_grouped<S, 2, 2>), g, b, 0, st, jobs, njobs); break;
-    case 3: hipLaunchKernelGGL((k_mode_product_grouped<S, 3, 2>), g, b, 0, st, jobs, njobs); break;
-    case 4: hipLaunchKernelGGL((k_mode_product_grouped<S, 4, 2>), g, b, 0, st, jobs, njobs); break;
-    case 5: hipLaunchKernelGGL((k_mode_product_grouped<S, 5, 1>), g, b, 0, st, jobs, njobs); break;
-    case 6: hipLaunchKernelGGL((k_mode_product_grouped<S, 6, 1>), g, b, 0, st, jobs, njobs); break;
-    case 7: hipLaunchKernelGGL((k_mode_product_grouped<S, 7, 1>), g, b, 0, st, jobs, njobs); break;
-    default: hipLaunchKernelGGL((k_mode_product_grouped<S, 8, 1>), g, b, 0, st, jobs, njobs); break;
-  }
-  HBP_HIP(hipGetLastError());
-}
-template void launch_mode_product_grouped<double>(hipStream_t, const MpJob *, int, int, int64_t);
-template void launch_mode_product_grouped<float>(hipStream_t, const MpJob *, int, int, int64_t);
-
-template <typename S>
-void launch_message_grouped(hipStream_t st, const MsgGJob *jobs, int njobs, int ncb,
-                            int64_t blocks) {
-  if (njobs <= 0 || blocks <= 0) return;
-  HBP_REQUIRE(ncb >= 1 && ncb <= 8, HBP_ERR_UNSUPPORTED, "grouped message: bond > 128");
-  const dim3 g((unsigned)blocks), b(256);
-  switch (ncb) {
-    case 1: hipLaunchKernelGGL((k_message_grouped<S, 1>), g, b, 0, st, jobs, njobs); break;
-    case 2: hipLaunchKernelGGL((k_message_grouped<S, 2>), g, b, 0, st, jobs, njobs); break;
-    case 3: hipLaunchKernelGGL((k_message_grouped<S, 3>), g, b, 0, st, jobs, njobs); break;
-    case 4: hipLaunchKernelGGL((k_message_grouped<S, 4>), g, b, 0, st, jobs, njobs); break;
-    case 5: hipLaunchKernelGGL((k_message_grouped<S, 5>), g, b, 0, st, jobs, njobs); break;
-    case 6: hipLaunchKernelGGL((k_message_grouped<S, 6>), g, b, 0, st, jobs, njobs); break;
-    case 7: hipLaunchKernelGGL((k_message_grouped<S, 7>), g, b, 0, st, jobs, njobs); break;
-    default: hipLaunchKernelGGL((k_message_grouped<S, 8>), g, b, 0, st, jobs, njobs); break;
-  }
-  HBP_HIP(hipGetLastError());
-}
-template void launch_message_grouped<double>(hipStream_t, const MsgGJob *, int, int, int64_t);
-template void launch_message_grouped<float>(hipStream_t, const MsgGJob *, int, int, int64_t);
-
-void launch_reduce_grouped(hipStream_t st, const RedJob *jobs, int njobs, int64_t blocks) {
-  if (njobs <= 0 || blocks <= 0) return;
-  hipLaunchKernelGGL(k_reduce_grouped, dim3((unsigned)blocks), dim3(256), 0, st, jobs, njobs);
-  HBP_HIP(hipGetLastError());
-}
-
 int64_t message_num_chunks(int64_t Q) {
   // enough workgroups to fill 256 CUs several times over: up to 1024 chunks,
   // at least MSG_QS rows each

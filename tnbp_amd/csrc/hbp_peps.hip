@@ -1250,255 +1250,6 @@ static void sweep_raw(Ctx *ctx, Peps *p, double2 *raw, const std::vector<int64_t
   if (red) red->flush();
 }
 
-// ---------------------------------------------------------- grouped sweep
-// Every site's messages with a handful of launches for the whole lattice
-// (or a group of sites that fits the workspace): with the bonds of a site
-// split in halves H1 = [0, h) and H2 = [h, z), h = z / 2, and A the (absorbed)
-// site tensor,
-//   level 1  X1 = A x_{H1[0]} E,          X2 = A x_{H2[0]} E
-//   level 2  P1 = X1 x_{H1[1]} E,         P2 = X2 x_{H2[1]} E   (halves of 2)
-//   level 3  T_k = P1 x_{H2 \ k} E (k in H2),  T_k = P2 x_{H1 \ k} E (k in H1)
-//   level 4  M_k = <T_k, conj A> over all bonds but k (split-K partials)
-//   level 5  fixed-order reduction of the partials into the raw messages
-// (z <= 4; a half with one bond makes P = X and T_k = P of the other half).
-// Each level is one grouped launch per column-block count.  The same mode
-// products as the per-site path in the same order per message; the message
-// partials use fewer, longer chunks, which changes rounding only.
-static bool grouped_eligible(const Peps *p) {
-  for (int s = 0; s < p->n_sites; ++s)
-    if (p->z(s) > 4) return false;
-  for (int m = 0; m < p->n_edges; ++m)
-    if (p->bond_dim[m] > 32) return false;
-  return true;
-}
-
-template <typename S>
-static void sweep_raw_grouped(Ctx *ctx, Peps *p, double2 *raw, const std::vector<int64_t> &raw_off) {
-  const int eb = p->elem_bytes();
-  size_t free_b = 0, total_b = 0;
-  HBP_HIP(hipMemGetInfo(&free_b, &total_b));
-  const size_t budget = std::max<size_t>(size_t(256) << 20, std::min(free_b / 4, size_t(16) << 30));
-  int s0 = 0;
-  while (s0 < p->n_sites) {
-    // ---- pick the group [s0, s1) and lay out its workspace
-    struct SiteWork {
-      const void *A;
-      void *buf[6];
-    };
-    std::vector<SiteWork> sw;
-    std::vector<size_t> ws_off;
-    size_t ws = 0;
-    int s1 = s0;
-    while (s1 < p->n_sites) {
-      const int z = p->z(s1);
-      const size_t nb = p->site_size(s1) * eb;
-      // buffers a site takes below: X1, X2 (z >= 2); P2 and two targets (z = 3);
-      // P1, P2 and two targets, reusing X1 and X2 for the other two (z = 4)
-      static const int kBufs[5] = {0, 0, 2, 5, 6};
-      const int nbuf = (p->pending[s1] ? 1 : 0) + kBufs[z];
-      const size_t need = (size_t)nbuf * ((nb + 255) / 256 * 256);
-      if (s1 > s0 && ws + need > budget) break;
-      ws_off.push_back(ws);
-      ws += need;
-      ++s1;
-    }
-    char *wsp = (char *)ctx->alloc(std::max<size_t>(ws, 256));
-    std::vector<MpJob> mp[3];
-    std::vector<MsgGJob> msg;
-    std::vector<RedJob> red;
-    std::vector<size_t> part_off;
-    size_t part_total = 0;
-    int32_t sh[HBP_MAX_ORDER];
-    for (int s = s0; s < s1; ++s) {
-      const int z = p->z(s), order = z + 1;
-      p->shape(s, sh);
-      const size_t nb = (p->site_size(s) * eb + 255) / 256 * 256;
-      char *base = wsp + ws_off[s - s0];
-      int nbuf = 0;
-      auto take = [&]() { return (void *)(base + (nbuf++) * nb); };
-      const void *A = p->V[s];
-      if (p->pending[s]) {
-        void *a = take();
-        materialize(ctx, p, s, a);
-        A = a;
-      }
-      auto job = [&](const void *T, void *Out, int k) {
-        MpJob j{};
-        j.T = T;
-        j.E = p->E + p->E_off[p->in_msg(s, k)];
-        j.Out = Out;
-        int64_t L = 1, R = 1;
-        for (int d = 0; d < k; ++d) L *= sh[d];
-        for (int d = k + 1; d < order; ++d) R *= sh[d];
-        j.L = L;
-        j.Q = L * R;
-        j.Din = j.Dout = sh[k];
-        return j;
-      };
-      const int h = z / 2;
-      // P1 = A x_{H1}, P2 = A x_{H2} (P2 only when H1 has targets)
-      const void *P1 = A, *P2 = A;
-      void *X1 = nullptr, *X2 = nullptr;
-      if (h >= 1) {
-        X1 = take();
-        mp[0].push_back(job(A, X1, 0));
-        P1 = X1;
-        X2 = take();
-        mp[0].push_back(job(A, X2, h));
-        P2 = X2;
-      }
-      if (h == 2) {
-        void *y = take();
-        mp[1].push_back(job(X1, y, 1));
-        P1 = y;
-      }
-      if (z - h == 2 && h >= 1) {
-        void *y = take();
-        mp[1].push_back(job(X2, y, h + 1));
-        P2 = y;
-      }
-      // targets: T_k for k in H2 from P1, for k in H1 from P2
-      std::vector<const void *> Tk(z, nullptr);
-      if (z - h == 1) Tk[h] = P1;
-      if (z - h == 2) {
-        // X1 is free once P1 exists (h == 2) or P1 is X1 (h == 1): reuse for T_h+1
-        void *b0 = h == 2 ? X1 : take();
-        void *b1 = take();
-        mp[2].push_back(job(P1, b0, h + 1));
-        Tk[h] = b0;
-        mp[2].push_back(job(P1, b1, h));
-        Tk[h + 1] = b1;
-      }
-      if (h == 1) Tk[0] = P2;
-      if (h == 2) {
-        void *b0 = X2 == P2 ? take() : X2;
-        void *b1 = take();
-        mp[2].push_back(job(P2, b0, 1));
-        Tk[0] = b0;
-        mp[2].push_back(job(P2, b1, 0));
-        Tk[1] = b1;
-      }
-      for (int k = 0; k < z; ++k) {
-        MsgGJob m{};
-        int64_t L = 1, R = 1;
-        for (int d = 0; d < k; ++d) L *= sh[d];
-        for (int d = k + 1; d < order; ++d) R *= sh[d];
-        const int64_t Q = L * R;
-        int64_t nch = std::min<int64_t>(32, (Q + 511) / 512);
-        int64_t chunk = (Q + nch - 1) / nch;
-        chunk = (chunk + MSG_ROWS_PER_STAGE - 1) / MSG_ROWS_PER_STAGE * MSG_ROWS_PER_STAGE;
-        nch = (Q + chunk - 1) / chunk;
-        m.T = Tk[k];
-        m.V = A;
-        m.L = L;
-        m.Q = Q;
-        m.chunk = chunk;
-        m.nch = nch;
-        m.D1 = m.D2 = sh[k];
-        part_off.push_back(part_total);
-        part_total += (size_t)nch * sh[k] * sh[k];
-        msg.push_back(m);
-        RedJob r{};
-        r.nch = (int)nch;
-        r.n = sh[k] * sh[k];
-        r.M = raw + raw_off[p->out_msg(s, k)];
-        red.push_back(r);
-      }
-      HBP_REQUIRE(nbuf * nb <= (ws_off.size() > (size_t)(s - s0 + 1) ? ws_off[s - s0 + 1] : ws) -
-                                   ws_off[s - s0],
-                  HBP_ERR_LOGIC, "grouped sweep: workspace layout");
-    }
-    double2 *parts = (double2 *)ctx->alloc(std::max<size_t>(part_total, 1) * 16);
-    for (size_t i = 0; i < msg.size(); ++i) {
-      msg[i].partials = parts + part_off[i];
-      red[i].partials = parts + part_off[i];
-    }
-    // ---- bucket by column blocks, number the blocks, one table upload
-    struct Launch {
-      int kind, ncb, njobs;
-      size_t off;  // byte offset of the first job in the table
-      int64_t blocks;
-      double flops, bytes;  // algorithmic work of the launch
-    };
-    std::vector<Launch> launches;
-    std::vector<char> table;
-    auto add_mp = [&](std::vector<MpJob> &v) {
-      for (int ncb = 1; ncb <= 2; ++ncb) {
-        std::vector<MpJob> b;
-        int64_t blocks = 0;
-        double fl = 0.0, by = 0.0;
-        for (MpJob j : v) {
-          if (mode_product_col_blocks(j.Dout) != ncb) continue;
-          fl += 8.0 * j.Q * j.Din * j.Dout;
-          by += (double)eb * j.Q * (double)(j.Din + j.Dout) + 16.0 * j.Din * j.Dout;
-          const int64_t tiles = mode_product_tiles_of(j.Dout, j.Q);
-          const int64_t tpb = std::max<int64_t>(1, std::min<int64_t>(8, tiles / 32));
-          j.nblocks = (tiles + tpb - 1) / tpb;
-          j.first_block = blocks;
-          blocks += j.nblocks;
-          b.push_back(j);
-        }
-        if (b.empty()) continue;
-        const size_t off = (table.size() + 63) / 64 * 64;
-        table.resize(off + b.size() * sizeof(MpJob));
-        std::memcpy(table.data() + off, b.data(), b.size() * sizeof(MpJob));
-        launches.push_back({0, ncb, (int)b.size(), off, blocks, fl, by});
-      }
-    };
-    for (int l = 0; l < 3; ++l) add_mp(mp[l]);
-    for (int ncb = 1; ncb <= 2; ++ncb) {
-      std::vector<MsgGJob> b;
-      int64_t blocks = 0;
-      double fl = 0.0, by = 0.0;
-      for (MsgGJob j : msg) {
-        if ((j.D2 + 15) / 16 != ncb) continue;
-        fl += 8.0 * j.Q * j.D1 * j.D2;
-        by += (double)eb * j.Q * (double)(j.D1 + j.D2);
-        j.first_block = blocks;
-        blocks += j.nch * ((j.D1 + MSG_OUT_ROWS - 1) / MSG_OUT_ROWS);
-        b.push_back(j);
-      }
-      if (b.empty()) continue;
-      const size_t off = (table.size() + 63) / 64 * 64;
-      table.resize(off + b.size() * sizeof(MsgGJob));
-      std::memcpy(table.data() + off, b.data(), b.size() * sizeof(MsgGJob));
-      launches.push_back({1, ncb, (int)b.size(), off, blocks, fl, by});
-    }
-    {
-      int64_t blocks = 0;
-      for (RedJob &r : red) {
-        r.first_block = blocks;
-        blocks += (r.n + RED_ENTRIES - 1) / RED_ENTRIES;
-      }
-      const size_t off = (table.size() + 63) / 64 * 64;
-      table.resize(off + red.size() * sizeof(RedJob));
-      std::memcpy(table.data() + off, red.data(), red.size() * sizeof(RedJob));
-      launches.push_back({2, 0, (int)red.size(), off, blocks, 0.0, 0.0});
-    }
-    char *dtab = (char *)ctx->alloc(std::max<size_t>(table.size(), 64));
-    ctx->upload(dtab, table.data(), table.size());
-    for (const Launch &L : launches) {
-      if (L.kind == 0) {
-        const int pr = ctx->prof_begin(HBP_PROF_MODE_PRODUCT, L.flops, L.bytes);
-        launch_mode_product_grouped<S>(ctx->stream, (const MpJob *)(dtab + L.off), L.njobs, L.ncb,
-                                       L.blocks);
-        ctx->prof_end(pr);
-      } else if (L.kind == 1) {
-        const int pr = ctx->prof_begin(HBP_PROF_MESSAGE, L.flops, L.bytes);
-        launch_message_grouped<S>(ctx->stream, (const MsgGJob *)(dtab + L.off), L.njobs, L.ncb,
-                                  L.blocks);
-        ctx->prof_end(pr);
-      } else {
-        launch_reduce_grouped(ctx->stream, (const RedJob *)(dtab + L.off), L.njobs, L.blocks);
-      }
-    }
-    ctx->release(dtab, std::max<size_t>(table.size(), 64));
-    ctx->release(parts, std::max<size_t>(part_total, 1) * 16);
-    ctx->release(wsp, std::max<size_t>(ws, 256));
-    s0 = s1;
-  }
-}
-
 // The reference's op sequence for every message, one TCI call at a time
 // (bp.h:128-175): T = copy(V); for every bond k: (k != target) T = T x_k E_in,
 // then normalize(T) as a norm pass and a scale pass; C = conj(copy(V));
@@ -1665,23 +1416,12 @@ static void sweep(Ctx *ctx, Peps *p, double2 *newE, bool with_residual, double *
     else
       sweep_raw_plain<float>(ctx, p, raw, p->E_off);
   } else {
-    bool grouped = false;
-    if (p->sweep_kernel == HBP_SWEEP_GROUPED) {
-      HBP_REQUIRE(grouped_eligible(p), HBP_ERR_UNSUPPORTED,
-                  "grouped sweep: needs site degree <= 4 and bond dims <= 32");
-      grouped = true;
-    }
     // AUTO runs the per-site launches with merged product-message kernels where
-    // the bonds allow; the grouped form cannot be requested at present
-    // (hbp_peps_set_sweep_kernel rejects it: the first GPU run that used it left
-    // the device unusable, cause not found)
-    if (p->precision == HBP_PREC_FP64) {
-      if (grouped) sweep_raw_grouped<double>(ctx, p, raw, p->E_off);
-      else sweep_raw<double>(ctx, p, raw, p->E_off);
-    } else {
-      if (grouped) sweep_raw_grouped<float>(ctx, p, raw, p->E_off);
-      else sweep_raw<float>(ctx, p, raw, p->E_off);
-    }
+    // the bonds allow (HBP_SWEEP_GROUPED is rejected by hbp_peps_set_sweep_kernel)
+    if (p->precision == HBP_PREC_FP64)
+      sweep_raw<double>(ctx, p, raw, p->E_off);
+    else
+      sweep_raw<float>(ctx, p, raw, p->E_off);
   }
   double *dist = (double *)ctx->alloc(nm * sizeof(double));
   hipLaunchKernelGGL(k_finalize_messages, dim3(nm), dim3(256), 0, ctx->stream, raw, newE,
@@ -2639,10 +2379,11 @@ hbp_status hbp_peps_set_sweep_kernel(hbp_peps *p, int kind) {
   HBP_REQUIRE(kind == HBP_SWEEP_AUTO || kind == HBP_SWEEP_PER_SITE || kind == HBP_SWEEP_GROUPED ||
                   kind == HBP_SWEEP_MERGED,
               HBP_ERR_INVALID_ARGUMENT, "peps: unknown sweep kernel");
-  // switched off until the fault of its first GPU run is explained
+  // the value stays reserved; its implementation was taken out
   HBP_REQUIRE(kind != HBP_SWEEP_GROUPED, HBP_ERR_UNSUPPORTED,
-              "peps: the grouped sweep is disabled (its first GPU run left the device unusable, "
-              "cause not found)");
+              "peps: the grouped sweep is disabled and removed (its first GPU run left the "
+              "device unusable, cause not found); hbp_peps_bp_run_batched batches sweeps of "
+              "bonds <= 16");
   p->sweep_kernel = kind;
   API_END
 }

@@ -71,8 +71,8 @@ class Peps:
         check(self.ctx.lib.hbp_peps_set_absorb_mode(self.h, int(mode)))
 
     def set_sweep_kernel(self, kind):
-        """HBP_SWEEP_AUTO (0), HBP_SWEEP_PER_SITE (1), HBP_SWEEP_GROUPED (2) or
-        HBP_SWEEP_MERGED (3)."""
+        """HBP_SWEEP_AUTO (0), HBP_SWEEP_PER_SITE (1) or HBP_SWEEP_MERGED (3);
+        HBP_SWEEP_GROUPED (2) is reserved and raises NotImplementedError."""
         check(self.ctx.lib.hbp_peps_set_sweep_kernel(self.h, int(kind)))
 
     def set_schedule(self, schedule):
