@@ -107,9 +107,13 @@ def test_footprint_model_scales(monkeypatch):
     big = serving.random_circuit_job(rng, 8, 2, 32, 4)
     fp64 = serving.random_circuit_job(rng, 8, 2, 32, 4, precision=HBP_PREC_FP64)
     assert serving.footprint_bytes(small) < serving.footprint_bytes(big) < serving.footprint_bytes(fp64)
-    # 16x16 chi=32 mixed kicked-Ising job: resident state ~4.3 GB plus D=64 work buffers
-    k = serving.kicked_job(16, 32, 1, 0.7, precision=HBP_PREC_MIXED)
+    # 16x16 chi=32 mixed kicked-Ising job at its steady state (bonds 32, absorbed
+    # 64 from the sixth step): resident state ~3.3 GB plus D=64 work buffers
+    k = serving.kicked_job(16, 32, 7, 0.7, precision=HBP_PREC_MIXED)
     assert 4e9 < serving.footprint_bytes(k) < 10e9
+    # one step from |0...0> never leaves bonds of 2
+    k1 = serving.kicked_job(16, 32, 1, 0.7, precision=HBP_PREC_MIXED)
+    assert serving.footprint_bytes(k1) < 0.5e9
 
 
 def test_small_jobs_absorb_eagerly(monkeypatch):
@@ -120,9 +124,30 @@ def test_small_jobs_absorb_eagerly(monkeypatch):
     # the eager footprint holds the absorbed lattice on top of the lazy one
     assert serving.footprint_bytes(small) > serving.absorbed_bytes(small)
     # 16x16 chi=32: the absorbed lattice (~52 GB mixed) is absorbed site by site
-    k = serving.kicked_job(16, 32, 1, 0.7, precision=HBP_PREC_MIXED)
+    k = serving.kicked_job(16, 32, 7, 0.7, precision=HBP_PREC_MIXED)
     assert serving.absorbed_bytes(k) > 40e9
     assert not serving.absorbs_eagerly(k)
+
+
+def test_bond_bounds_follow_the_gates(monkeypatch):
+    """Bond bounds: a bond grows by the TPO bond dimension of each layer that
+    puts a gate on it and is cut to chi after the layer."""
+    monkeypatch.undo()
+    from tnbp_amd import circuits as pc
+    edges = pc.square_lattice(3, 3)
+    n = 9
+    rzz = pc.rzz(0.4)
+    # layer 1: one gate on edge 0; layer 2: edge 0 again and edge 5; layer 3: edge 0
+    layers = [[(rzz, edges[0])], [(rzz, edges[0]), (rzz, edges[5])], [(rzz, edges[0])]]
+    job = serving.EstimatorJob(edges, [pc.layer_tpo(edges, g) for g in layers], chi=2,
+                               terms=[(1.0, "Z" + "I" * (n - 1))])
+    b = serving.bond_bounds(job)
+    assert [r[0] for r, a in b] == [1, 2, 2]          # resident bond 0 per layer
+    assert [a[0] for r, a in b] == [2, 4, 4]          # absorbed bond 0 per layer
+    assert [a[5] for r, a in b] == [1, 2, 2]
+    assert all(a[1] == 1 for r, a in b)
+    # the uniform chi * t model would have put every bond at 4
+    assert serving.footprint_bytes(job) < 200 << 20
 
 
 def test_no_device_fails_jobs_instead_of_hanging():

@@ -1833,11 +1833,19 @@ static void mp_variant(int v, hipStream_t st, const void *T, const double2 *E, v
       else
         HBP_REQUIRE(false, HBP_ERR_LOGIC, "mode product: no such variant");
       break;
-    default:
+    case 6:
       // FP32, whole tiles of 64 -> 64 columns with buffer operations (the host
-      // checks shape and size), three workgroups per CU
+      // checks shape and size), three workgroups per CU (168 registers, four of
+      // them spilled with the three-product accumulators)
       if constexpr (NCB == 4 && sizeof(S) == 4)
         mp_launch<S, NCB, 1, 18, B>(st, T, E, Out, L, Din, Dout, Q, e_trans, scale, parts, np);
+      else
+        HBP_REQUIRE(false, HBP_ERR_LOGIC, "mode product: no such variant");
+      break;
+    default:
+      // the same at two workgroups per CU: no spills, fewer waves in flight
+      if constexpr (NCB == 4 && sizeof(S) == 4)
+        mp_launch<S, NCB, 1, 18, 2>(st, T, E, Out, L, Din, Dout, Q, e_trans, scale, parts, np);
       else
         HBP_REQUIRE(false, HBP_ERR_LOGIC, "mode product: no such variant");
       break;
@@ -1864,10 +1872,13 @@ void launch_mode_product(hipStream_t st, const void *T, const double2 *E, void *
   // reproduce the other variants bitwise - its messages differed by ~1e-5 -
   // and is not offered)
   const bool buf = whole && sizeof(S) == 4 && Q * 64 * (int64_t)(2 * sizeof(S)) < ((int64_t)1 << 31);
-  int vars[7] = {0, 1, 2, 3, 0, 0, 0}, nv = 4;
+  int vars[8] = {0, 1, 2, 3, 0, 0, 0, 0}, nv = 4;
   if (Din <= MP_KC && sizeof(S) == 4) vars[nv++] = 4;
   if (full) vars[nv++] = 5;
-  if (buf) vars[nv++] = 6;
+  if (buf) {
+    vars[nv++] = 6;
+    vars[nv++] = 7;
+  }
   const TuneKey key{0, (int)sizeof(S), ncb, ilog2(Q),
                     (L == 1 ? 1 : 0) + 2 * (Din > MP_KC ? 1 : 0) + 4 * (full ? 1 : 0) +
                         8 * (Din <= MP_KC ? 1 : 0) + 16 * (buf ? 1 : 0)};

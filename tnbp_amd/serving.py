@@ -57,36 +57,59 @@ class EstimatorJob:
     bypassed: int = field(default=0, repr=False)
 
 
-def _degrees(edges):
-    deg = {}
-    for a, b in edges:
-        deg[a] = deg.get(a, 0) + 1
-        deg[b] = deg.get(b, 0) + 1
-    return deg
+def _incident(edges):
+    """Incident edges of every site in ascending edge order (graph.h:40-55)."""
+    inc = {}
+    for m, (a, b) in enumerate(edges):
+        inc.setdefault(a, []).append(m)
+        inc.setdefault(b, []).append(m)
+    return inc
 
 
-# A job whose absorbed lattice (every bond at chi * t) takes at most this many
-# bytes absorbs each layer eagerly: its sites are then absorbed once per layer
-# instead of once per BP sweep, which at small bond dimensions saves more host
-# time (launches) than the memory is worth.
+def bond_bounds(job: EstimatorJob):
+    """Upper bounds of every bond's dimension per circuit layer, from
+    |0...0> (bonds of 1) and each layer's TPO bond dimensions t: while the
+    layer's BP loop runs a bond is at most D t (absorbed), after its
+    truncation at most min(chi, D t) (sv_min may cut further, never less).
+    Returns [(resident, absorbed)] per layer (lists over edges)."""
+    inc = _incident(job.edges)
+    sites = sorted(inc)
+    D = [1] * len(job.edges)
+    out = []
+    for tpo in job.tpos:
+        t = [1] * len(job.edges)
+        for s, O in zip(sites, tpo):
+            if O.ndim > 2:
+                for k, m in enumerate(inc[s]):
+                    t[m] = max(t[m], O.shape[k])
+        absorbed = [d * x for d, x in zip(D, t)]
+        out.append((D, absorbed))
+        D = [min(job.chi, x) for x in absorbed]
+    return out
+
+
+# A job whose absorbed lattice takes at most this many bytes absorbs each
+# layer eagerly: its sites are then absorbed once per layer instead of once per
+# BP sweep, which at small bond dimensions saves more host time (launches)
+# than the memory is worth.
 EAGER_ABSORB_MAX_BYTES = 512 << 20
 
 
-def _sizes(job: EstimatorJob):
+def _site_bytes(job, dims):
+    """Bytes of every site tensor with bond dimensions `dims` (physical 2)."""
     elem = 16 if job.precision == HBP_PREC_FP64 else 8
-    deg = _degrees(job.edges)
-    t = 1
-    for tpo in job.tpos:
-        for O in tpo:
-            if O.ndim > 2:
-                t = max(t, max(O.shape[:-2]))
-    return elem, deg, job.chi * t
+    out = {}
+    for s, ms in _incident(job.edges).items():
+        n = 2
+        for m in ms:
+            n *= dims[m]
+        out[s] = n * elem
+    return out
 
 
 def absorbed_bytes(job: EstimatorJob) -> int:
-    """Bytes of the lattice with every bond at D = chi * t."""
-    elem, deg, D = _sizes(job)
-    return sum(D ** z * 2 * elem for z in deg.values())
+    """Bytes of the largest absorbed lattice over the job's layers."""
+    return max((sum(_site_bytes(job, ab).values()) for _, ab in bond_bounds(job)), default=0)
 
 
 def absorbs_eagerly(job: EstimatorJob) -> bool:
@@ -95,30 +118,47 @@ def absorbs_eagerly(job: EstimatorJob) -> bool:
 
 def pair_cache_bytes(job: EstimatorJob) -> int:
     """Bytes of the merged sweep's pair-front site copies (sites of degree 3
-    and 4 at D = chi * t, when the merged kernels take D, i.e. 32 < D <= 64,
-    and the job absorbs eagerly; 0 otherwise)."""
-    elem, deg, D = _sizes(job)
-    if job.mode == 1 or not absorbs_eagerly(job) or not 32 < D <= 64:
+    and 4 whose target pair, bonds h and h + 1 with h = z // 2, both lie in
+    (32, 64] after absorption, when the job absorbs eagerly), at the largest
+    layer; 0 otherwise."""
+    if job.mode == 1 or not absorbs_eagerly(job):
         return 0
-    return sum(D ** z * 2 * elem for z in deg.values() if z in (3, 4))
+    inc = _incident(job.edges)
+    best = 0
+    for _, ab in bond_bounds(job):
+        sb = _site_bytes(job, ab)
+        tot = 0
+        for s, ms in inc.items():
+            z = len(ms)
+            if z not in (3, 4):
+                continue
+            h = z // 2
+            if all(32 < ab[ms[k]] <= 64 for k in (h, h + 1)):
+                tot += sb[s]
+        best = max(best, tot)
+    return best
 
 
 def footprint_bytes(job: EstimatorJob) -> int:
-    """Device bytes a job holds at its peak: resident chi-sized sites, the
-    absorbed lattice when it absorbs eagerly (the plain mode always does) and
-    then also the pair-front copies of the merged sweep, the messages (three
-    sets during a sweep), the work buffers of the per-site sweep (four of the
-    largest site at D = chi * t), split-K partials, and slack."""
-    elem, deg, D = _sizes(job)
-    chi, d = job.chi, 2
-    resident = sum(chi ** z * d * elem for z in deg.values())
-    zmax = max(deg.values())
-    work = 4 * D ** zmax * d * elem
+    """Device bytes a job holds at its peak, from the bond bounds of every
+    layer: resident sites, the absorbed lattice when it absorbs eagerly (the
+    plain mode always does) and then also the pair-front copies of the merged
+    sweep, the messages (three sets during a sweep), the work buffers of the
+    per-site sweep (four of the largest absorbed site), split-K partials, and
+    slack."""
+    peak = 0
+    for res, ab in bond_bounds(job):
+        resident = max(sum(_site_bytes(job, res).values()),
+                       sum(_site_bytes(job, [min(job.chi, x) for x in ab]).values()))
+        work = 4 * max(_site_bytes(job, ab).values())
+        msgs = 3 * 2 * sum(x * x for x in ab) * 16
+        dmax = max(ab)
+        partials = 1024 * min(dmax, 128) ** 2 * 16 * 2
+        peak = max(peak, resident + work + msgs + partials)
     if absorbs_eagerly(job):
-        work += absorbed_bytes(job) + pair_cache_bytes(job)
-    msgs = 3 * 2 * len(job.edges) * D * D * 16
-    partials = 1024 * min(D, 128) ** 2 * 16 * 2
-    return int(1.15 * (resident + work + msgs + partials)) + (64 << 20)
+        peak += absorbed_bytes(job) + pair_cache_bytes(job)
+    # + the context's fixed arenas (deferred split-K partials: 64 MiB) and tables
+    return int(1.15 * peak) + (128 << 20)
 
 
 def run_job(ctx: Context, job: EstimatorJob):
