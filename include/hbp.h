@@ -229,6 +229,18 @@ hbp_status hbp_peps_set_residual_floor(hbp_peps *p, double floor);
 hbp_status hbp_peps_set_pair_cache(hbp_peps *p, int policy);
 enum { HBP_SCHEDULE_JACOBI = 0, HBP_SCHEDULE_CHECKERBOARD = 1 };
 hbp_status hbp_peps_set_schedule(hbp_peps *p, int schedule);
+/* FUSED mode: Anderson mixing of the BP loop's message vector (no reference
+ * counterpart; default depth 0, off).  Each iteration still computes
+ * G = BP(E) and the residual of E as bp.h:266-355 does and stops on the same
+ * test, keeping E (main.cc:138-158); once the residual is below `start` the
+ * next E is the normalised combination sum_i a_i G_i of the last depth + 1
+ * sweep outputs with the real coefficients (sum 1) that minimise the norm of
+ * the same combination of their residual vectors G_i - E_i.  A fixed point of
+ * the plain iteration is one of the mixed one, so converged results agree
+ * within the BP tolerance; the number of sweeps falls.  Applies to whole BP
+ * loops run by one call (hbp_peps_bp_run, hbp_peps_bp_iterate); the window
+ * does not outlive the call.  depth <= 8. */
+hbp_status hbp_peps_set_accel(hbp_peps *p, int depth, double start);
 /* FUSED mode only (default on): fix each message's free scalar phase, trace
  * real and positive, and compare messages modulo that phase in the residual.
  * bp.h:321-350 compares phases too; rounding seeds them and each sweep

@@ -76,6 +76,7 @@ SIGNATURES = {
     "hbp_peps_set_mode": [_vp, C.c_int],
     "hbp_peps_set_residual_floor": [_vp, C.c_double],
     "hbp_peps_set_schedule": [_vp, C.c_int],
+    "hbp_peps_set_accel": [_vp, C.c_int, C.c_double],
     "hbp_peps_set_pair_cache": [_vp, C.c_int],
     "hbp_peps_set_phase_gauge": [_vp, C.c_int],
     "hbp_peps_set_absorb_mode": [_vp, C.c_int],

@@ -184,6 +184,73 @@ __global__ void k_residual(const double *__restrict__ dist, int n, double vol,
   if (threadIdx.x == 0) out[0] = s;
 }
 
+// ------------------------------------------------ Anderson mixing (BP loop)
+// The message set is one vector of n complex numbers (all 2 n_e messages,
+// contiguous).  Up to AA_MAX + 1 vectors of the mixing window per call.
+constexpr int AA_MAX = 8;
+constexpr int AA_BLOCKS = 512;
+struct AAVecs {
+  const double2 *v[AA_MAX + 1];
+  double c[AA_MAX + 1];
+  int n;
+};
+
+// f = G - E, and per block the partial sums of Re<f, h_i> for the window's
+// residual vectors h_0 .. h_{nh-1} followed by <f, f>:
+// part[(nh + 1) * block + i]; the blocks' partials are added by k_aa_sum.
+__global__ __launch_bounds__(256) void k_aa_diff_dots(const double2 *__restrict__ G,
+                                                       const double2 *__restrict__ E,
+                                                       double2 *__restrict__ f, int64_t n,
+                                                       AAVecs h, double *__restrict__ part) {
+  __shared__ double sh[8];
+  double s[AA_MAX + 2];
+  for (int i = 0; i <= AA_MAX + 1; ++i) s[i] = 0.0;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const double2 g = G[e], x = E[e];
+    const double2 d = make_double2(g.x - x.x, g.y - x.y);
+    f[e] = d;
+#pragma unroll
+    for (int i = 0; i < AA_MAX + 1; ++i)
+      if (i < h.n) {
+        const double2 w = h.v[i][e];
+        s[i] += d.x * w.x + d.y * w.y;
+      }
+    s[AA_MAX + 1] += d.x * d.x + d.y * d.y;
+  }
+#pragma unroll
+  for (int i = 0; i < AA_MAX + 2; ++i) {
+    if (i > h.n) break;  // uniform over the block
+    const double t = bsum(i < h.n ? s[i] : s[AA_MAX + 1], sh);
+    if (threadIdx.x == 0) part[(int64_t)(h.n + 1) * blockIdx.x + i] = t;
+  }
+}
+
+// out[i] = sum over the blocks of part[(m) * b + i], in block order
+__global__ void k_aa_sum(const double *__restrict__ part, int m, int blocks,
+                         double *__restrict__ out) {
+  const int i = threadIdx.x;
+  if (i >= m) return;
+  double s = 0.0;
+  for (int b = 0; b < blocks; ++b) s += part[(int64_t)m * b + i];
+  out[i] = s;
+}
+
+// out = sum_i c_i v_i (real coefficients: Hermitian messages stay Hermitian)
+__global__ __launch_bounds__(256) void k_aa_combine(AAVecs g, double2 *__restrict__ out,
+                                                     int64_t n) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    double re = 0.0, im = 0.0;
+    for (int i = 0; i < g.n; ++i) {
+      const double2 v = g.v[i][e];
+      re += g.c[i] * v.x;
+      im += g.c[i] * v.y;
+    }
+    out[e] = make_double2(re, im);
+  }
+}
+
 // ------------------------------------------------------------- absorb
 struct AbsorbArgs {
   int z;
@@ -635,6 +702,11 @@ struct Peps {
   // CHECKERBOARD (bipartite lattices) first updates the messages leaving the
   // sites of colour 0, then those leaving colour 1 from the new ones
   int schedule = HBP_SCHEDULE_JACOBI;
+  // Anderson mixing of the fused BP loop (hbp_peps_set_accel): window depth
+  // (0: off, the plain fixed-point iteration E <- BP(E)) and the residual
+  // below which the loop starts mixing
+  int aa_depth = 0;
+  double aa_start = 1.0e-2;
   std::vector<char> colour;  // site -> 0 / 1; empty until computed, or not bipartite
   int bipartite = -1;        // -1 unknown, 0 no, 1 yes
 
@@ -2425,6 +2497,17 @@ hbp_status hbp_peps_set_residual_floor(hbp_peps *p, double floor) {
   API_END
 }
 
+hbp_status hbp_peps_set_accel(hbp_peps *p, int depth, double start) {
+  API_BEGIN
+  HBP_REQUIRE(p, HBP_ERR_INVALID_ARGUMENT, "peps: null");
+  HBP_REQUIRE(depth >= 0 && depth <= AA_MAX, HBP_ERR_INVALID_ARGUMENT,
+              "peps: mixing depth outside 0.." + std::to_string(AA_MAX));
+  HBP_REQUIRE(start > 0.0, HBP_ERR_INVALID_ARGUMENT, "peps: mixing start residual <= 0");
+  p->aa_depth = depth;
+  p->aa_start = start;
+  API_END
+}
+
 hbp_status hbp_peps_bp_sweep(hbp_ctx *ctx, hbp_peps *p) {
   API_BEGIN
   double2 *nE = (double2 *)ctx->alloc(p->E_bytes);
@@ -2445,6 +2528,163 @@ hbp_status hbp_peps_bp_condition(hbp_ctx *ctx, hbp_peps *p, double *residual) {
   ctx->release(dres, sizeof(double));
   ctx->release(nE, p->E_bytes);
   API_END
+}
+
+// Solve (H + reg I) y = 1 by Gaussian elimination with partial pivoting and
+// return a = y / sum(y), the minimiser of |sum_i a_i f_i| with sum_i a_i = 1
+// for the Gram matrix H_ij = Re<f_i, f_j>; false when that is ill-posed.
+static bool aa_coefficients(std::vector<std::vector<double>> H, std::vector<double> *a) {
+  const int w = (int)H.size();
+  double tr = 0.0;
+  for (int i = 0; i < w; ++i) tr += H[i][i];
+  if (!(tr > 0.0) || !std::isfinite(tr)) return false;
+  const double reg = 1e-10 * tr / w;
+  std::vector<double> y(w, 1.0);
+  for (int i = 0; i < w; ++i) H[i][i] += reg;
+  for (int c = 0; c < w; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < w; ++r)
+      if (std::fabs(H[r][c]) > std::fabs(H[piv][c])) piv = r;
+    if (!(std::fabs(H[piv][c]) > 0.0)) return false;
+    std::swap(H[c], H[piv]);
+    std::swap(y[c], y[piv]);
+    for (int r = c + 1; r < w; ++r) {
+      const double q = H[r][c] / H[c][c];
+      for (int k = c; k < w; ++k) H[r][k] -= q * H[c][k];
+      y[r] -= q * y[c];
+    }
+  }
+  for (int c = w - 1; c >= 0; --c) {
+    double t = y[c];
+    for (int k = c + 1; k < w; ++k) t -= H[c][k] * y[k];
+    y[c] = t / H[c][c];
+  }
+  double sum = 0.0;
+  for (double v : y) sum += v;
+  if (!(std::fabs(sum) > 0.0) || !std::isfinite(sum)) return false;
+  a->resize(w);
+  double l1 = 0.0;
+  for (int i = 0; i < w; ++i) {
+    (*a)[i] = y[i] / sum;
+    l1 += std::fabs((*a)[i]);
+  }
+  return std::isfinite(l1) && l1 < 1e3;
+}
+
+// The fused BP loop with Anderson mixing (type II, constrained form) of the
+// message vector: the window keeps the last sweep outputs G_i = BP(E_i)
+// and residual vectors f_i = G_i - E_i (at most aa_depth + 1), and the next
+// iterate is E = normalise(sum_i a_i G_i) with the real a (Hermitian messages
+// stay Hermitian) that minimise |sum_i a_i f_i| under sum_i a_i = 1.  Mixing
+// starts once the residual is below aa_start; the window restarts from the
+// newest pair when the residual rises above four times its smallest value
+// since the window began or the coefficients are ill-posed.  Every iteration
+// still computes G = BP(E) and the residual of E exactly as the plain loop
+// does (bp.h:266-355), and the loop stops on the same test, keeping E
+// (main.cc:138-158): a fixed point of one iteration is one of the other, so
+// only the path there changes (and the number of sweeps).  Returns the
+// number of iterations run.
+static int bp_loop_mixed(Ctx *ctx, Peps *p, int n, double stop, int last, double *dres,
+                         double *residuals, int *conv) {
+  const int nm = 2 * p->n_edges;
+  const size_t bytes = p->E_bytes;
+  const int64_t len = (int64_t)(bytes / sizeof(double2));
+  const int depth = std::min(p->aa_depth, AA_MAX);
+  std::vector<MsgJob> jobs(nm);
+  for (int m = 0; m < nm; ++m) jobs[m] = MsgJob{p->E_off[m], p->E_off[m], p->E_off[m], p->msg_dim(m)};
+  MsgJob *dj = (MsgJob *)ctx->alloc(nm * sizeof(MsgJob));
+  ctx->upload(dj, jobs.data(), nm * sizeof(MsgJob));
+  const size_t part_bytes = (size_t)AA_BLOCKS * (AA_MAX + 2) * sizeof(double);
+  double *part = (double *)ctx->alloc(part_bytes);
+  double *ddots = (double *)ctx->alloc((AA_MAX + 2) * sizeof(double));
+  std::vector<double2 *> wG, wf;             // the window, oldest first
+  std::vector<std::vector<double>> H;        // H[i][j] = Re<f_i, f_j>
+  double rmin = 0.0;
+  auto drop_oldest = [&]() {
+    ctx->release(wG.front(), bytes);
+    ctx->release(wf.front(), bytes);
+    wG.erase(wG.begin());
+    wf.erase(wf.begin());
+    H.erase(H.begin());
+    for (auto &row : H) row.erase(row.begin());
+  };
+  int it = 0;
+  *conv = 0;
+  for (it = 1; it <= n; ++it) {
+    double2 *G = (double2 *)ctx->alloc(bytes);
+    sweep(ctx, p, G, true, dres);
+    double2 *f = (double2 *)ctx->alloc(bytes);
+    AAVecs hv{};
+    hv.n = (int)wf.size();
+    for (int i = 0; i < hv.n; ++i) hv.v[i] = wf[i];
+    const int blocks = (int)std::min<int64_t>(AA_BLOCKS, std::max<int64_t>(1, (len + 255) / 256));
+    hipLaunchKernelGGL(k_aa_diff_dots, dim3(blocks), dim3(256), 0, ctx->stream, G, p->E, f, len,
+                       hv, part);
+    HBP_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_aa_sum, dim3(1), dim3(64), 0, ctx->stream, part, hv.n + 1, blocks, ddots);
+    HBP_HIP(hipGetLastError());
+    double r = 0.0;
+    std::vector<double> dots(hv.n + 1);
+    HBP_HIP(hipMemcpyAsync(&r, dres, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HBP_HIP(hipMemcpyAsync(dots.data(), ddots, dots.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+    if (residuals) residuals[it - 1] = r;
+    if (r < stop || (it == n && last)) {
+      *conv = r < stop ? 1 : 0;
+      ctx->release(G, bytes);
+      ctx->release(f, bytes);
+      break;
+    }
+    // window update
+    const bool restart = r >= p->aa_start || (!wf.empty() && r > 4.0 * rmin);
+    if (restart) {
+      while (!wf.empty()) drop_oldest();
+    } else {
+      for (int i = 0; i < (int)wf.size(); ++i) H[i].push_back(dots[i]);
+    }
+    std::vector<double> row(wf.size() + 1);
+    for (size_t i = 0; i < wf.size(); ++i) row[i] = dots[i];
+    row.back() = dots[hv.n];
+    H.push_back(row);
+    wG.push_back(G);
+    wf.push_back(f);
+    rmin = wf.size() == 1 ? r : std::min(rmin, r);
+    while ((int)wf.size() > depth + 1) drop_oldest();
+    std::vector<double> a;
+    if (wf.size() > 1 && !aa_coefficients(H, &a)) {
+      while (wf.size() > 1) drop_oldest();
+      rmin = r;
+    }
+    double2 *nE = (double2 *)ctx->alloc(bytes);
+    if (wf.size() == 1) {
+      // the plain iteration: E <- BP(E)
+      HBP_HIP(hipMemcpyAsync(nE, G, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    } else {
+      AAVecs gv{};
+      gv.n = (int)wG.size();
+      for (int i = 0; i < gv.n; ++i) {
+        gv.v[i] = wG[i];
+        gv.c[i] = a[i];
+      }
+      double2 *comb = (double2 *)ctx->alloc(bytes);
+      hipLaunchKernelGGL(k_aa_combine, dim3(gridn((size_t)len)), dim3(256), 0, ctx->stream, gv,
+                         comb, len);
+      HBP_HIP(hipGetLastError());
+      hipLaunchKernelGGL(k_finalize_messages, dim3(nm), dim3(256), 0, ctx->stream, comb, nE,
+                         nullptr, dj, nullptr, p->phase_gauge ? 1 : 0, nullptr);
+      HBP_HIP(hipGetLastError());
+      ctx->release(comb, bytes);
+    }
+    ctx->release(p->E, bytes);
+    p->E = nE;
+    p->E_diag.assign(nm, 0);
+  }
+  while (!wf.empty()) drop_oldest();
+  ctx->release(ddots, (AA_MAX + 2) * sizeof(double));
+  ctx->release(part, part_bytes);
+  ctx->release(dj, nm * sizeof(MsgJob));
+  return std::min(it, n);
 }
 
 // Up to n iterations of the driver's BP loop (main.cc:121-159).  `first`
@@ -2488,21 +2728,25 @@ static void bp_iterate(Ctx *ctx, Peps *p, int n, double tol, int first, int last
       p->E = nE;
       p->E_diag.assign(2 * p->n_edges, 0);
     }
-    for (it = 1; it <= n; ++it) {
-      double2 *G = (double2 *)ctx->alloc(p->E_bytes);
-      sweep(ctx, p, G, true, dres);
-      double r = 0.0;
-      HBP_HIP(hipMemcpyAsync(&r, dres, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-      ctx->sync();
-      if (residuals) residuals[it - 1] = r;
-      if (r < stop || (it == n && last)) {
-        conv = r < stop ? 1 : 0;
-        ctx->release(G, p->E_bytes);
-        break;
+    if (p->aa_depth > 0 && p->n_edges > 0) {
+      it = bp_loop_mixed(ctx, p, n, stop, last, dres, residuals, &conv);
+    } else {
+      for (it = 1; it <= n; ++it) {
+        double2 *G = (double2 *)ctx->alloc(p->E_bytes);
+        sweep(ctx, p, G, true, dres);
+        double r = 0.0;
+        HBP_HIP(hipMemcpyAsync(&r, dres, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        ctx->sync();
+        if (residuals) residuals[it - 1] = r;
+        if (r < stop || (it == n && last)) {
+          conv = r < stop ? 1 : 0;
+          ctx->release(G, p->E_bytes);
+          break;
+        }
+        ctx->release(p->E, p->E_bytes);
+        p->E = G;
+        p->E_diag.assign(2 * p->n_edges, 0);
       }
-      ctx->release(p->E, p->E_bytes);
-      p->E = G;
-      p->E_diag.assign(2 * p->n_edges, 0);
     }
   }
   if (iters) *iters = std::min(it, n);

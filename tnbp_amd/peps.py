@@ -83,6 +83,12 @@ class Peps:
         """-1 by free memory (default), 0 never, 1 always (memory budgeted by the caller)."""
         check(self.ctx.lib.hbp_peps_set_pair_cache(self.h, int(policy)))
 
+    def set_accel(self, depth, start=1e-2):
+        """Anderson mixing of the BP loop (depth 0: off, the default): once the
+        residual is below ``start`` the next message set combines the last
+        ``depth + 1`` sweep outputs; same fixed point and stopping test."""
+        check(self.ctx.lib.hbp_peps_set_accel(self.h, int(depth), float(start)))
+
     def set_residual_floor(self, floor):
         """Mixed precision: BP also stops once the residual is below this."""
         check(self.ctx.lib.hbp_peps_set_residual_floor(self.h, float(floor)))
