@@ -78,6 +78,8 @@ def parse():
     ap.add_argument("--accel", type=int, default=4,
                     help="Anderson mixing depth of the headline path's BP loop (0: the plain "
                          "fixed-point iteration; the FP64 leg never mixes)")
+    ap.add_argument("--accel-start", type=float, default=1e-2,
+                    help="residual below which the mixing starts")
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--max-bp-iterations", type=int, default=50,
                     help="the kicked driver's default (apps/kicked_ising_lattice/option.h:19)")
@@ -122,14 +124,14 @@ class Job:
 class Runner:
     """One PEPS on one context, stepping the job."""
 
-    def __init__(self, ctx, job, precision, sweep, schedule="jacobi", accel=0):
+    def __init__(self, ctx, job, precision, sweep, schedule="jacobi", accel=0, accel_start=1e-2):
         from tnbp_amd.peps import HBP_PREC_FP64, HBP_PREC_MIXED, InitTensorProductState
         self.ctx, self.job = ctx, job
         prec = HBP_PREC_FP64 if precision == "fp64" else HBP_PREC_MIXED
         self.peps = InitTensorProductState(ctx, job.edges, [2] * len(job.sites), prec)
         self.peps.set_sweep_kernel({"auto": 0, "per-site": 1, "merged": 3}[sweep])
         self.peps.set_schedule({"jacobi": 0, "checkerboard": 1}[schedule])
-        self.peps.set_accel(accel)
+        self.peps.set_accel(accel, accel_start)
         self.tag = precision
 
     def step(self, args):
@@ -179,7 +181,8 @@ def run_headline(ctx, job, args):
     """Warm-up, then `args.steps` timed Floquet steps as jobs of 20 steps."""
     from tnbp_amd._lib import check
     if args.warmup > 0:
-        w = Runner(ctx, job, args.precision, args.sweep, args.schedule, args.accel)
+        w = Runner(ctx, job, args.precision, args.sweep, args.schedule, args.accel,
+                   args.accel_start)
         for k in range(args.warmup):
             rec = w.step(args)
             log({"warmup_step": k + 1, **rec})
@@ -194,7 +197,8 @@ def run_headline(ctx, job, args):
     while left > 0:
         n = min(JOB_STEPS, left)
         left -= n
-        r = Runner(ctx, job, args.precision, args.sweep, args.schedule, args.accel)
+        r = Runner(ctx, job, args.precision, args.sweep, args.schedule, args.accel,
+                   args.accel_start)
         ctx.synchronize()
         t_job = time.perf_counter()
         paused = 0.0
@@ -447,7 +451,7 @@ def main():
                                f"iterations, 256 <Z> + 480 <ZZ> at the end of the job; a bench step "
                                f"is one Floquet step",
                    "precision": args.precision, "sweep": args.sweep, "schedule": args.schedule,
-                   "bp_mixing_depth": args.accel,
+                   "bp_mixing_depth": args.accel, "bp_mixing_start": args.accel_start,
                    "L": job.L, "chi": job.chi,
                    "jobs_timed": n_jobs, "values_per_job": job.n_values},
         "latency_s": {"per_job": {"n": n_jobs, "p50": float(np.percentile(job_lat, 50)),

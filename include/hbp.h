@@ -85,6 +85,12 @@ hbp_status hbp_ctx_trim(hbp_ctx *ctx);
  * class, NUL-terminated, into buf (len bytes); needed receives the size the
  * full report takes.  No reference counterpart. */
 hbp_status hbp_autotune_report(char *buf, size_t len, size_t *needed);
+/* HIP graphs of hbp_peps_bp_run_batched on this context so far: graphs
+ * recorded, replays, and recordings dropped because the iteration needed
+ * something a graph cannot hold (the loop then ran without graphs).  No
+ * reference counterpart. */
+hbp_status hbp_ctx_graph_stats(hbp_ctx *ctx, int64_t *recorded, int64_t *replays,
+                               int64_t *dropped);
 /* Free and total device memory of a device (hipMemGetInfo). */
 hbp_status hbp_device_memory(int device, size_t *free_bytes, size_t *total_bytes);
 /* Kernel timing (no reference counterpart; the reference times nothing but
@@ -331,16 +337,22 @@ hbp_status hbp_peps_bp_run(hbp_ctx *ctx, hbp_peps *p, int max_iter, double tol, 
                            double *residuals);
 /* hbp_peps_bp_run for n PEPS of one context at once (no reference
  * counterpart; each PEPS gets the iterations, residuals and state that
- * hbp_peps_bp_run gives it alone).  PEPS in fused mode with the Jacobi
- * schedule, site degree <= 4 and every bond <= 16 iterate in lockstep: each
- * iteration is one persistent launch over every site of every such PEPS of
- * a precision (one workgroup per site and stride), a finalisation and a
- * residual launch, captured in a HIP graph after the first iteration and
- * replayed (launch grid picked by the autotuner); a PEPS leaves the batch
- * when it converges or at max_iter.  The others run hbp_peps_bp_run one
- * after another.  residuals (may be NULL): n rows of max_iter values;
- * iters, converged, batched (each may be NULL): n values, batched[i] = 1
- * where PEPS i ran in the batch. */
+ * hbp_peps_bp_run gives it alone).  The PEPS in fused mode with the Jacobi
+ * schedule and no mixing iterate in lockstep, one BP iteration of all of
+ * them per replay of a HIP graph: those with site degree <= 4 and every
+ * bond <= 16 (HBP_BATCH_SMALL) in one persistent launch per precision over
+ * every site of every such PEPS (one workgroup per site and stride, launch
+ * grid picked by the autotuner), a finalisation and a residual launch; the
+ * others (HBP_BATCH_GRAPH) with their own fused sweeps (per-site launches,
+ * variants picked by the autotuner), bitwise as hbp_peps_bp_run.  The first
+ * iteration runs unrecorded; the graph of an iteration is recorded once per
+ * message-set parity and replayed until a PEPS leaves (it converged or
+ * reached max_iter).  The other PEPS (plain mode, checkerboard schedule,
+ * mixing) run hbp_peps_bp_run one after another.  residuals (may be NULL):
+ * n rows of max_iter values; iters, converged, batched (each may be NULL):
+ * n values, batched[i] = HBP_BATCH_SMALL, HBP_BATCH_GRAPH or 0 (on its
+ * own). */
+enum { HBP_BATCH_ALONE = 0, HBP_BATCH_SMALL = 1, HBP_BATCH_GRAPH = 2 };
 hbp_status hbp_peps_bp_run_batched(hbp_ctx *ctx, int n, hbp_peps *const *ps, int max_iter,
                                    double tol, int *iters, double *residuals, int *converged,
                                    int *batched);

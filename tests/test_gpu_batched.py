@@ -10,8 +10,12 @@ replayed from a HIP graph.
   residuals, messages and <Z> within 1e-12 (FP64; different kernels, so not
   bitwise); a PEPS run in the batch alone equals itself run in the mix bit
   for bit; in mixed precision within 1e-5 of FP64.
-* PEPS the batch does not take (a bond above 16, the checkerboard schedule)
-  run on their own and still get bp_loop's results.
+* PEPS above the small-bond limit (bonds of 32 and 64, both precisions) run
+  their own fused sweeps recorded into the same per-iteration graph: every
+  one gets bp_loop's iteration count, residuals and messages bit for bit,
+  also while others leave the batch and the graph is recorded again.
+* PEPS the batch does not take (the checkerboard schedule, mixing) run on
+  their own and still get bp_loop's results.
 """
 import json
 import math
@@ -173,18 +177,62 @@ def _kicked_big(ctx):
 
 
 def test_ineligible_peps_run_on_their_own(ctx):
-    from tnbp_amd.peps import HBP_SCHEDULE_CHECKERBOARD, bp_loop, bp_loop_batched
+    from tnbp_amd.peps import (HBP_BATCH_ALONE, HBP_BATCH_GRAPH, HBP_BATCH_SMALL,
+                               HBP_SCHEDULE_CHECKERBOARD, bp_loop, bp_loop_batched)
     announce("batch with PEPS it does not take")
     tol = 1e-11
     big, big_ref = _kicked_big(ctx), _kicked_big(ctx)
     c = _prepared(ctx, 0, seed=5)
     c[0][1].set_schedule(HBP_SCHEDULE_CHECKERBOARD)
+    c[2][1].set_accel(4)
     d = _prepared(ctx, 0, seed=5)
     d[0][1].set_schedule(HBP_SCHEDULE_CHECKERBOARD)
-    out = bp_loop_batched(ctx, [big, c[0][1], c[1][1]], 200, tol)
-    assert [bat for *_, bat in out] == [False, False, True]
-    for (it, res, conv, bat), q in zip(out[:2], [big_ref, d[0][1]]):
+    d[2][1].set_accel(4)
+    out = bp_loop_batched(ctx, [big, c[0][1], c[1][1], c[2][1]], 200, tol)
+    assert [bat for *_, bat in out] == [HBP_BATCH_GRAPH, HBP_BATCH_ALONE, HBP_BATCH_SMALL,
+                                        HBP_BATCH_ALONE]
+    for (it, res, conv, bat), p, q in zip([out[0], out[1], out[3]], [big, c[0][1], c[2][1]],
+                                         [big_ref, d[0][1], d[2][1]]):
         it0, res0 = bp_loop(ctx, q, 200, tol)
         assert it == it0 and res == res0
+        for a, b in zip(_state(p), _state(q)):
+            np.testing.assert_array_equal(a, b)
     for p in [big, big_ref] + [p for _, p in c + d]:
+        p.close()
+
+
+def _graph_members(ctx):
+    """PEPS above the small-bond limit: the 3x3 kicked-Ising state at bonds
+    of 32 (FP64, converges within a few iterations) and seeded random 3x3
+    states with every bond at 64 in both precisions (the merged
+    product-message and buffer-form mode-product kernels of the 16x16 chi=32
+    steady state; random messages, so they iterate to max_iter)."""
+    from tnbp_amd.kernel_probe import build_state
+    return [_kicked_big(ctx), build_state(ctx, 3, 64, 0, seed=3), build_state(ctx, 3, 64, 1, seed=4)]
+
+
+def test_graph_members_equal_bp_loop_bitwise(ctx):
+    from tnbp_amd.peps import HBP_BATCH_GRAPH, HBP_BATCH_SMALL, bp_loop, bp_loop_batched
+    announce("graph-recorded sweeps of bond-32/64 PEPS with small ones vs bp_loop alone")
+    tol, max_iter = 1e-11, 30
+    mine = _graph_members(ctx)
+    small = [p for _, p in _prepared(ctx, 0, seed=9)] + [p for _, p in _prepared(ctx, 1, seed=9)]
+    rec0, rep0, drop0 = ctx.graph_stats()
+    out = bp_loop_batched(ctx, mine + small, max_iter, tol)
+    rec, rep, drop = (a - b for a, b in zip(ctx.graph_stats(), (rec0, rep0, drop0)))
+    its = [o[0] for o in out]
+    announce(f"  iterations {its}; graphs recorded {rec}, replays {rep}, dropped {drop}")
+    assert [b for *_, b in out] == [HBP_BATCH_GRAPH] * 3 + [HBP_BATCH_SMALL] * 8
+    # every iteration after the first is a replay; members leaving at different
+    # iterations make the loop record again for the rest
+    assert drop == 0 and rep == max(its) - 1 and rec >= 2
+    assert len(set(its)) > 1
+    refs = _graph_members(ctx)
+    for (it, res, conv, _), p, q in zip(out[:3], mine, refs):
+        it0, res0 = bp_loop(ctx, q, max_iter, tol)
+        announce(f"  iterations {it} (alone {it0}), last residual {res[-1]:.3e}")
+        assert it == it0 and res == res0
+        for a, b in zip(_state(p), _state(q)):
+            np.testing.assert_array_equal(a, b)
+    for p in mine + refs + small:
         p.close()

@@ -18,6 +18,7 @@ HBP_ABSORB_AUTO, HBP_ABSORB_LAZY, HBP_ABSORB_EAGER = 0, 1, 2
 HBP_SWEEP_AUTO, HBP_SWEEP_PER_SITE, HBP_SWEEP_GROUPED, HBP_SWEEP_MERGED = 0, 1, 2, 3
 HBP_MODE_FUSED, HBP_MODE_PLAIN = 0, 1
 HBP_SCHEDULE_JACOBI, HBP_SCHEDULE_CHECKERBOARD = 0, 1
+HBP_BATCH_ALONE, HBP_BATCH_SMALL, HBP_BATCH_GRAPH = 0, 1, 2
 
 STATUS = {
     1: ValueError,       # HBP_ERR_INVALID_ARGUMENT (std::invalid_argument)
@@ -42,6 +43,7 @@ SIGNATURES = {
     "hbp_ctx_trim": [_vp],
     "hbp_autotune_report": [C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)],
     "hbp_device_memory": [C.c_int, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)],
+    "hbp_ctx_graph_stats": [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
     "hbp_ctx_profile": [_vp, C.c_int],
     "hbp_ctx_profile_read": [_vp, C.c_int, C.POINTER(C.c_int64), _dp, _dp, _dp],
     "hbp_tensor_create": [_vp, C.c_int, C.c_int, _i32p, C.POINTER(_vp)],

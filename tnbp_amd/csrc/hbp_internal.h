@@ -49,6 +49,8 @@ struct Ctx {
   hipStream_t stream = nullptr;
   hipMemPool_t pool = nullptr;
   size_t in_use = 0, peak = 0;
+  // HIP graphs of batched BP loops: recorded, replays, recordings dropped
+  int64_t graphs_recorded = 0, graph_replays = 0, graphs_dropped = 0;
   // grow-only scratch for split-K partials and reductions
   void *scratch = nullptr;
   size_t scratch_bytes = 0;
@@ -139,6 +141,12 @@ struct LaunchTiming {
 LaunchTiming &launch_timing();
 // true while tuned_launch times variants on this thread
 bool &launch_tuning();
+// true while this thread records a context stream into a HIP graph.  What a
+// graph cannot hold raises HBP_ERR_UNSUPPORTED instead of being recorded:
+// a device allocation that the context's block cache cannot serve, a host
+// wait (Ctx::sync, Ctx::upload), an autotuner trial; timing brackets are
+// skipped and released blocks stay in the cache (no stream-ordered free).
+bool &graph_capturing();
 template <typename K, typename... Args>
 inline void timed_launch(K kern, dim3 grid, dim3 block, hipStream_t st, Args... args) {
   LaunchTiming &t = launch_timing();

@@ -1734,6 +1734,8 @@ void tuned_launch(hipStream_t st, const TuneKey &key, int nv, F run) {
       return;
     }
   }
+  HBP_REQUIRE(!graph_capturing(), HBP_ERR_UNSUPPORTED,
+              "autotuner trial of a new launch shape while a graph is recorded");
   hipEvent_t a, b;
   HBP_HIP(hipEventCreate(&a));
   HBP_HIP(hipEventCreate(&b));

@@ -748,6 +748,9 @@ static void layout_messages(Ctx *ctx, Peps *p, double2 **newE, size_t *bytes,
 
 static void ensure_work(Ctx *ctx, Peps *p, size_t bytes) {
   if (bytes <= p->buf_bytes) return;
+  // blocks that outlive a recording must not come from the cache then (a
+  // recorded graph may use them as its temporaries)
+  HBP_REQUIRE(!graph_capturing(), HBP_ERR_UNSUPPORTED, "work buffers grow while a graph is recorded");
   for (int i = 0; i < 3; ++i)
     if (p->buf[i]) ctx->release(p->buf[i], p->buf_bytes);
   for (int i = 0; i < 3; ++i) p->buf[i] = ctx->alloc(bytes);
@@ -775,6 +778,8 @@ static const void *site_data(Ctx *ctx, Peps *p, int s) {
   if (p->abs_site == s) return p->abs_buf;
   const size_t bytes = p->site_size(s) * p->elem_bytes();
   if (bytes > p->abs_bytes) {
+    HBP_REQUIRE(!graph_capturing(), HBP_ERR_UNSUPPORTED,
+                "absorption buffer grows while a graph is recorded");
     if (p->abs_buf) ctx->release(p->abs_buf, p->abs_bytes);
     p->abs_buf = ctx->alloc(bytes);
     p->abs_bytes = bytes;
@@ -1166,10 +1171,14 @@ static const void *pair_front(Ctx *ctx, Peps *p, int s, const void *Vs, const in
   for (int d = 0; d < t; ++d) Lb *= sh[d];
   for (int d = t + 2; d < order; ++d) R *= sh[d];
   if (Lb == 1) return Vs;
+  HBP_REQUIRE(p->pair_cache >= 0 || !graph_capturing(), HBP_ERR_UNSUPPORTED,
+              "pair-front copies planned while a graph is recorded");
   if (p->pair_cache < 0) plan_pair_cache(ctx, p);
   if (p->Vpair[s]) return p->Vpair[s];
   void *dst = p->buf[2];
   if (p->pair_cache == 1 && !p->pending[s]) {
+    HBP_REQUIRE(!graph_capturing(), HBP_ERR_UNSUPPORTED,
+                "pair-front copy built while a graph is recorded");
     const size_t vb = p->site_size(s) * p->elem_bytes();
     p->Vpair[s] = ctx->alloc(vb);
     p->Vpair_bytes[s] = vb;
@@ -1457,8 +1466,11 @@ static void sweep_checkerboard(Ctx *ctx, Peps *p, double2 *raw, double2 *newE, d
   ctx->release(mix, std::max<size_t>(p->E_bytes, 16));
 }
 
-// one BP sweep: newE = normalise(BP(E)); returns residual on device if dist != null
-static void sweep(Ctx *ctx, Peps *p, double2 *newE, bool with_residual, double *dev_res) {
+// one BP sweep: newE = normalise(BP(E)); returns residual on device if dist != null.
+// dj_pre (Jacobi schedule only): the message table (MsgJob per message,
+// offsets p->E_off) already on the device, so that the sweep needs no upload
+static void sweep(Ctx *ctx, Peps *p, double2 *newE, bool with_residual, double *dev_res,
+                  const MsgJob *dj_pre = nullptr) {
   const int nm = 2 * p->n_edges;
   if (p->mode == HBP_MODE_FUSED && p->schedule == HBP_SCHEDULE_CHECKERBOARD && colour_sites(p)) {
     double2 *raw = (double2 *)ctx->alloc(std::max<size_t>(p->E_bytes, 16));
@@ -1477,10 +1489,16 @@ static void sweep(Ctx *ctx, Peps *p, double2 *newE, bool with_residual, double *
     return;
   }
   // job table first: its upload waits on the stream, which is then still short
-  std::vector<MsgJob> jobs(nm);
-  for (int m = 0; m < nm; ++m) jobs[m] = MsgJob{p->E_off[m], p->E_off[m], p->E_off[m], p->msg_dim(m)};
-  MsgJob *dj = (MsgJob *)ctx->alloc(nm * sizeof(MsgJob));
-  ctx->upload(dj, jobs.data(), nm * sizeof(MsgJob));
+  const MsgJob *dj = dj_pre;
+  MsgJob *dj_own = nullptr;
+  if (!dj) {
+    std::vector<MsgJob> jobs(nm);
+    for (int m = 0; m < nm; ++m)
+      jobs[m] = MsgJob{p->E_off[m], p->E_off[m], p->E_off[m], p->msg_dim(m)};
+    dj_own = (MsgJob *)ctx->alloc(nm * sizeof(MsgJob));
+    ctx->upload(dj_own, jobs.data(), nm * sizeof(MsgJob));
+    dj = dj_own;
+  }
   double2 *raw = (double2 *)ctx->alloc(std::max<size_t>(p->E_bytes, 16));
   if (p->mode == HBP_MODE_PLAIN) {
     if (p->precision == HBP_PREC_FP64)
@@ -1506,7 +1524,7 @@ static void sweep(Ctx *ctx, Peps *p, double2 *newE, bool with_residual, double *
     HBP_HIP(hipGetLastError());
   }
   ctx->release(dist, nm * sizeof(double));
-  ctx->release(dj, nm * sizeof(MsgJob));
+  if (dj_own) ctx->release(dj_own, nm * sizeof(MsgJob));
   ctx->release(raw, std::max<size_t>(p->E_bytes, 16));
 }
 
@@ -2758,9 +2776,10 @@ static void bp_iterate(Ctx *ctx, Peps *p, int n, double tol, int first, int last
 
 // ------------------------------------------------------ batched BP loops
 // PEPS the batched small-bond sweep takes: fused mode with the reference's
-// Jacobi schedule, degree 1..4, every bond <= SMALL_D_MAX.
+// Jacobi schedule and no mixing, degree 1..4, every bond <= SMALL_D_MAX.
 static bool small_eligible(const Peps *p) {
-  if (p->mode != HBP_MODE_FUSED || p->schedule != HBP_SCHEDULE_JACOBI) return false;
+  if (p->mode != HBP_MODE_FUSED || p->schedule != HBP_SCHEDULE_JACOBI || p->aa_depth > 0)
+    return false;
   for (int s = 0; s < p->n_sites; ++s)
     if (p->z(s) < 1 || p->z(s) > 4) return false;
   for (int m = 0; m < p->n_edges; ++m)
@@ -2768,12 +2787,22 @@ static bool small_eligible(const Peps *p) {
   return true;
 }
 
+// PEPS whose own fused sweep (per-site launches) the batch records into its
+// per-iteration graph: fused mode, Jacobi schedule, no mixing.
+static bool graph_eligible(const Peps *p) {
+  return p->mode == HBP_MODE_FUSED && p->schedule == HBP_SCHEDULE_JACOBI && p->aa_depth == 0 &&
+         p->n_edges > 0;
+}
+
 namespace {
 // one PEPS in a batched BP loop
 struct BatchMember {
   Peps *p = nullptr;
+  int kind = 0;            // HBP_BATCH_SMALL or HBP_BATCH_GRAPH
   double2 *G = nullptr;    // the other message set (E <-> G swap per iteration)
-  double2 *raw = nullptr;  // raw messages of the sweep
+  double2 *raw = nullptr;  // raw messages of the small-bond sweep
+  MsgJob *dj = nullptr;    // message table of the PEPS's own sweep
+  int slot = 0;            // position of its residual (own sweep)
   int index = 0;           // position in the caller's arrays
   double stop = 0.0;
   int iters = 0, conv = 0;
@@ -2864,108 +2893,229 @@ void launch_batch_sweep(Ctx *ctx, const BatchTables &t, void *scratch, int64_t s
     HBP_HIP(hipGetLastError());
   }
 }
+
+// The members of one precision that the small-bond kernel takes, with the
+// kernel's scratch (two site-sized buffers per workgroup of the largest grid)
+// and per-parity tables.  act[i]'s residual lands at res[res_off + i].
+struct SmallGroup {
+  bool fp32 = false;
+  std::vector<BatchMember *> act;
+  int64_t slab = 1;
+  void *scratch = nullptr;
+  size_t scratch_bytes = 0;
+  double *dist = nullptr;
+  size_t dist_bytes = 0;
+  int variant = 0;
+  int res_off = 0;
+  BatchTables tabs[2];
+  bool have[2] = {false, false};
+  void sweep(Ctx *ctx, int parity, double *res) const {
+    if (fp32)
+      launch_batch_sweep<float>(ctx, tabs[parity], scratch, slab, variant, dist, res + res_off, true);
+    else
+      launch_batch_sweep<double>(ctx, tabs[parity], scratch, slab, variant, dist, res + res_off,
+                                 true);
+  }
+};
 }  // namespace
 
 // The driver's BP loop (as bp_iterate with first = last = 1) for several
-// PEPS of one precision in lockstep: every iteration is one batched sweep of
-// all members still iterating; a member leaves when its residual is below
-// its stop value or at max_iter, keeping E like the reference.  After the
-// first iteration (which also times the launch variants) the launches of an
-// iteration are captured in a HIP graph per message-set parity and replayed.
-template <typename S>
+// PEPS in lockstep: every iteration advances every member still iterating by
+// one sweep (E -> G) and computes its residual; a member leaves when its
+// residual is below its stop value or at max_iter, keeping E like the
+// reference (main.cc:138-158).  Members with bonds <= 16 run the persistent
+// small-bond kernel, one launch per precision over all their sites; the
+// others run their own fused sweeps, which read the device tables prepared
+// here and are bitwise those of bp_iterate.  One iteration of all members is
+// recorded into a HIP graph per message-set parity and replayed until a
+// member leaves; the first iteration (and the loop's first sweep) runs
+// unrecorded, so that the autotuner's trials, the work buffers, the
+// pair-front copies and the block cache are settled before recording.  A
+// recording that meets anything a graph cannot hold (graph_capturing) is
+// dropped and the loop goes on without graphs.
 static void bp_run_batched_impl(Ctx *ctx, std::vector<BatchMember> &mem, int max_iter) {
-  int64_t slab = 1;
+  SmallGroup grp[2];
+  grp[1].fp32 = true;
+  std::vector<BatchMember *> big;
   for (auto &b : mem) {
     Peps *p = b.p;
-    drop_pair_cache(ctx, p);
-    for (int s = 0; s < p->n_sites; ++s) make_resident(ctx, p, s);
-    for (int s = 0; s < p->n_sites; ++s) slab = std::max<int64_t>(slab, (int64_t)p->site_size(s));
+    if (b.kind == HBP_BATCH_SMALL) {
+      SmallGroup &g = grp[p->precision == HBP_PREC_MIXED ? 1 : 0];
+      drop_pair_cache(ctx, p);
+      for (int s = 0; s < p->n_sites; ++s) make_resident(ctx, p, s);
+      for (int s = 0; s < p->n_sites; ++s)
+        g.slab = std::max<int64_t>(g.slab, (int64_t)p->site_size(s));
+      b.raw = (double2 *)ctx->alloc(std::max<size_t>(p->E_bytes, 16));
+      g.act.push_back(&b);
+    } else {
+      const int nm = 2 * p->n_edges;
+      std::vector<MsgJob> jobs(nm);
+      for (int m = 0; m < nm; ++m)
+        jobs[m] = MsgJob{p->E_off[m], p->E_off[m], p->E_off[m], p->msg_dim(m)};
+      b.dj = (MsgJob *)ctx->alloc(nm * sizeof(MsgJob));
+      ctx->upload(b.dj, jobs.data(), nm * sizeof(MsgJob));
+      big.push_back(&b);
+    }
     b.G = (double2 *)ctx->alloc(std::max<size_t>(p->E_bytes, 16));
-    b.raw = (double2 *)ctx->alloc(std::max<size_t>(p->E_bytes, 16));
   }
-  int nsites_all = 0, nmsgs_all = 0;
-  for (auto &b : mem) {
-    nsites_all += b.p->n_sites;
-    nmsgs_all += 2 * b.p->n_edges;
+  int nres = 0;
+  for (auto &g : grp) {
+    g.res_off = nres;
+    nres += (int)g.act.size();
   }
-  const int grid_max = small_sweep_grid(SMALL_GRID_VARIANTS - 1, nsites_all);
-  const size_t scratch_bytes = (size_t)grid_max * 2 * (size_t)slab * 2 * sizeof(S);
-  void *scratch = ctx->alloc(scratch_bytes);
-  double *dist = (double *)ctx->alloc(std::max(nmsgs_all, 1) * sizeof(double));
-  double *res = (double *)ctx->alloc(std::max<size_t>(mem.size(), 1) * sizeof(double));
+  for (auto *b : big) b->slot = nres++;
+  const size_t res_bytes = (size_t)std::max(nres, 1) * sizeof(double);
+  double *res = (double *)ctx->alloc(res_bytes);
+  for (auto &g : grp) {
+    if (g.act.empty()) continue;
+    int nsites = 0, nmsgs = 0;
+    for (auto *b : g.act) {
+      nsites += b->p->n_sites;
+      nmsgs += 2 * b->p->n_edges;
+    }
+    const int grid_max = small_sweep_grid(SMALL_GRID_VARIANTS - 1, nsites);
+    g.scratch_bytes = (size_t)grid_max * 2 * (size_t)g.slab * 2 * (g.fp32 ? sizeof(float) : sizeof(double));
+    g.scratch = ctx->alloc(g.scratch_bytes);
+    g.dist_bytes = (size_t)std::max(nmsgs, 1) * sizeof(double);
+    g.dist = (double *)ctx->alloc(g.dist_bytes);
+  }
   auto swap_sets = [&](BatchMember &b) {
     std::swap(b.p->E, b.G);
     b.p->E_diag.assign(2 * b.p->n_edges, 0);
   };
   // the loop's first sweep E_1 = BP(E_0), no residual (main.cc:121-141)
-  std::vector<BatchMember *> act;
-  for (auto &b : mem) act.push_back(&b);
-  int variant = 0;
-  {
-    BatchTables t = build_tables(ctx, act, false);
-    variant = launch_sweep_small_tuned<S>(ctx->stream, t.sites, t.nsites, scratch, slab);
+  for (auto &g : grp) {
+    if (g.act.empty()) continue;
+    BatchTables t = build_tables(ctx, g.act, false);
+    g.variant = g.fp32 ? launch_sweep_small_tuned<float>(ctx->stream, t.sites, t.nsites, g.scratch, g.slab)
+                       : launch_sweep_small_tuned<double>(ctx->stream, t.sites, t.nsites, g.scratch, g.slab);
     hipLaunchKernelGGL(k_finalize_batched, dim3((unsigned)t.nmsgs), dim3(256), 0, ctx->stream,
-                       t.msgs, dist);
+                       t.msgs, g.dist);
     HBP_HIP(hipGetLastError());
     ctx->sync();
     t.release(ctx);
-    for (auto *b : act) swap_sets(*b);
   }
-  std::vector<double> hres(mem.size());
-  // graphs of the current member set, one per parity of the E/G swap
+  for (auto *b : big) sweep(ctx, b->p, b->G, false, nullptr, b->dj);
+  for (auto &b : mem) swap_sets(b);
+
+  std::vector<double> hres((size_t)std::max(nres, 1));
   hipGraphExec_t exec[2] = {nullptr, nullptr};
-  BatchTables tabs[2];
-  bool have[2] = {false, false};
+  bool graphs = true;                        // false once a recording failed
+  const int unrecorded = big.empty() ? 0 : 1;  // iterations run before recording
   auto drop_graphs = [&]() {
     for (int q = 0; q < 2; ++q) {
       if (exec[q]) (void)hipGraphExecDestroy(exec[q]);
       exec[q] = nullptr;
-      if (have[q]) tabs[q].release(ctx);
-      have[q] = false;
+      for (auto &g : grp)
+        if (g.have[q]) {
+          g.tabs[q].release(ctx);
+          g.have[q] = false;
+        }
+    }
+  };
+  // one iteration of every active member (E -> G and residuals), enqueued
+  auto enqueue = [&](int parity) {
+    for (auto &g : grp)
+      if (!g.act.empty()) g.sweep(ctx, parity, res);
+    for (auto *b : big) sweep(ctx, b->p, b->G, true, res + b->slot, b->dj);
+  };
+  // Tables of both parities are built together, before either graph is
+  // recorded: a recorded sweep takes its temporaries from the block cache and
+  // gives them back at the end of the recording, so nothing may allocate
+  // outside a recording while a graph that uses those blocks can still be
+  // replayed (both graphs are dropped together when a member leaves).
+  auto build_both = [&](int parity) {
+    for (auto &g : grp) {
+      if (g.act.empty() || g.have[0] || g.have[1]) continue;
+      g.tabs[parity] = build_tables(ctx, g.act, true);
+      for (auto *b : g.act) std::swap(b->p->E, b->G);  // the other parity's sets
+      g.tabs[parity ^ 1] = build_tables(ctx, g.act, true);
+      for (auto *b : g.act) std::swap(b->p->E, b->G);
+      g.have[0] = g.have[1] = true;
     }
   };
   int parity = 0;
-  for (int it = 1; it <= max_iter && !act.empty(); ++it) {
-    if (!have[parity]) {
-      tabs[parity] = build_tables(ctx, act, true);
-      have[parity] = true;
-      hipGraph_t g = nullptr;
-      HBP_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      launch_batch_sweep<S>(ctx, tabs[parity], scratch, slab, variant, dist, res, true);
-      HBP_HIP(hipStreamEndCapture(ctx->stream, &g));
-      HBP_HIP(hipGraphInstantiate(&exec[parity], g, nullptr, nullptr, 0));
-      HBP_HIP(hipGraphDestroy(g));
+  for (int it = 1; it <= max_iter; ++it) {
+    if (grp[0].act.empty() && grp[1].act.empty() && big.empty()) break;
+    build_both(parity);
+    bool replay = graphs && it > unrecorded;
+    if (replay && !exec[parity]) {
+      hipGraph_t gr = nullptr;
+      try {
+        HBP_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        graph_capturing() = true;
+        enqueue(parity);
+        graph_capturing() = false;
+        HBP_HIP(hipStreamEndCapture(ctx->stream, &gr));
+        HBP_HIP(hipGraphInstantiate(&exec[parity], gr, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(gr);
+        gr = nullptr;
+        ++ctx->graphs_recorded;
+      } catch (const Error &) {
+        // nothing of the recording ran: forget host state that it advanced
+        // (the absorbed site held in the work buffer, pair-front copies whose
+        // rearrangement was only recorded) and go on without graphs
+        graph_capturing() = false;
+        hipGraph_t dead = nullptr;
+        (void)hipStreamEndCapture(ctx->stream, &dead);
+        if (dead) (void)hipGraphDestroy(dead);
+        if (gr) (void)hipGraphDestroy(gr);
+        if (exec[parity]) (void)hipGraphExecDestroy(exec[parity]);
+        exec[parity] = nullptr;
+        (void)hipGetLastError();
+        for (auto *b : big) {
+          b->p->abs_site = -1;
+          drop_pair_cache(ctx, b->p);
+        }
+        graphs = replay = false;
+        ++ctx->graphs_dropped;
+      }
     }
-    HBP_HIP(hipGraphLaunch(exec[parity], ctx->stream));
-    HBP_HIP(hipMemcpyAsync(hres.data(), res, act.size() * sizeof(double), hipMemcpyDeviceToHost,
+    if (replay) {
+      HBP_HIP(hipGraphLaunch(exec[parity], ctx->stream));
+      ++ctx->graph_replays;
+    } else
+      enqueue(parity);
+    HBP_HIP(hipMemcpyAsync(hres.data(), res, (size_t)nres * sizeof(double), hipMemcpyDeviceToHost,
                            ctx->stream));
     ctx->sync();
     bool left = false;
-    std::vector<BatchMember *> next;
-    for (size_t i = 0; i < act.size(); ++i) {
-      BatchMember *b = act[i];
+    // true while b iterates on (its sets swapped: E <- G)
+    auto advance = [&](BatchMember *b, double r) {
       b->iters = it;
-      b->history.push_back(hres[i]);
-      b->conv = hres[i] < b->stop ? 1 : 0;
+      b->history.push_back(r);
+      b->conv = r < b->stop ? 1 : 0;
       b->done = b->conv || it == max_iter;
       if (b->done) {
         left = true;  // keeps E, like the reference at its stop (main.cc:138-158)
-      } else {
-        swap_sets(*b);
-        next.push_back(b);
+        return false;
       }
+      swap_sets(*b);
+      return true;
+    };
+    for (auto &g : grp) {
+      std::vector<BatchMember *> next;
+      for (size_t i = 0; i < g.act.size(); ++i)
+        if (advance(g.act[i], hres[g.res_off + i])) next.push_back(g.act[i]);
+      g.act.swap(next);
     }
+    std::vector<BatchMember *> next;
+    for (auto *b : big)
+      if (advance(b, hres[b->slot])) next.push_back(b);
+    big.swap(next);
     if (left) drop_graphs();
-    act.swap(next);
     parity ^= 1;
   }
   drop_graphs();
-  ctx->release(res, std::max<size_t>(mem.size(), 1) * sizeof(double));
-  ctx->release(dist, std::max(nmsgs_all, 1) * sizeof(double));
-  ctx->release(scratch, scratch_bytes);
+  ctx->release(res, res_bytes);
+  for (auto &g : grp) {
+    if (g.scratch) ctx->release(g.scratch, g.scratch_bytes);
+    if (g.dist) ctx->release(g.dist, g.dist_bytes);
+  }
   for (auto &b : mem) {
     ctx->release(b.G, std::max<size_t>(b.p->E_bytes, 16));
-    ctx->release(b.raw, std::max<size_t>(b.p->E_bytes, 16));
+    if (b.raw) ctx->release(b.raw, std::max<size_t>(b.p->E_bytes, 16));
+    if (b.dj) ctx->release(b.dj, 2 * b.p->n_edges * sizeof(MsgJob));
   }
 }
 
@@ -2980,32 +3130,31 @@ hbp_status hbp_peps_bp_run_batched(hbp_ctx *ctx, int n, hbp_peps *const *ps, int
   for (int i = 0; i < n; ++i)
     for (int j = 0; j < i; ++j)
       HBP_REQUIRE(ps[i] != ps[j], HBP_ERR_INVALID_ARGUMENT, "bp batched: a PEPS appears twice");
-  std::vector<BatchMember> mem[2];
+  std::vector<BatchMember> mem;
   std::vector<int> alone;
   for (int i = 0; i < n; ++i) {
     Peps *p = ps[i];
-    if (small_eligible(p)) {
+    const int kind = small_eligible(p) ? HBP_BATCH_SMALL : graph_eligible(p) ? HBP_BATCH_GRAPH : 0;
+    if (kind) {
       BatchMember b;
       b.p = p;
+      b.kind = kind;
       b.index = i;
       b.stop = p->precision == HBP_PREC_MIXED ? std::max(tol, p->mixed_floor) : tol;
-      mem[p->precision == HBP_PREC_MIXED ? 1 : 0].push_back(b);
+      mem.push_back(b);
     } else {
       alone.push_back(i);
     }
   }
-  auto report = [&](int i, int it, int conv, const double *hist, int nh, int in_batch) {
+  auto report = [&](int i, int it, int conv, const double *hist, int nh, int kind) {
     if (iters) iters[i] = it;
     if (converged) converged[i] = conv;
-    if (batched) batched[i] = in_batch;
+    if (batched) batched[i] = kind;
     if (residuals)
       for (int k = 0; k < nh && k < max_iter; ++k) residuals[(size_t)i * max_iter + k] = hist[k];
   };
-  if (!mem[0].empty()) bp_run_batched_impl<double>(ctx, mem[0], max_iter);
-  if (!mem[1].empty()) bp_run_batched_impl<float>(ctx, mem[1], max_iter);
-  for (int q = 0; q < 2; ++q)
-    for (auto &b : mem[q])
-      report(b.index, b.iters, b.conv, b.history.data(), (int)b.history.size(), 1);
+  if (!mem.empty()) bp_run_batched_impl(ctx, mem, max_iter);
+  for (auto &b : mem) report(b.index, b.iters, b.conv, b.history.data(), (int)b.history.size(), b.kind);
   for (int i : alone) {
     int it = 0, conv = 0;
     std::vector<double> h(max_iter, 0.0);

@@ -112,6 +112,8 @@ void *Ctx::alloc(size_t bytes) {
     peak = std::max(peak, in_use);
     return p;
   }
+  HBP_REQUIRE(!graph_capturing(), HBP_ERR_UNSUPPORTED,
+              "device allocation of " + std::to_string(bytes) + " bytes while a graph is recorded");
   void *p = nullptr;
   hipError_t e = hipMallocFromPoolAsync(&p, bytes, pool, stream);
   if (e != hipSuccess && cached > 0) {
@@ -137,7 +139,7 @@ void Ctx::release(void *p, size_t bytes) {
   const size_t b = it != block_sizes.end() ? it->second : (bytes == 0 ? (size_t)16 : bytes);
   if (it != block_sizes.end()) block_sizes.erase(it);
   in_use -= std::min(in_use, b);
-  if (b >= kBigBlock || cached_small + b <= kSmallCacheCap) {
+  if (b >= kBigBlock || cached_small + b <= kSmallCacheCap || graph_capturing()) {
     free_blocks.emplace(b, p);
     cached += b;
     if (b < kBigBlock) cached_small += b;
@@ -155,6 +157,7 @@ void Ctx::flush_blocks() {
 
 void *Ctx::get_scratch(size_t bytes) {
   if (bytes > scratch_bytes) {
+    HBP_REQUIRE(!graph_capturing(), HBP_ERR_UNSUPPORTED, "scratch growth while a graph is recorded");
     if (scratch) release(scratch, scratch_bytes);
     size_t nb = std::max(bytes, scratch_bytes * 2);
     nb = (nb + 4095) & ~(size_t)4095;
@@ -166,6 +169,7 @@ void *Ctx::get_scratch(size_t bytes) {
 
 void *Ctx::get_arena(size_t bytes) {
   if (bytes > arena_bytes) {
+    HBP_REQUIRE(!graph_capturing(), HBP_ERR_UNSUPPORTED, "arena growth while a graph is recorded");
     if (arena) release(arena, arena_bytes);
     arena_bytes = (std::max(bytes, arena_bytes * 2) + 4095) & ~(size_t)4095;
     arena = alloc(arena_bytes);
@@ -173,7 +177,10 @@ void *Ctx::get_arena(size_t bytes) {
   return arena;
 }
 
-void Ctx::sync() { HBP_HIP(hipStreamSynchronize(stream)); }
+void Ctx::sync() {
+  HBP_REQUIRE(!graph_capturing(), HBP_ERR_UNSUPPORTED, "host wait while a graph is recorded");
+  HBP_HIP(hipStreamSynchronize(stream));
+}
 
 size_t Ctx::free_bytes(size_t *total) {
   size_t fr = 0, tot = 0;
@@ -231,9 +238,13 @@ bool &launch_tuning() {
   static thread_local bool on = false;
   return on;
 }
+bool &graph_capturing() {
+  static thread_local bool on = false;
+  return on;
+}
 
 int Ctx::prof_begin(int kind, double flops, double bytes) {
-  if (!prof) return -1;
+  if (!prof || graph_capturing()) return -1;
   if (prof_recs.size() >= kProfMaxLive) prof_fold();
   ProfRec r{prof_event(), prof_event(), kind, flops, bytes, false};
   LaunchTiming &t = launch_timing();
@@ -255,6 +266,7 @@ void Ctx::prof_end(int idx) {
 }
 
 void Ctx::upload(void *dst, const void *src, size_t bytes) {
+  HBP_REQUIRE(!graph_capturing(), HBP_ERR_UNSUPPORTED, "upload while a graph is recorded");
   HBP_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
   HBP_HIP(hipStreamSynchronize(stream));
 }
@@ -721,6 +733,16 @@ hbp_status hbp_ctx_trim(hbp_ctx *ctx) {
   ctx->sync();
   HBP_HIP(hipMemPoolTrimTo(ctx->pool, 0));
   ctx->peak = ctx->in_use;
+  HBP_API_END
+}
+
+hbp_status hbp_ctx_graph_stats(hbp_ctx *ctx, int64_t *recorded, int64_t *replays,
+                               int64_t *dropped) {
+  HBP_API_BEGIN
+  HBP_REQUIRE(ctx, HBP_ERR_INVALID_ARGUMENT, "null context");
+  if (recorded) *recorded = ctx->graphs_recorded;
+  if (replays) *replays = ctx->graph_replays;
+  if (dropped) *dropped = ctx->graphs_dropped;
   HBP_API_END
 }
 

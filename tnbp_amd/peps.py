@@ -13,7 +13,8 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import (HBP_ABSORB_AUTO, HBP_ABSORB_EAGER, HBP_ABSORB_LAZY, HBP_MAX_ORDER,
+from ._lib import (HBP_ABSORB_AUTO, HBP_ABSORB_EAGER, HBP_ABSORB_LAZY, HBP_BATCH_ALONE,
+                   HBP_BATCH_GRAPH, HBP_BATCH_SMALL, HBP_MAX_ORDER,
                    HBP_MODE_FUSED, HBP_MODE_PLAIN, HBP_SCHEDULE_CHECKERBOARD,
                    HBP_SCHEDULE_JACOBI,
                    HBP_PREC_FP64, HBP_PREC_MIXED, HBP_SWEEP_AUTO, HBP_SWEEP_GROUPED,
@@ -23,7 +24,8 @@ from .tci import Context
 __all__ = ["Peps", "InitTensorProductState", "AbsorbTPO", "BeliefPropagation",
            "BeliefPropagationCondition", "bp_loop", "bp_loop_batched", "Truncation", "Measure", "MeasureEdges",
            "HBP_PREC_FP64", "HBP_PREC_MIXED", "HBP_MODE_FUSED", "HBP_MODE_PLAIN",
-           "HBP_SCHEDULE_JACOBI", "HBP_SCHEDULE_CHECKERBOARD"]
+           "HBP_SCHEDULE_JACOBI", "HBP_SCHEDULE_CHECKERBOARD", "HBP_BATCH_ALONE", "HBP_BATCH_SMALL",
+           "HBP_BATCH_GRAPH"]
 
 
 class Peps:
@@ -242,10 +244,12 @@ def bp_loop(ctx, peps: Peps, max_iter: int, tol: float, progress=None):
 
 def bp_loop_batched(ctx, peps_list, max_iter: int, tol: float):
     """The BP loop of several PEPS of one context at once
-    (``hbp_peps_bp_run_batched``): PEPS with bonds <= 16 iterate in lockstep,
-    one graph-replayed persistent launch per iteration for all of them; each
-    gets what ``bp_loop`` gives it alone.  Returns a list of (iterations,
-    residuals, converged, batched) per PEPS."""
+    (``hbp_peps_bp_run_batched``): PEPS in fused mode with the Jacobi schedule
+    and no mixing iterate in lockstep, one replay of a HIP graph per iteration
+    for all of them (those with bonds <= 16 in one persistent launch, the
+    others with their own sweeps); each gets what ``bp_loop`` gives it alone.
+    Returns a list of (iterations, residuals, converged, batched) per PEPS,
+    batched = HBP_BATCH_SMALL, HBP_BATCH_GRAPH or 0 (run on its own)."""
     n = len(peps_list)
     if n == 0:
         return []
@@ -256,7 +260,7 @@ def bp_loop_batched(ctx, peps_list, max_iter: int, tol: float):
     res = np.zeros((n, max_iter), dtype=np.float64)
     check(ctx.lib.hbp_peps_bp_run_batched(ctx.h, n, hs, max_iter, tol, it, res.ctypes.data_as(
         C.POINTER(C.c_double)), conv, bat))
-    return [(it[i], list(res[i, : it[i]]), bool(conv[i]), bool(bat[i])) for i in range(n)]
+    return [(it[i], list(res[i, : it[i]]), bool(conv[i]), int(bat[i])) for i in range(n)]
 
 
 def BPIterate(ctx, peps: Peps, n: int, tol: float, first: bool, last: bool):

@@ -60,9 +60,11 @@ def run_stream(jobs, rate, slots, seed, server="threaded"):
     if server == "batched":
         st["rounds"] = len(srv.rounds)
         st["mean_jobs_per_round"] = float(np.mean([r["jobs"] for r in srv.rounds]))
-        # share of (job, layer) BP loops that ran in the batched launch
-        st["batched_fraction"] = (sum(r["batched"] for r in srv.rounds) /
-                                  max(1, sum(r["jobs"] for r in srv.rounds)))
+        # share of (job, layer) BP loops that ran in the small-bond batched
+        # launch, and in the iteration graph with their own sweeps
+        nloops = max(1, sum(r["jobs"] for r in srv.rounds))
+        st["batched_fraction"] = sum(r["batched"] for r in srv.rounds) / nloops
+        st["graph_fraction"] = sum(r.get("graph", 0) for r in srv.rounds) / nloops
     return st, res
 
 

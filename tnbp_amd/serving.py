@@ -34,7 +34,8 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import circuits, pauli
-from ._lib import HBP_ABSORB_EAGER, HBP_ABSORB_LAZY, HBP_PREC_FP64, HBP_PREC_MIXED
+from ._lib import (HBP_ABSORB_EAGER, HBP_ABSORB_LAZY, HBP_BATCH_GRAPH, HBP_BATCH_SMALL, HBP_PREC_FP64,
+                   HBP_PREC_MIXED)
 from .peps import AbsorbTPO, InitTensorProductState, Truncation, bp_loop
 from .tci import Context, device_memory
 
@@ -330,10 +331,10 @@ class BatchedServer:
 
     One scheduler thread keeps every admitted job's PEPS resident and runs
     rounds: each resident job absorbs its next TPO slice, then the BP loops of
-    all resident jobs run as one ``hbp_peps_bp_run_batched`` call (jobs whose
-    bonds are <= 16 iterate together: one persistent launch over every site
-    of every such job per BP iteration, replayed from a HIP graph; larger
-    ones run their own fused sweeps in the same call), then each job
+    all resident jobs run as one ``hbp_peps_bp_run_batched`` call (one
+    replay of a HIP graph per BP iteration for all of them: jobs whose bonds
+    are <= 16 in one persistent launch over every site of every such job,
+    larger ones with their own fused sweeps recorded in the same graph), then each job
     truncates; a job whose last slice is done is measured and leaves.  Queued
     jobs are admitted by footprint at every round boundary, so a job that
     arrives while others run joins the next round.  A job's result does not
@@ -444,16 +445,17 @@ class BatchedServer:
                 groups = {}
                 for r in active:
                     groups.setdefault((r["job"].max_bp_iterations, r["job"].bp_tolerance), []).append(r)
-                nb = 0
+                nb = ng = 0
                 for (mi, tol), rs in groups.items():
                     out = bp_loop_batched(ctx, [r["peps"] for r in rs], mi, tol)
                     for r, (it, res, conv, bat) in zip(rs, out):
                         r["iters"].append(it)
-                        nb += int(bat)
+                        nb += int(bat == HBP_BATCH_SMALL)
+                        ng += int(bat == HBP_BATCH_GRAPH)
                 for r in active:
                     Truncation(ctx, r["peps"], r["job"].chi, r["job"].sv_min)
                     r["layer"] += 1
-                self.rounds.append({"jobs": len(active), "batched": nb,
+                self.rounds.append({"jobs": len(active), "batched": nb, "graph": ng,
                                     "seconds": time.perf_counter() - t0})
                 still = []
                 for r in active:

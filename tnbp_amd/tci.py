@@ -42,6 +42,12 @@ class Context:
         """Give the pool's unused memory back to the device."""
         check(self.lib.hbp_ctx_trim(self.h))
 
+    def graph_stats(self):
+        """(recorded, replays, dropped) HIP graphs of batched BP loops so far."""
+        r, n, d = C.c_int64(), C.c_int64(), C.c_int64()
+        check(self.lib.hbp_ctx_graph_stats(self.h, C.byref(r), C.byref(n), C.byref(d)))
+        return r.value, n.value, d.value
+
     def __enter__(self):
         return self
 
