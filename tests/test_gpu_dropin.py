@@ -7,6 +7,9 @@ header (tnbp_amd/dropin), run on the GPU.
   runs the reference's op sequence one TCI call at a time.  Both run config 1
   (4x4 kicked Ising, chi = 4) and must print the golden fixture's iteration
   counts and <X>, <Z>, <ZZ> (tolerance 1e-10 relative, floor 1).
+* kicked_ising_exvalue_single: the drivers' own USE_SINGLE build
+  (complex<float> tensors) on config 1 with BP to 1e-5, within 1e-5 of the
+  FP64 fixture.
 * estimator (fused and plain): a QASM circuit on a tree (u3 / rx / h / cx /
   rzz layers separated by barriers) and a sparse-Pauli file of weight-1 and
   nearest-neighbour weight-2 terms; on a tree BP is exact, so every printed
@@ -39,6 +42,45 @@ def _bin(name):
     if not os.path.exists(path):
         pytest.skip(f"drop-in driver {name} not built (needs the reference sources at build time)")
     return path
+
+
+def _kicked_values(stdout):
+    vals = {}
+    for m in re.finditer(r"measurement of (X|Z) at site (\d+) time step (\d+) = (\S+)", stdout):
+        vals[(m.group(1), int(m.group(2)), int(m.group(3)))] = float(m.group(4))
+    for m in re.finditer(r"measurement of ZZ at sites \((\d+),(\d+)\) time step (\d+) = (\S+)", stdout):
+        vals[("ZZ", (int(m.group(1)), int(m.group(2))), int(m.group(3)))] = float(m.group(4))
+    return vals
+
+
+def test_reference_kicked_driver_single_precision():
+    """The drivers' own USE_SINGLE build (complex<float> tensors through the
+    TCI calls; contractions and decompositions computed in complex128 and
+    rounded to single precision) on config 1 with BP to 1e-5 (single
+    precision does not reach the fixture's 1e-10): every <X>, <Z>, <ZZ> of
+    every step within 1e-5 of the FP64 golden fixture (relative, floor 1)."""
+    exe = _bin("kicked_ising_exvalue_single")
+    g = json.load(open(os.path.join(ROOT, "tests", "golden", "config1_kicked_4x4_chi4.json")))
+    a = g["args"]
+    cmd = [exe, "--lattice", "square", "--L", str(a["L"]), "--num_steps", str(a["num_steps"]),
+           "--Jz", repr(a["Jz"]), "--hz", repr(a["hz"]), "--hx", repr(a["hx"]), "--dt", repr(a["dt"]),
+           "--max_bp_iterations", "200", "--bp_tolerance", "1e-5",
+           "--max_bond_dim", str(a["max_bond_dim"]), "--sv_min", repr(a["sv_min"])]
+    announce("drop-in single precision " + " ".join(cmd[1:]))
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    vals = _kicked_values(r.stdout)
+    worst, n = 0.0, 0
+    for s in g["steps"]:
+        for op in ("Z", "X"):
+            for i, want in enumerate(s[op.lower()]):
+                worst = max(worst, abs(vals[(op, i, s["step"])] - want) / max(abs(want), 1.0))
+                n += 1
+        for (u, v), want in zip(g["edges"], s["zz"]):
+            worst = max(worst, abs(vals[("ZZ", (u, v), s["step"])] - want) / max(abs(want), 1.0))
+            n += 1
+    announce(f"drop-in single precision: {n} values, worst relative error vs FP64 {worst:.3e}")
+    assert worst <= 1e-5
 
 
 @pytest.mark.parametrize("name", ["kicked_ising_exvalue", "kicked_ising_exvalue_plain"])

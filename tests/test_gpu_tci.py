@@ -187,3 +187,65 @@ def test_context_block_cache_accounting():
         del keep
     finally:
         c.close()
+
+
+def test_single_precision_and_real_tensors(ctx):
+    """Element types of the drivers' USE_SINGLE build (complex<float>, real
+    float) and real tensors through contract, svd, trunc_svd, trace and
+    linear_combine: the operands are taken exactly to complex128, so each
+    result equals the oracle's on the same values up to the final rounding
+    (1e-6 relative for single precision, 1e-12 for double); the result type
+    follows the operands (complex if either is, single if both are)."""
+    from tnbp_amd import tci as T
+    from tnbp_amd._lib import HBP_C64, HBP_C128, HBP_F32, HBP_F64
+    rng = np.random.default_rng(41)
+    a = _rand(rng, (8, 6, 5, 2)).astype(np.complex64)
+    e = _rand(rng, (6, 7)).astype(np.complex64)
+    ta, te = T.Tensor.from_numpy(ctx, a), T.Tensor.from_numpy(ctx, e)
+    assert ta.dtype == HBP_C64
+    got = T.contract(ctx, ta, [0, -1, 2, 3], te, [-1, 1], [0, 1, 2, 3])
+    assert got.dtype == HBP_C64
+    want = ot.contract(a.astype(np.complex128), [0, -1, 2, 3], e.astype(np.complex128), [-1, 1],
+                       [0, 1, 2, 3])
+    assert np.max(np.abs(got.to_numpy() - want)) <= 1e-6 * np.max(np.abs(want))
+    # message-shaped contraction of a single tensor with a double one -> complex128
+    b = _rand(rng, (8, 6, 5, 2))
+    got = T.contract(ctx, ta, [0, -1, -2, -3], T.Tensor.from_numpy(ctx, b), [1, -1, -2, -3], [0, 1])
+    assert got.dtype == HBP_C128
+    want = ot.contract(a.astype(np.complex128), [0, -1, -2, -3], b, [1, -1, -2, -3], [0, 1])
+    assert np.max(np.abs(got.to_numpy() - want)) <= 1e-12 * np.max(np.abs(want))
+    # real x real stays real
+    r1, r2 = rng.standard_normal((5, 4)), rng.standard_normal((4, 3))
+    got = T.contract(ctx, T.Tensor.from_numpy(ctx, r1), [0, -1], T.Tensor.from_numpy(ctx, r2), [-1, 1],
+                     [0, 1])
+    assert got.dtype == HBP_F64
+    np.testing.assert_allclose(got.to_numpy(), r1 @ r2, atol=1e-13)
+    got = T.contract(ctx, T.Tensor.from_numpy(ctx, r1.astype(np.float32)), [0, -1],
+                     T.Tensor.from_numpy(ctx, r2.astype(np.float32)), [-1, 1], [0, 1])
+    assert got.dtype == HBP_F32
+    np.testing.assert_allclose(got.to_numpy(), r1.astype(np.float32).astype(np.float64)
+                               @ r2.astype(np.float32).astype(np.float64), rtol=1e-6, atol=1e-6)
+    # svd / trunc_svd of a complex64 matrix: complex64 factors, float32 values
+    m = _rand(rng, (40, 24)).astype(np.complex64)
+    u, s, vd = T.svd(ctx, T.Tensor.from_numpy(ctx, m), 1)
+    assert (u.dtype, s.dtype, vd.dtype) == (HBP_C64, HBP_F32, HBP_C64)
+    U, S, Vd = u.to_numpy(), s.to_numpy(), vd.to_numpy()
+    rec = (U.astype(np.complex128) * S.astype(np.float64)) @ Vd.astype(np.complex128)
+    assert np.max(np.abs(rec - m)) <= 1e-5 * np.max(np.abs(m))
+    np.testing.assert_allclose(S, np.linalg.svd(m.astype(np.complex128), compute_uv=False),
+                               rtol=1e-5)
+    u, s, vd, err = T.trunc_svd(ctx, T.Tensor.from_numpy(ctx, m), 1, 10, 0.0)
+    assert (u.dtype, s.dtype, vd.dtype) == (HBP_C64, HBP_F32, HBP_C64) and s.shape == (10,)
+    # trace and linear_combine keep complex64
+    t = _rand(rng, (4, 3, 4)).astype(np.complex64)
+    got = T.trace(ctx, T.Tensor.from_numpy(ctx, t), [(0, 2)])
+    assert got.dtype == HBP_C64
+    np.testing.assert_allclose(got.to_numpy(), np.einsum("iji->j", t.astype(np.complex128)),
+                               rtol=1e-6, atol=1e-6)
+    x, y = _rand(rng, (3, 5)).astype(np.complex64), _rand(rng, (3, 5)).astype(np.complex64)
+    got = T.linear_combine(ctx, [T.Tensor.from_numpy(ctx, x), T.Tensor.from_numpy(ctx, y)],
+                           [0.5 - 1j, 2.0])
+    assert got.dtype == HBP_C64
+    np.testing.assert_allclose(got.to_numpy(), (0.5 - 1j) * x.astype(np.complex128) + 2.0 * y,
+                               rtol=1e-6, atol=1e-6)
+    announce("single-precision and real tensors through contract / svd / trace / linear_combine")

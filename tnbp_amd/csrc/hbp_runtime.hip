@@ -414,10 +414,59 @@ static void message_c128(Ctx *ctx, const void *T, const void *V, const int32_t *
   launch_reduce_partials(ctx->stream, parts, n, D * D, M);
 }
 
+// Element types other than complex128 (the apps' single-precision build,
+// real tensors) in contract, svd, linear_combine and trace: the operands are
+// converted to complex128 (exactly), the complex128 kernels run, and each
+// result is rounded to the operands' type: complex when an operand is
+// complex, single precision when every operand is.
+static bool single_dtype(int d) { return d == HBP_C64 || d == HBP_F32; }
+static int common_dtype(int a, int b) {
+  const bool cplx = dtype_complex(a) || dtype_complex(b);
+  const bool single = single_dtype(a) && single_dtype(b);
+  return cplx ? (single ? HBP_C64 : HBP_C128) : (single ? HBP_F32 : HBP_F64);
+}
+// x itself when complex128, else its complex128 copy in tmp
+static const Tensor *as_c128(Ctx *ctx, const Tensor *x, Tensor *tmp) {
+  if (x->dtype == HBP_C128) return x;
+  tensor_resize(ctx, tmp, HBP_C128, x->order, x->shape);
+  launch_convert(ctx->stream, x->data, x->dtype, tmp->data, HBP_C128, x->size);
+  return tmp;
+}
+// round a complex128 (or float64) result to dtype in place
+static void round_to(Ctx *ctx, Tensor *t, int dtype) {
+  if (t->dtype == dtype) return;
+  const size_t bytes = std::max<size_t>(t->size, 1) * dtype_size(dtype);
+  void *d = ctx->alloc(bytes);
+  launch_convert(ctx->stream, t->data, t->dtype, d, dtype, t->size);
+  ctx->release(t->data, t->bytes);
+  t->data = d;
+  t->bytes = bytes;
+  t->dtype = dtype;
+}
+static void drop_tmp(Ctx *ctx, Tensor *tmp) {
+  if (tmp->data) ctx->release(tmp->data, tmp->bytes);
+  tmp->data = nullptr;
+}
+
+static void contract_c128(Ctx *ctx, const Tensor *a, const int32_t *la, const Tensor *b,
+                          const int32_t *lb, Tensor *c, const int32_t *lc, int nc);
 static void contract_impl(Ctx *ctx, const Tensor *a, const int32_t *la, const Tensor *b,
                           const int32_t *lb, Tensor *c, const int32_t *lc, int nc) {
-  HBP_REQUIRE(a->dtype == HBP_C128 && b->dtype == HBP_C128, HBP_ERR_UNSUPPORTED,
-              "contract: complex128 operands only");
+  if (a->dtype == HBP_C128 && b->dtype == HBP_C128) {
+    contract_c128(ctx, a, la, b, lb, c, lc, nc);
+    return;
+  }
+  const int dt = common_dtype(a->dtype, b->dtype);
+  Tensor ta, tb;
+  const Tensor *pa = as_c128(ctx, a, &ta), *pb = as_c128(ctx, b, &tb);
+  contract_c128(ctx, pa, la, pb, lb, c, lc, nc);
+  drop_tmp(ctx, &ta);
+  drop_tmp(ctx, &tb);
+  round_to(ctx, c, dt);
+}
+
+static void contract_c128(Ctx *ctx, const Tensor *a, const int32_t *la, const Tensor *b,
+                          const int32_t *lb, Tensor *c, const int32_t *lc, int nc) {
   HBP_REQUIRE(nc <= HBP_MAX_ORDER, HBP_ERR_INVALID_ARGUMENT, "contract: output order too large");
   ContractPlan p = plan_labels(a, la, b, lb, lc, nc);
   int32_t out_shape[HBP_MAX_ORDER];
@@ -965,17 +1014,24 @@ hbp_status hbp_linear_combine(hbp_ctx *ctx, int n, const hbp_tensor *const *ins,
                               const double *coefs, hbp_tensor *out) {
   HBP_API_BEGIN
   HBP_REQUIRE(n >= 1, HBP_ERR_INVALID_ARGUMENT, "linear_combine: no inputs");
+  int dt = ins[0]->dtype;
   for (int i = 0; i < n; ++i) {
-    HBP_REQUIRE(ins[i]->dtype == HBP_C128, HBP_ERR_UNSUPPORTED, "linear_combine: complex128 only");
     HBP_REQUIRE(ins[i]->size == ins[0]->size, HBP_ERR_INVALID_ARGUMENT,
                 "linear_combine: shapes differ");
+    dt = common_dtype(dt, ins[i]->dtype);
   }
+  // the coefficients are complex: the result is complex in any case
+  dt = single_dtype(dt) ? HBP_C64 : HBP_C128;
   const size_t bytes = ins[0]->size * 16;
   double2 *acc = (double2 *)ctx->alloc(bytes);
   launch_fill(ctx->stream, acc, ins[0]->size, HBP_C128, 0.0, 0.0);
-  for (int i = 0; i < n; ++i)
-    launch_axpy_c128(ctx->stream, (const double2 *)ins[i]->data, acc, ins[0]->size,
+  for (int i = 0; i < n; ++i) {
+    Tensor ti;
+    const Tensor *x = as_c128(ctx, ins[i], &ti);
+    launch_axpy_c128(ctx->stream, (const double2 *)x->data, acc, ins[0]->size,
                      coefs[2 * i], coefs[2 * i + 1]);
+    drop_tmp(ctx, &ti);
+  }
   const int order = ins[0]->order;
   int32_t sh[HBP_MAX_ORDER];
   for (int d = 0; d < order; ++d) sh[d] = ins[0]->shape[d];
@@ -986,6 +1042,7 @@ hbp_status hbp_linear_combine(hbp_ctx *ctx, int n, const hbp_tensor *const *ins,
   out->order = order;
   out->size = ins[0]->size;
   for (int d = 0; d < HBP_MAX_ORDER; ++d) out->shape[d] = d < order ? sh[d] : 0;
+  round_to(ctx, out, dt);
   touch(out);
   HBP_API_END
 }
@@ -993,7 +1050,9 @@ hbp_status hbp_linear_combine(hbp_ctx *ctx, int n, const hbp_tensor *const *ins,
 hbp_status hbp_trace(hbp_ctx *ctx, const hbp_tensor *in, int npairs, const int32_t *pairs,
                      hbp_tensor *out) {
   HBP_API_BEGIN
-  HBP_REQUIRE(in->dtype == HBP_C128, HBP_ERR_UNSUPPORTED, "trace: complex128 only");
+  const int dt = in->dtype;
+  Tensor tin;
+  const Tensor *x = as_c128(ctx, in, &tin);
   std::vector<int> used(in->order, 0);
   for (int p = 0; p < npairs; ++p) {
     const int i = pairs[2 * p], j = pairs[2 * p + 1];
@@ -1008,7 +1067,8 @@ hbp_status hbp_trace(hbp_ctx *ctx, const hbp_tensor *in, int npairs, const int32
     if (!used[d]) sh[order++] = in->shape[d];
   const size_t n = shape_size(order, sh);
   double2 *buf = (double2 *)ctx->alloc(n * 16);
-  launch_trace(ctx->stream, (const double2 *)in->data, buf, in->order, in->shape, npairs, pairs);
+  launch_trace(ctx->stream, (const double2 *)x->data, buf, in->order, in->shape, npairs, pairs);
+  drop_tmp(ctx, &tin);
   if (out->data) ctx->release(out->data, out->bytes);
   out->data = buf;
   out->bytes = n * 16;
@@ -1016,6 +1076,7 @@ hbp_status hbp_trace(hbp_ctx *ctx, const hbp_tensor *in, int npairs, const int32
   out->order = order;
   out->size = n;
   for (int d = 0; d < HBP_MAX_ORDER; ++d) out->shape[d] = d < order ? sh[d] : 0;
+  round_to(ctx, out, dt);
   touch(out);
   HBP_API_END
 }
@@ -1024,7 +1085,11 @@ hbp_status hbp_diag(hbp_ctx *ctx, const hbp_tensor *vec, hbp_tensor *out) {
   HBP_API_BEGIN
   const int64_t n = (int64_t)vec->size;
   HBP_REQUIRE(n <= (1 << 15), HBP_ERR_UNSUPPORTED, "diag: vector too long");
-  const int dtype = (vec->dtype == HBP_F64 && out->dtype == HBP_C128) ? HBP_C128 : vec->dtype;
+  // a real vector into a complex tensor gives the complex matrix (tci::diag
+  // then to_cplx in one call)
+  const int dtype = (!dtype_complex(vec->dtype) && dtype_complex(out->dtype))
+                        ? (vec->dtype == HBP_F32 ? HBP_C64 : HBP_C128)
+                        : vec->dtype;
   const int32_t sh[2] = {(int32_t)n, (int32_t)n};
   const size_t es = dtype_size(dtype);
   void *buf = ctx->alloc((size_t)n * n * es);
@@ -1083,10 +1148,26 @@ hbp_status hbp_get_elem(hbp_ctx *ctx, const hbp_tensor *t, const int32_t *coors,
   HBP_API_END
 }
 
+// svd_impl on a complex copy of a complex64 tensor; u, vdag rounded to
+// complex64 and s to float32 (tensor_traits of complex<float>: real_ten_t
+// float).  Real tensors are refused (the reference decomposes complex ones).
+static void svd_any(Ctx *ctx, const Tensor *a, int nrows, Tensor *u, Tensor *s, Tensor *vd,
+                    int keep_max, double s_min, double *trunc_err) {
+  HBP_REQUIRE(dtype_complex(a->dtype), HBP_ERR_UNSUPPORTED, "svd: complex tensors only");
+  Tensor ta;
+  svd_impl(ctx, as_c128(ctx, a, &ta), nrows, u, s, vd, keep_max, s_min, trunc_err);
+  drop_tmp(ctx, &ta);
+  if (a->dtype == HBP_C64) {
+    round_to(ctx, u, HBP_C64);
+    round_to(ctx, vd, HBP_C64);
+    round_to(ctx, s, HBP_F32);
+  }
+}
+
 hbp_status hbp_svd(hbp_ctx *ctx, const hbp_tensor *a, int nrows, hbp_tensor *u, hbp_tensor *s,
                    hbp_tensor *vdag) {
   HBP_API_BEGIN
-  svd_impl(ctx, a, nrows, u, s, vdag, -1, 0.0, nullptr);
+  svd_any(ctx, a, nrows, u, s, vdag, -1, 0.0, nullptr);
   HBP_API_END
 }
 
@@ -1135,7 +1216,7 @@ hbp_status hbp_trunc_svd(hbp_ctx *ctx, const hbp_tensor *a, int nrows, int chi_m
                          double *trunc_err) {
   HBP_API_BEGIN
   HBP_REQUIRE(chi_max >= 1, HBP_ERR_INVALID_ARGUMENT, "trunc_svd: chi_max < 1");
-  svd_impl(ctx, a, nrows, u, s, vdag, chi_max, s_min, trunc_err);
+  svd_any(ctx, a, nrows, u, s, vdag, chi_max, s_min, trunc_err);
   HBP_API_END
 }
 

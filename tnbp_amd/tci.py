@@ -13,7 +13,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import HBP_C128, HBP_F64, HBP_MAX_ORDER, check, i32
+from ._lib import HBP_C64, HBP_C128, HBP_F32, HBP_F64, HBP_MAX_ORDER, check, i32
 
 
 class Context:
@@ -73,7 +73,7 @@ def device_memory(device: int = 0):
 
 
 class Tensor:
-    """A device tensor (column-major); ``dtype`` HBP_C128 or HBP_F64."""
+    """A device tensor (column-major); ``dtype`` HBP_C128, HBP_F64, HBP_C64 or HBP_F32."""
 
     def __init__(self, ctx: Context, shape=(), dtype=HBP_C128):
         self.ctx = ctx
@@ -111,18 +111,24 @@ class Tensor:
 
     def to_numpy(self):
         dt, order, shape, size = self.info()
-        npdt = np.complex128 if dt == HBP_C128 else np.float64
+        npdt = {HBP_C128: np.complex128, HBP_F64: np.float64, HBP_C64: np.complex64,
+                HBP_F32: np.float32}[dt]
         out = np.empty(size, dtype=npdt)
         check(self.ctx.lib.hbp_tensor_download(self.ctx.h, self.h, out.ctypes.data_as(C.c_void_p)))
         return out.reshape(shape, order="F") if order else out.reshape(())
 
     @staticmethod
     def from_numpy(ctx: Context, a):
+        """complex64 / float32 arrays give single-precision tensors, every
+        other array complex128 or float64."""
         a = np.asarray(a)
         real = not np.iscomplexobj(a)
-        t = Tensor(ctx, a.shape, HBP_F64 if real else HBP_C128)
-        buf = np.ascontiguousarray(a.reshape(-1, order="F"),
-                                   dtype=np.float64 if real else np.complex128)
+        single = a.dtype in (np.complex64, np.float32)
+        dt, npdt = {(True, False): (HBP_F64, np.float64), (False, False): (HBP_C128, np.complex128),
+                    (True, True): (HBP_F32, np.float32), (False, True): (HBP_C64, np.complex64)}[
+                        (real, single)]
+        t = Tensor(ctx, a.shape, dt)
+        buf = np.ascontiguousarray(a.reshape(-1, order="F"), dtype=npdt)
         check(ctx.lib.hbp_tensor_upload(ctx.h, t.h, buf.ctypes.data_as(C.c_void_p)))
         return t
 
