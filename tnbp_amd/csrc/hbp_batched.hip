@@ -25,25 +25,25 @@ template <typename S> struct CT;
 template <> struct CT<double> { using type = double2; };
 template <> struct CT<float> { using type = float2; };
 
-// Out[a, j, b] = sum_i In[a, i, b] E[i, j] over bond `l` (In: L x D x R)
-// with E staged in LDS as complex double; every thread of the block takes
-// outputs tid, tid + 256, ...  All reads of In happen-before the barrier at
-// the caller's next step.
+// Out[a, j, b] = sum_i In[a, i, b] E[i, j] over bond `l` (In: L x Din x R,
+// Out: L x Dout x R) with E staged in LDS as complex double, E[i + Din j];
+// every thread of the block takes outputs tid, tid + 256, ...  All reads of
+// In happen-before the barrier at the caller's next step.
 template <typename S>
 __device__ void small_mode_product(const typename CT<S>::type *__restrict__ in,
-                                   typename CT<S>::type *__restrict__ out, int64_t L, int D,
-                                   int64_t R, const double2 *__restrict__ Es) {
+                                   typename CT<S>::type *__restrict__ out, int64_t L, int Din,
+                                   int Dout, int64_t R, const double2 *__restrict__ Es) {
   using C = typename CT<S>::type;
-  const int64_t n = L * D * R;
+  const int64_t n = L * Dout * R;
   for (int64_t o = threadIdx.x; o < n; o += blockDim.x) {
     const int64_t a = o % L, t = o / L;
-    const int j = (int)(t % D);
-    const int64_t b = t / D;
-    const C *src = in + a + L * D * b;
+    const int j = (int)(t % Dout);
+    const int64_t b = t / Dout;
+    const C *src = in + a + L * Din * b;
     double re = 0.0, im = 0.0;
-    for (int i = 0; i < D; ++i) {
+    for (int i = 0; i < Din; ++i) {
       const C x = src[L * i];
-      const double2 e = Es[i + D * j];
+      const double2 e = Es[i + Din * j];
       re += (double)x.x * e.x - (double)x.y * e.y;
       im += (double)x.x * e.y + (double)x.y * e.x;
     }
@@ -117,7 +117,7 @@ __global__ __launch_bounds__(256) void k_sweep_small(const SmallSiteJob *__restr
         const double2 *E = job.E + job.in_off[l];
         for (int e = threadIdx.x; e < D * D; e += blockDim.x) Es[e] = E[e];
         __syncthreads();
-        small_mode_product<S>(cur, nxt, L, D, R, Es);
+        small_mode_product<S>(cur, nxt, L, D, D, R, Es);
         __syncthreads();
         cur = nxt;
         nxt = nxt == bufA ? bufB : bufA;
@@ -130,7 +130,86 @@ __global__ __launch_bounds__(256) void k_sweep_small(const SmallSiteJob *__restr
   }
 }
 
+// Projectors into the sites of a truncation, bonds <= SMALL_D_MAX: one
+// workgroup per site (striding over the sites), the z mode products in order
+// through the workgroup's ping-pong buffers, then the tensor's squared norm
+// summed in FP64 in a fixed order (every thread its strided elements, then a
+// fixed tree) and the normalised tensor written to the site's new storage.
+template <typename S>
+__global__ __launch_bounds__(256) void k_project_small(const SmallProjJob *__restrict__ jobs,
+                                                       int njobs, void *__restrict__ scratch,
+                                                       int64_t slab_elems) {
+  using C = typename CT<S>::type;
+  __shared__ double2 Es[SMALL_D_MAX * SMALL_D_MAX];
+  __shared__ double red[256];
+  C *bufA = (C *)scratch + (int64_t)blockIdx.x * 2 * slab_elems;
+  C *bufB = bufA + slab_elems;
+  const int tid = threadIdx.x;
+  for (int w = blockIdx.x; w < njobs; w += gridDim.x) {
+    const SmallProjJob job = jobs[w];
+    const int z = job.z;
+    int dims[5];
+    for (int d = 0; d <= z; ++d) dims[d] = job.dims[d];
+    const C *cur = (const C *)job.V;
+    C *nxt = bufA;
+    for (int k = 0; k < z; ++k) {
+      int64_t L = 1, R = 1;
+      for (int d = 0; d < k; ++d) L *= dims[d];
+      for (int d = k + 1; d <= z; ++d) R *= dims[d];
+      const int Din = dims[k], Dout = job.keep[k];
+      const double2 *P = job.P[k];
+      for (int e = tid; e < Din * Dout; e += blockDim.x) {
+        const int i = e % Din, j = e / Din;
+        Es[e] = job.trans[k] ? P[j + Dout * i] : P[i + Din * j];
+      }
+      __syncthreads();
+      small_mode_product<S>(cur, nxt, L, Din, Dout, R, Es);
+      __syncthreads();
+      dims[k] = Dout;
+      cur = nxt;
+      nxt = nxt == bufA ? bufB : bufA;
+    }
+    int64_t n = 1;
+    for (int d = 0; d <= z; ++d) n *= dims[d];
+    double s2 = 0.0;
+    for (int64_t o = tid; o < n; o += blockDim.x) {
+      const C x = cur[o];
+      s2 += (double)x.x * x.x + (double)x.y * x.y;
+    }
+    red[tid] = s2;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+      if (tid < h) red[tid] += red[tid + h];
+      __syncthreads();
+    }
+    const double v = red[0];
+    const double sc = v > 0.0 ? 1.0 / sqrt(v) : 1.0;
+    C *out = (C *)job.out;
+    for (int64_t o = tid; o < n; o += blockDim.x) {
+      const C x = cur[o];
+      C y;
+      y.x = (S)((double)x.x * sc);
+      y.y = (S)((double)x.y * sc);
+      out[o] = y;
+    }
+    __syncthreads();  // red[0] and the buffers are reused by the next site
+  }
+}
+
 }  // namespace
+
+template <typename S>
+void launch_project_small(hipStream_t st, const SmallProjJob *jobs_dev, int njobs, void *scratch,
+                          int64_t slab_elems, int grid) {
+  if (njobs <= 0) return;
+  hipLaunchKernelGGL(k_project_small<S>, dim3((unsigned)grid), dim3(256), 0, st, jobs_dev, njobs,
+                     scratch, slab_elems);
+  HBP_HIP(hipGetLastError());
+}
+template void launch_project_small<double>(hipStream_t, const SmallProjJob *, int, void *, int64_t,
+                                           int);
+template void launch_project_small<float>(hipStream_t, const SmallProjJob *, int, void *, int64_t,
+                                          int);
 
 template <typename S>
 void launch_sweep_small(hipStream_t st, const SmallSiteJob *jobs_dev, int njobs, void *scratch,

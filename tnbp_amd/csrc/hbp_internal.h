@@ -268,6 +268,22 @@ struct SmallSiteJob {
 template <typename S>
 void launch_sweep_small(hipStream_t st, const SmallSiteJob *jobs_dev, int njobs, void *scratch,
                         int64_t slab_elems, int grid);
+// Truncation of one site with bonds <= SMALL_D_MAX (truncation.h:156-217):
+// out = normalise(V x_0 P_0 x_1 ... x_{z-1} P_{z-1}) with the projector of
+// every bond (dims[k] x keep[k], complex double: P[i + dims[k] j], or
+// P[j + keep[k] i] where trans[k]), one workgroup per site.
+struct SmallProjJob {
+  const void *V;
+  void *out;
+  const double2 *P[4];
+  int dims[5];  // bond dims before the projectors, then the physical dim at index z
+  int keep[4];
+  int trans[4];
+  int z;
+};
+template <typename S>
+void launch_project_small(hipStream_t st, const SmallProjJob *jobs_dev, int njobs, void *scratch,
+                          int64_t slab_elems, int grid);
 // Grids the batched sweep is launched with: GRID_MULT[v] workgroups per CU
 // (variant v), at most one per work item; scratch for the largest.
 constexpr int SMALL_GRID_VARIANTS = 3;

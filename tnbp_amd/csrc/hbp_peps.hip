@@ -283,12 +283,11 @@ __device__ __forceinline__ uint32_t fast_div(uint32_t n, uint32_t m, uint32_t s)
   return (uint32_t)(((uint64_t)__umulhi(n, m) + n) >> s);
 }
 
-// V'[(v o)..., p'] = sum_p O[o..., p, p'] V[v..., p]  (apply.h:52-66)
+// V'[(v o)..., p'] = sum_p O[o..., p, p'] V[v..., p]  (apply.h:52-66), element e
 template <typename C>
-__global__ void k_absorb_site(const C *__restrict__ V, const double2 *__restrict__ O,
-                              C *__restrict__ Vn, size_t n, AbsorbArgs a) {
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
-       e += (size_t)gridDim.x * blockDim.x) {
+__device__ __forceinline__ C absorb_elem(const C *__restrict__ V, const double2 *__restrict__ O,
+                                         size_t e, const AbsorbArgs &a) {
+  {
     int64_t vb = 0, ob = 0;
     int pp;
     if (a.fast) {
@@ -325,7 +324,47 @@ __global__ void k_absorb_site(const C *__restrict__ V, const double2 *__restrict
     C out;
     out.x = re;
     out.y = im;
-    Vn[e] = out;
+    return out;
+  }
+}
+
+template <typename C>
+__global__ void k_absorb_site(const C *__restrict__ V, const double2 *__restrict__ O,
+                              C *__restrict__ Vn, size_t n, AbsorbArgs a) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (size_t)gridDim.x * blockDim.x)
+    Vn[e] = absorb_elem(V, O, e, a);
+}
+
+// Every site of an absorption in one launch: job j owns blocks
+// [first[j], first[j + 1]) of the grid, ABS_EPB elements each (the last one
+// partial); a block finds its job by binary search over first (first[0] = 0,
+// first[njobs] = the grid size, each job at least one block).
+constexpr int ABS_EPB = 1024;
+struct AbsorbJob {
+  const void *V;
+  const double2 *O;
+  void *Vn;
+  size_t n;
+  AbsorbArgs a;
+};
+template <typename C>
+__global__ __launch_bounds__(256) void k_absorb_batch(const AbsorbJob *__restrict__ jobs,
+                                                      const int *__restrict__ first, int njobs) {
+  const int b = blockIdx.x;
+  int lo = 0, hi = njobs;  // first[lo] <= b < first[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (first[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  const AbsorbJob &j = jobs[lo];
+  const size_t e0 = (size_t)(b - first[lo]) * ABS_EPB;
+  const C *V = (const C *)j.V;
+  C *Vn = (C *)j.Vn;
+  for (int i = threadIdx.x; i < ABS_EPB; i += blockDim.x) {
+    const size_t e = e0 + i;
+    if (e < j.n) Vn[e] = absorb_elem(V, j.O, e, j.a);
   }
 }
 
@@ -787,6 +826,33 @@ static const void *site_data(Ctx *ctx, Peps *p, int s) {
   materialize(ctx, p, s, p->abs_buf);
   p->abs_site = s;
   return p->abs_buf;
+}
+
+// The absorptions of `jobs` (sites of p, any sizes) in one launch of
+// k_absorb_batch; the same arithmetic per element as materialize().
+static void absorb_batch(Ctx *ctx, Peps *p, const std::vector<AbsorbJob> &jobs) {
+  const int nj = (int)jobs.size();
+  if (nj == 0) return;
+  std::vector<int> first(nj + 1, 0);
+  for (int j = 0; j < nj; ++j) {
+    const size_t nb = (jobs[j].n + ABS_EPB - 1) / ABS_EPB;
+    HBP_REQUIRE(nb >= 1 && (size_t)first[j] + nb < (size_t)INT32_MAX, HBP_ERR_UNSUPPORTED,
+                "absorb: batched grid too large");
+    first[j + 1] = first[j] + (int)nb;
+  }
+  AbsorbJob *dj = (AbsorbJob *)ctx->alloc(nj * sizeof(AbsorbJob));
+  int *dfirst = (int *)ctx->alloc((nj + 1) * sizeof(int));
+  ctx->upload(dj, jobs.data(), nj * sizeof(AbsorbJob));
+  ctx->upload(dfirst, first.data(), (nj + 1) * sizeof(int));
+  if (p->precision == HBP_PREC_FP64)
+    hipLaunchKernelGGL(k_absorb_batch<double2>, dim3((unsigned)first[nj]), dim3(256), 0,
+                       ctx->stream, dj, dfirst, nj);
+  else
+    hipLaunchKernelGGL(k_absorb_batch<float2>, dim3((unsigned)first[nj]), dim3(256), 0,
+                       ctx->stream, dj, dfirst, nj);
+  HBP_HIP(hipGetLastError());
+  ctx->release(dfirst, (nj + 1) * sizeof(int));
+  ctx->release(dj, nj * sizeof(AbsorbJob));
 }
 
 // Give up site s's current memory (a borrowed site is only forgotten).
@@ -1534,7 +1600,78 @@ static void apply_projectors(Ctx *ctx, Peps *p, const double2 *proj,
                              const std::vector<EdgeJob> &ej, const std::vector<int> &keep) {
   // new bond dims after all projectors
   std::vector<int> old_dim = p->bond_dim;
+  // Sites with every bond <= SMALL_D_MAX (and resident) take all their
+  // projectors and the normalisation in one launch of the small-site kernel
+  // over all of them (per-site launches of a few microseconds each were most
+  // of a truncation's time at these sizes); the others go bond by bond below.
+  std::vector<char> small(p->n_sites, 0);
+  {
+    std::vector<SmallProjJob> jobs;
+    std::vector<std::pair<int, size_t>> made;  // (site, bytes) of each new tensor
+    std::vector<AbsorbJob> absorb;             // lazily absorbed small sites, materialised first
+    std::vector<std::pair<void *, size_t>> tmp;
+    int64_t slab = 1;
+    for (int s = 0; s < p->n_sites; ++s) {
+      const int z = p->z(s);
+      if (z > 4) continue;
+      bool ok = true;
+      // (a projector never widens a bond: keep <= the bond's dim, so the
+      // kernel's ping-pong buffers hold every intermediate)
+      for (int k = 0; k < z; ++k) {
+        const int m = p->bonds[s][k];
+        ok = ok && old_dim[m] <= SMALL_D_MAX && keep[m] <= old_dim[m] && keep[m] >= 1;
+      }
+      if (!ok) continue;
+      SmallProjJob j{};
+      j.V = p->V[s];
+      if (p->pending[s]) {
+        const size_t vb = p->site_size(s) * p->elem_bytes();
+        void *t = ctx->alloc(vb);
+        absorb.push_back(AbsorbJob{p->V[s], (const double2 *)p->pend_O[s], t, p->site_size(s),
+                                   p->pend_args[s]});
+        tmp.emplace_back(t, vb);
+        j.V = t;
+      }
+      j.z = z;
+      size_t n_old = p->phys[s], n_new = p->phys[s];
+      for (int k = 0; k < z; ++k) {
+        const int m = p->bonds[s][k];
+        j.dims[k] = old_dim[m];
+        j.keep[k] = keep[m];
+        j.trans[k] = p->ea[m] == s ? 0 : 1;
+        j.P[k] = proj + (p->ea[m] == s ? ej[m].pa_off : ej[m].pb_off);
+        n_old *= (size_t)old_dim[m];
+        n_new *= (size_t)keep[m];
+      }
+      j.dims[z] = p->phys[s];
+      const size_t bytes = n_new * p->elem_bytes();
+      j.out = ctx->alloc(bytes);
+      slab = std::max<int64_t>(slab, (int64_t)n_old);
+      jobs.push_back(j);
+      made.emplace_back(s, bytes);
+      small[s] = 1;
+    }
+    absorb_batch(ctx, p, absorb);
+    if (!jobs.empty()) {
+      const int nj = (int)jobs.size();
+      const int grid = small_sweep_grid(0, nj);
+      const size_t scratch_bytes = (size_t)grid * 2 * (size_t)slab * p->elem_bytes();
+      void *scratch = ctx->alloc(scratch_bytes);
+      SmallProjJob *dj = (SmallProjJob *)ctx->alloc(nj * sizeof(SmallProjJob));
+      ctx->upload(dj, jobs.data(), nj * sizeof(SmallProjJob));
+      if (p->precision == HBP_PREC_FP64)
+        launch_project_small<double>(ctx->stream, dj, nj, scratch, slab, grid);
+      else
+        launch_project_small<float>(ctx->stream, dj, nj, scratch, slab, grid);
+      ctx->release(dj, nj * sizeof(SmallProjJob));
+      ctx->release(scratch, scratch_bytes);
+      for (auto &t : tmp) ctx->release(t.first, t.second);
+      for (int i = 0; i < nj; ++i)
+        set_resident(ctx, p, made[i].first, jobs[i].out, made[i].second);
+    }
+  }
   for (int s = 0; s < p->n_sites; ++s) {
+    if (small[s]) continue;
     const int z = p->z(s);
     int32_t sh[HBP_MAX_ORDER];
     for (int k = 0; k < z; ++k) sh[k] = old_dim[p->bonds[s][k]];
@@ -2367,6 +2504,11 @@ hbp_status hbp_peps_absorb_tpo(hbp_ctx *ctx, hbp_peps *p, const double *const *o
     op_off[s + 1] = op_off[s] + on;
   }
   double2 *dAll = nullptr;
+  // eager sites up to kAbsorbBatchMax elements are absorbed in one launch
+  constexpr size_t kAbsorbBatchMax = (size_t)1 << 24;
+  std::vector<AbsorbJob> batch;
+  std::vector<int> batch_site;
+  std::vector<size_t> batch_bytes;
   if (!lazy) {
     std::vector<double2> packed(std::max<size_t>(op_off[ns], 1));
     for (int s = 0; s < ns; ++s)
@@ -2420,6 +2562,14 @@ hbp_status hbp_peps_absorb_tpo(hbp_ctx *ctx, hbp_peps *p, const double *const *o
     const size_t bytes = n * p->elem_bytes();
     void *nv = ctx->alloc(bytes);
     const double2 *dO = dAll + op_off[s];
+    if (n <= kAbsorbBatchMax) {
+      // joins the one launch over the small sites below (the old tensor is
+      // released once that launch is enqueued)
+      batch.push_back(AbsorbJob{p->V[s], dO, nv, n, a});
+      batch_site.push_back(s);
+      batch_bytes.push_back(bytes);
+      continue;
+    }
     if (p->precision == HBP_PREC_FP64)
       hipLaunchKernelGGL(k_absorb_site<double2>, dim3(gridn(n)), dim3(256), 0, ctx->stream,
                          (const double2 *)p->V[s], dO, (double2 *)nv, n, a);
@@ -2430,6 +2580,16 @@ hbp_status hbp_peps_absorb_tpo(hbp_ctx *ctx, hbp_peps *p, const double *const *o
     release_site(ctx, p, s);
     p->V[s] = nv;
     p->V_bytes[s] = bytes;
+  }
+  if (!batch.empty()) {
+    absorb_batch(ctx, p, batch);
+    const int nj = (int)batch.size();
+    for (int j = 0; j < nj; ++j) {
+      const int s = batch_site[j];
+      release_site(ctx, p, s);
+      p->V[s] = batch[j].Vn;
+      p->V_bytes[s] = batch_bytes[j];
+    }
   }
   if (dAll) ctx->release(dAll, std::max<size_t>(op_off[ns], 1) * 16);
   std::vector<int> old = p->bond_dim;

@@ -65,6 +65,8 @@ def run_stream(jobs, rate, slots, seed, server="threaded"):
         nloops = max(1, sum(r["jobs"] for r in srv.rounds))
         st["batched_fraction"] = sum(r["batched"] for r in srv.rounds) / nloops
         st["graph_fraction"] = sum(r.get("graph", 0) for r in srv.rounds) / nloops
+        st["phase_seconds"] = {k: round(sum(r[k] for r in srv.rounds), 3)
+                               for k in ("absorb_s", "bp_s", "truncate_s", "measure_s")}
     return st, res
 
 

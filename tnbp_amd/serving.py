@@ -442,6 +442,7 @@ class BatchedServer:
                 t0 = time.perf_counter()
                 for r in active:
                     AbsorbTPO(ctx, r["job"].tpos[r["layer"]], r["peps"])
+                t1 = time.perf_counter()
                 groups = {}
                 for r in active:
                     groups.setdefault((r["job"].max_bp_iterations, r["job"].bp_tolerance), []).append(r)
@@ -452,17 +453,23 @@ class BatchedServer:
                         r["iters"].append(it)
                         nb += int(bat == HBP_BATCH_SMALL)
                         ng += int(bat == HBP_BATCH_GRAPH)
+                t2 = time.perf_counter()
                 for r in active:
                     Truncation(ctx, r["peps"], r["job"].chi, r["job"].sv_min)
                     r["layer"] += 1
-                self.rounds.append({"jobs": len(active), "batched": nb, "graph": ng,
-                                    "seconds": time.perf_counter() - t0})
+                t3 = time.perf_counter()
                 still = []
                 for r in active:
                     if r["layer"] < len(r["job"].tpos):
                         still.append(r)
                         continue
                     self._retire(ctx, r, len(active))
+                t4 = time.perf_counter()
+                # wall seconds of the round's phases (each call returns once its
+                # work is done on the device)
+                self.rounds.append({"jobs": len(active), "batched": nb, "graph": ng,
+                                    "seconds": t4 - t0, "absorb_s": t1 - t0, "bp_s": t2 - t1,
+                                    "truncate_s": t3 - t2, "measure_s": t4 - t3})
                 active = still
         except BaseException as e:  # fail every resident, queued and later job
             for r in active:
