@@ -196,7 +196,117 @@ __global__ __launch_bounds__(256) void k_project_small(const SmallProjJob *__res
   }
 }
 
+template <typename S>
+__global__ __launch_bounds__(256) void k_chain_small(const SmallChainSite *__restrict__ sites,
+                                                     const SmallChainJob *__restrict__ jobs,
+                                                     int njobs, const double2 *__restrict__ E,
+                                                     void *__restrict__ scratch,
+                                                     int64_t slab_elems, double2 *__restrict__ val,
+                                                     double *__restrict__ lg) {
+  using C = typename CT<S>::type;
+  __shared__ double2 Es[SMALL_D_MAX * SMALL_D_MAX];
+  __shared__ double2 X[SMALL_D_MAX * SMALL_D_MAX];
+  __shared__ double2 M[SMALL_D_MAX * SMALL_D_MAX];
+  __shared__ double2 red[256];
+  C *bufA = (C *)scratch + (int64_t)blockIdx.x * 2 * slab_elems;
+  C *bufB = bufA + slab_elems;
+  const int tid = threadIdx.x;
+  for (int w = blockIdx.x; w < njobs; w += gridDim.x) {
+    const SmallChainJob job = jobs[w];
+    double lsum = 0.0;
+    for (int i = 0; i < job.n; ++i) {
+      const SmallChainSite st = sites[job.first + i];
+      const int z = st.z;
+      const C *V = (const C *)st.V;
+      const C *cur = V;
+      C *nxt = bufA;
+      auto product = [&](int b, int D, const double2 *src, bool lds) {
+        int64_t L = 1, R = 1;
+        for (int d = 0; d < b; ++d) L *= st.dims[d];
+        for (int d = b + 1; d <= z; ++d) R *= st.dims[d];
+        if (!lds)
+          for (int e = tid; e < D * D; e += blockDim.x) Es[e] = src[e];
+        __syncthreads();
+        small_mode_product<S>(cur, nxt, L, D, D, R, lds ? src : Es);
+        __syncthreads();
+        cur = nxt;
+        nxt = nxt == bufA ? bufB : bufA;
+      };
+      for (int b = 0; b < z; ++b)
+        if (b != st.kin && b != st.kout) product(b, st.dims[b], E + st.in_off[b], false);
+      if (st.kin >= 0 && st.kout >= 0) product(st.kin, st.dims[st.kin], X, true);
+      if (st.op) product(z, st.dims[z], st.op, false);
+      const int t = st.kout >= 0 ? st.kout : (st.kin >= 0 ? st.kin : z);
+      int64_t L = 1, R = 1;
+      for (int d = 0; d < t; ++d) L *= st.dims[d];
+      for (int d = t + 1; d <= z; ++d) R *= st.dims[d];
+      const int D = st.dims[t], n = D * D;
+      small_message<S>(cur, V, M, L, D, R, red);  // ends with a barrier
+      if (st.kout >= 0) {
+        // X <- M / |M|_F, lg += log |M|_F
+        double s2 = 0.0;
+        for (int e = tid; e < n; e += blockDim.x) s2 += M[e].x * M[e].x + M[e].y * M[e].y;
+        red[tid].x = s2;
+        __syncthreads();
+        for (int h = 128; h > 0; h >>= 1) {
+          if (tid < h) red[tid].x += red[tid + h].x;
+          __syncthreads();
+        }
+        const double nrm = sqrt(red[0].x);
+        const double inv = nrm > 0.0 ? 1.0 / nrm : 1.0;
+        for (int e = tid; e < n; e += blockDim.x) X[e] = make_double2(M[e].x * inv, M[e].y * inv);
+        lsum += nrm > 0.0 ? log(nrm) : 0.0;
+        __syncthreads();
+      } else {
+        double re = 0.0, im = 0.0;
+        if (st.kin >= 0) {
+          for (int e = tid; e < n; e += blockDim.x) {
+            const double2 a = X[e], b = M[e];
+            re += a.x * b.x - a.y * b.y;
+            im += a.x * b.y + a.y * b.x;
+          }
+        } else {
+          for (int q = tid; q < D; q += blockDim.x) {
+            re += M[q + D * q].x;
+            im += M[q + D * q].y;
+          }
+        }
+        red[tid] = make_double2(re, im);
+        __syncthreads();
+        for (int h = 128; h > 0; h >>= 1) {
+          if (tid < h) {
+            red[tid].x += red[tid + h].x;
+            red[tid].y += red[tid + h].y;
+          }
+          __syncthreads();
+        }
+        if (tid == 0) {
+          val[job.slot] = red[0];
+          lg[job.slot] = lsum;
+        }
+        __syncthreads();
+      }
+    }
+  }
+}
+
 }  // namespace
+
+template <typename S>
+void launch_chain_small(hipStream_t st, const SmallChainSite *sites, const SmallChainJob *jobs,
+                        int njobs, const double2 *E, void *scratch, int64_t slab_elems,
+                        double2 *val, double *lg, int grid) {
+  if (njobs <= 0) return;
+  hipLaunchKernelGGL(k_chain_small<S>, dim3((unsigned)grid), dim3(256), 0, st, sites, jobs, njobs,
+                     E, scratch, slab_elems, val, lg);
+  HBP_HIP(hipGetLastError());
+}
+template void launch_chain_small<double>(hipStream_t, const SmallChainSite *, const SmallChainJob *,
+                                         int, const double2 *, void *, int64_t, double2 *,
+                                         double *, int);
+template void launch_chain_small<float>(hipStream_t, const SmallChainSite *, const SmallChainJob *,
+                                        int, const double2 *, void *, int64_t, double2 *,
+                                        double *, int);
 
 template <typename S>
 void launch_project_small(hipStream_t st, const SmallProjJob *jobs_dev, int njobs, void *scratch,

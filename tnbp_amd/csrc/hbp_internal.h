@@ -284,6 +284,28 @@ struct SmallProjJob {
 template <typename S>
 void launch_project_small(hipStream_t st, const SmallProjJob *jobs_dev, int njobs, void *scratch,
                           int64_t slab_elems, int grid);
+// Products of operators along paths (measure.h:14-94 site by site, the path's
+// transfer matrix carried from site to site), bonds <= SMALL_D_MAX, one
+// workgroup per path.  Site i of a path: T = V x_{b != kin, kout} E_in(b)
+// [x_kin X] [x_phys op]; M = <T, conj V> over all but the bond t = kout, else
+// kin, else the physical index; X <- M / |M|_F with lg += log |M|_F while
+// kout >= 0; at the last site val = sum X .* M (or the trace of M on a
+// one-site path).
+struct SmallChainSite {
+  const void *V;
+  const double2 *op;  // (d, d) or null
+  int64_t in_off[4];  // incoming messages of the bonds (offsets into E)
+  int dims[5];        // bond dims, then the physical dim at index z
+  int z, kin, kout;
+};
+struct SmallChainJob {
+  int first, n;  // its sites: sites[first .. first + n)
+  int slot;      // val[slot], lg[slot]
+};
+template <typename S>
+void launch_chain_small(hipStream_t st, const SmallChainSite *sites, const SmallChainJob *jobs,
+                        int njobs, const double2 *E, void *scratch, int64_t slab_elems,
+                        double2 *val, double *lg, int grid);
 // Grids the batched sweep is launched with: GRID_MULT[v] workgroups per CU
 // (variant v), at most one per work item; scratch for the largest.
 constexpr int SMALL_GRID_VARIANTS = 3;

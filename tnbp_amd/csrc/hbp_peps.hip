@@ -2079,14 +2079,66 @@ static void measure_paths_impl(Ctx *ctx, Peps *p, int n, const int *offsets, con
   double2 *M = (double2 *)ctx->alloc((size_t)maxD * maxD * 16);
   double2 *val = (double2 *)ctx->alloc((size_t)nslots * 16);
   double *lg = (double *)ctx->alloc((size_t)nslots * 8);
+  // Paths whose sites all have bonds <= SMALL_D_MAX (resident, degree <= 4)
+  // go into one launch of the small-bond chain kernel, one workgroup per
+  // path; the others run site by site (chain_value).
+  auto small_site = [&](int s) {
+    if (p->z(s) > 4 || p->pending[s] || p->phys[s] > SMALL_D_MAX) return false;
+    for (int b = 0; b < p->z(s); ++b)
+      if (p->bond_dim[p->bonds[s][b]] > SMALL_D_MAX) return false;
+    return true;
+  };
+  std::vector<SmallChainSite> csites;
+  std::vector<SmallChainJob> cjobs;
+  int64_t slab = 1;
+  auto add_small = [&](const std::vector<int> &path, const std::vector<int> &kin,
+                       const std::vector<int> &kout, const double2 *ops_dev, int slot) {
+    for (int s : path)
+      if (!small_site(s)) return false;
+    cjobs.push_back(SmallChainJob{(int)csites.size(), (int)path.size(), slot});
+    for (size_t i = 0; i < path.size(); ++i) {
+      const int s = path[i];
+      SmallChainSite c{};
+      c.V = p->V[s];
+      c.op = ops_dev ? ops_dev + i * d * d : nullptr;
+      c.z = p->z(s);
+      c.kin = kin[i];
+      c.kout = kout[i];
+      for (int b = 0; b < c.z; ++b) {
+        c.dims[b] = p->bond_dim[p->bonds[s][b]];
+        c.in_off[b] = p->E_off[p->in_msg(s, b)];
+      }
+      c.dims[c.z] = p->phys[s];
+      slab = std::max<int64_t>(slab, (int64_t)p->site_size(s));
+      csites.push_back(c);
+    }
+    return true;
+  };
   for (const auto &kv : den_slot) {
     const int t = (int)(std::find(paths.begin(), paths.end(), kv.first) - paths.begin());
-    chain_value<S>(ctx, p, kv.first, kins[t], kouts[t], nullptr, X, M, val + kv.second,
-                   lg + kv.second);
+    if (!add_small(kv.first, kins[t], kouts[t], nullptr, kv.second))
+      chain_value<S>(ctx, p, kv.first, kins[t], kouts[t], nullptr, X, M, val + kv.second,
+                     lg + kv.second);
   }
-  for (int t = 0; t < n; ++t)
-    chain_value<S>(ctx, p, paths[t], kins[t], kouts[t],
-                   dops + (size_t)(offsets[t] - offsets[0]) * d * d, X, M, val + t, lg + t);
+  for (int t = 0; t < n; ++t) {
+    const double2 *ot = dops + (size_t)(offsets[t] - offsets[0]) * d * d;
+    if (!add_small(paths[t], kins[t], kouts[t], ot, t))
+      chain_value<S>(ctx, p, paths[t], kins[t], kouts[t], ot, X, M, val + t, lg + t);
+  }
+  if (!cjobs.empty()) {
+    const int nj = (int)cjobs.size();
+    const int grid = small_sweep_grid(0, nj);
+    const size_t scratch_bytes = (size_t)grid * 2 * (size_t)slab * p->elem_bytes();
+    void *scratch = ctx->alloc(scratch_bytes);
+    SmallChainSite *dcs = (SmallChainSite *)ctx->alloc(csites.size() * sizeof(SmallChainSite));
+    SmallChainJob *dcj = (SmallChainJob *)ctx->alloc(nj * sizeof(SmallChainJob));
+    ctx->upload(dcs, csites.data(), csites.size() * sizeof(SmallChainSite));
+    ctx->upload(dcj, cjobs.data(), nj * sizeof(SmallChainJob));
+    launch_chain_small<S>(ctx->stream, dcs, dcj, nj, p->E, scratch, slab, val, lg, grid);
+    ctx->release(dcj, nj * sizeof(SmallChainJob));
+    ctx->release(dcs, csites.size() * sizeof(SmallChainSite));
+    ctx->release(scratch, scratch_bytes);
+  }
   int *dden = (int *)ctx->alloc(n * sizeof(int));
   double2 *dout = (double2 *)ctx->alloc((size_t)n * 16);
   ctx->upload(dden, den.data(), n * sizeof(int));

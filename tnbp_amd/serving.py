@@ -464,6 +464,11 @@ class BatchedServer:
                         still.append(r)
                         continue
                     self._retire(ctx, r, len(active))
+                if not still:
+                    # nothing resident: give the context's kept blocks back to
+                    # the device (while jobs stay resident the blocks a retired
+                    # job freed serve the others' allocations)
+                    ctx.trim()
                 t4 = time.perf_counter()
                 # wall seconds of the round's phases (each call returns once its
                 # work is done on the device)
@@ -502,7 +507,6 @@ class BatchedServer:
             r["fut"].set_exception(e)
             return
         r["peps"].close()
-        ctx.trim()
         done = time.perf_counter()
         self._finish_fp(r["fp"])
         res = {"value": total, "values": vals, "bp_iterations": r["iters"], "n_values": len(vals)}
