@@ -367,6 +367,18 @@ hbp_status hbp_peps_bp_iterate(hbp_ctx *ctx, hbp_peps *p, int n, double tol, int
  * bond_dims / trunc_errs (may be NULL) receive one value per edge. */
 hbp_status hbp_peps_truncate(hbp_ctx *ctx, hbp_peps *p, int chi_max, double sv_min,
                              int32_t *bond_dims, double *trunc_errs);
+/* hbp_peps_truncate for n PEPS of one context at once (no reference
+ * counterpart): every message's SVD, the roots, T = Ra^T Rb, its SVD and the
+ * projectors of all n PEPS in one launch each (the SVDs in two when some PEPS
+ * has bonds above 64 and others do not), the projectors of every site
+ * with bonds <= 16 in one launch per precision, the other sites site by site,
+ * all new messages in one launch.  PEPS i keeps chi_max[i] values above
+ * sv_min[i]; its bond dims and truncation errors are written at
+ * bond_dims / trunc_errs + (n_edges of PEPS 0 .. i-1) (each may be NULL).
+ * Each PEPS's result is bitwise the one hbp_peps_truncate gives it alone. */
+hbp_status hbp_peps_truncate_batched(hbp_ctx *ctx, int n, hbp_peps *const *ps,
+                                     const int32_t *chi_max, const double *sv_min,
+                                     int32_t *bond_dims, double *trunc_errs);
 /* tnbp::Measure for single sites (measure.h:14-94): ops are n complex 2x2
  * (d x d) operators O(in, out) column-major; out receives n complex values. */
 hbp_status hbp_peps_measure_sites(hbp_ctx *ctx, hbp_peps *p, int n, const int32_t *sites,

@@ -236,3 +236,38 @@ def test_graph_members_equal_bp_loop_bitwise(ctx):
             np.testing.assert_array_equal(a, b)
     for p in mine + refs + small:
         p.close()
+
+
+def test_truncate_batched_equals_each_alone(ctx):
+    """hbp_peps_truncate_batched over PEPS of both precisions and of small
+    (<= 16), medium (32) and large (64, 96: SVDs outside LDS) bonds, each with
+    its own chi and sv_min: bond dims, truncation errors, site tensors and
+    messages bitwise those of Truncation on each PEPS alone."""
+    from tnbp_amd.kernel_probe import build_state
+    from tnbp_amd.peps import Truncation, TruncationBatched, bp_loop
+    announce("batched truncation vs each PEPS alone")
+
+    def states():
+        out = [p for _, p in _prepared(ctx, 0, seed=23)] + [p for _, p in _prepared(ctx, 1, seed=23)]
+        out.append(_kicked_big(ctx))
+        for D, prec, seed in ((64, 0, 5), (64, 1, 6), (96, 0, 7)):
+            q = build_state(ctx, 3, D, prec, seed=seed)
+            bp_loop(ctx, q, 2, 1e-30)
+            out.append(q)
+        return out
+
+    chis = [4, 8, 3, 6, 4, 8, 3, 6, 16, 32, 24, 48]
+    svs = [1e-10, 1e-12, 1e-8, 0.0] * 3
+    a, b = states(), states()
+    assert len(a) == len(chis)
+    got = TruncationBatched(ctx, a, chis, svs)
+    for p, q, c, e, (dims, errs) in zip(a, b, chis, svs, got):
+        d0, e0 = Truncation(ctx, q, c, e)
+        assert dims == d0 and errs == e0
+        for s in range(len(p.sites)):
+            np.testing.assert_array_equal(p.get_site(s), q.get_site(s))
+        for x, y in zip(_state(p), _state(q)):
+            np.testing.assert_array_equal(x, y)
+    announce(f"  {len(a)} PEPS, bond dims after: {[max(g[0]) for g in got]}")
+    for p in a + b:
+        p.close()

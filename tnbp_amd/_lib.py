@@ -99,6 +99,7 @@ SIGNATURES = {
     "hbp_peps_bp_iterate": [_vp, _vp, C.c_int, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int), _dp,
                             C.POINTER(C.c_int)],
     "hbp_peps_truncate": [_vp, _vp, C.c_int, C.c_double, _i32p, _dp],
+    "hbp_peps_truncate_batched": [_vp, C.c_int, C.POINTER(_vp), _i32p, _dp, _i32p, _dp],
     "hbp_peps_measure_sites": [_vp, _vp, C.c_int, _i32p, _dp, _dp],
     "hbp_peps_measure_edges": [_vp, _vp, C.c_int, _i32p, _dp, _dp],
     "hbp_peps_measure_paths": [_vp, _vp, C.c_int, _i32p, _i32p, _dp, _dp],

@@ -391,17 +391,17 @@ __global__ __launch_bounds__(256) void k_expand_messages(const double2 *__restri
 // Copies of n-element blocks to two places each (after truncation both
 // directed messages of an edge are diag(Lambda)): block b does jobs[b].
 struct CopyJob {
-  int64_t src, dst0, dst1;
+  int64_t src;         // offset in src
+  double2 *dst0, *dst1;
   int n;
 };
 __global__ __launch_bounds__(256) void k_copy_twice(const double2 *__restrict__ src,
-                                                    double2 *__restrict__ dst,
                                                     const CopyJob *__restrict__ jobs) {
   const CopyJob j = jobs[blockIdx.x];
   for (int e = threadIdx.x; e < j.n; e += blockDim.x) {
     const double2 v = src[j.src + e];
-    dst[j.dst0 + e] = v;
-    dst[j.dst1 + e] = v;
+    j.dst0[e] = v;
+    j.dst1[e] = v;
   }
 }
 
@@ -410,6 +410,7 @@ struct RootJob {
   int64_t u_off, s_off, vd_off;  // SVD of the message
   int64_t r_off, si_off;         // outputs R, S (D x D)
   int D;
+  double sv_min;                 // of the PEPS the message belongs to
 };
 
 // SquareRootAndInverse (root.h:65-148): R = U diag(d) Vd, S = Vd^H diag(f) U^H
@@ -418,9 +419,10 @@ __global__ __launch_bounds__(256) void k_roots(const double2 *__restrict__ U,
                                                 const double *__restrict__ sv,
                                                 const double2 *__restrict__ Vd,
                                                 double2 *__restrict__ out,
-                                                const RootJob *__restrict__ jobs, double sv_min) {
+                                                const RootJob *__restrict__ jobs) {
   const RootJob j = jobs[blockIdx.x];
   const int D = j.D;
+  const double sv_min = j.sv_min;
   __shared__ double dd[512], ff[512];
   for (int c = threadIdx.x; c < D; c += blockDim.x) {
     const double e = sv[j.s_off + c];
@@ -455,6 +457,8 @@ struct EdgeJob {
   int64_t u_off, s_off, vd_off;            // SVD of T
   int64_t pa_off, pb_off, e_off;           // projectors and new messages
   int D;
+  int chi;                                 // chi_max and s_min of the PEPS
+  double s_min;
 };
 
 // T = Ra^T Rb, normalised (truncation.h:110-121)
@@ -493,11 +497,12 @@ __global__ __launch_bounds__(256) void k_projectors(const double2 *__restrict__ 
                                                      const double *__restrict__ sv,
                                                      const double2 *__restrict__ Vd,
                                                      double2 *__restrict__ proj,
-                                                     const EdgeJob *__restrict__ jobs, int chi,
-                                                     double s_min, int *__restrict__ keep_out,
+                                                     const EdgeJob *__restrict__ jobs,
+                                                     int *__restrict__ keep_out,
                                                      double *__restrict__ err_out) {
   const EdgeJob j = jobs[blockIdx.x];
-  const int D = j.D;
+  const int D = j.D, chi = j.chi;
+  const double s_min = j.s_min;
   __shared__ int keep_sh;
   __shared__ double sq[512];
   if (threadIdx.x == 0) {
@@ -1595,93 +1600,106 @@ static void sweep(Ctx *ctx, Peps *p, double2 *newE, bool with_residual, double *
 }
 
 // ----------------------------------------------------------- truncation
-template <typename S>
-static void apply_projectors(Ctx *ctx, Peps *p, const double2 *proj,
-                             const std::vector<EdgeJob> &ej, const std::vector<int> &keep) {
-  // new bond dims after all projectors
-  std::vector<int> old_dim = p->bond_dim;
-  // Sites with every bond <= SMALL_D_MAX (and resident) take all their
-  // projectors and the normalisation in one launch of the small-site kernel
-  // over all of them (per-site launches of a few microseconds each were most
-  // of a truncation's time at these sizes); the others go bond by bond below.
-  std::vector<char> small(p->n_sites, 0);
-  {
-    std::vector<SmallProjJob> jobs;
-    std::vector<std::pair<int, size_t>> made;  // (site, bytes) of each new tensor
-    std::vector<AbsorbJob> absorb;             // lazily absorbed small sites, materialised first
-    std::vector<std::pair<void *, size_t>> tmp;
-    int64_t slab = 1;
-    for (int s = 0; s < p->n_sites; ++s) {
-      const int z = p->z(s);
-      if (z > 4) continue;
-      bool ok = true;
-      // (a projector never widens a bond: keep <= the bond's dim, so the
-      // kernel's ping-pong buffers hold every intermediate)
-      for (int k = 0; k < z; ++k) {
-        const int m = p->bonds[s][k];
-        ok = ok && old_dim[m] <= SMALL_D_MAX && keep[m] <= old_dim[m] && keep[m] >= 1;
-      }
-      if (!ok) continue;
-      SmallProjJob j{};
-      j.V = p->V[s];
-      if (p->pending[s]) {
-        const size_t vb = p->site_size(s) * p->elem_bytes();
-        void *t = ctx->alloc(vb);
-        absorb.push_back(AbsorbJob{p->V[s], (const double2 *)p->pend_O[s], t, p->site_size(s),
+// Sites of a truncation with every bond <= SMALL_D_MAX and degree <= 4, of
+// any number of PEPS of one precision: all their projectors and
+// normalisations run in one launch of the small-site kernel (per-site
+// launches of a few microseconds each were most of a truncation's time at
+// these sizes); lazily absorbed ones are materialised first, in one launch.
+struct SmallProjBatch {
+  std::vector<SmallProjJob> jobs;
+  std::vector<std::pair<Peps *, int>> site;  // (PEPS, site) of each job
+  std::vector<size_t> bytes;                 // of each new tensor
+  std::vector<AbsorbJob> absorb;
+  std::vector<std::pair<void *, size_t>> tmp;
+  int64_t slab = 1;
+  Peps *any = nullptr;
+};
+
+// Adds the small sites of p to b and marks them in `small`.
+static void collect_small_projections(Ctx *ctx, Peps *p, const double2 *proj,
+                                      const std::vector<EdgeJob> &ej,
+                                      const std::vector<int> &keep, std::vector<char> &small,
+                                      SmallProjBatch &b) {
+  small.assign(p->n_sites, 0);
+  for (int s = 0; s < p->n_sites; ++s) {
+    const int z = p->z(s);
+    if (z > 4) continue;
+    // (a projector never widens a bond: keep <= the bond's dim, so the
+    // kernel's ping-pong buffers hold every intermediate)
+    bool ok = true;
+    for (int k = 0; k < z; ++k) {
+      const int m = p->bonds[s][k];
+      ok = ok && p->bond_dim[m] <= SMALL_D_MAX && keep[m] <= p->bond_dim[m] && keep[m] >= 1;
+    }
+    if (!ok) continue;
+    SmallProjJob j{};
+    j.V = p->V[s];
+    if (p->pending[s]) {
+      const size_t vb = p->site_size(s) * p->elem_bytes();
+      void *t = ctx->alloc(vb);
+      b.absorb.push_back(AbsorbJob{p->V[s], (const double2 *)p->pend_O[s], t, p->site_size(s),
                                    p->pend_args[s]});
-        tmp.emplace_back(t, vb);
-        j.V = t;
-      }
-      j.z = z;
-      size_t n_old = p->phys[s], n_new = p->phys[s];
-      for (int k = 0; k < z; ++k) {
-        const int m = p->bonds[s][k];
-        j.dims[k] = old_dim[m];
-        j.keep[k] = keep[m];
-        j.trans[k] = p->ea[m] == s ? 0 : 1;
-        j.P[k] = proj + (p->ea[m] == s ? ej[m].pa_off : ej[m].pb_off);
-        n_old *= (size_t)old_dim[m];
-        n_new *= (size_t)keep[m];
-      }
-      j.dims[z] = p->phys[s];
-      const size_t bytes = n_new * p->elem_bytes();
-      j.out = ctx->alloc(bytes);
-      slab = std::max<int64_t>(slab, (int64_t)n_old);
-      jobs.push_back(j);
-      made.emplace_back(s, bytes);
-      small[s] = 1;
+      b.tmp.emplace_back(t, vb);
+      j.V = t;
     }
-    absorb_batch(ctx, p, absorb);
-    if (!jobs.empty()) {
-      const int nj = (int)jobs.size();
-      const int grid = small_sweep_grid(0, nj);
-      const size_t scratch_bytes = (size_t)grid * 2 * (size_t)slab * p->elem_bytes();
-      void *scratch = ctx->alloc(scratch_bytes);
-      SmallProjJob *dj = (SmallProjJob *)ctx->alloc(nj * sizeof(SmallProjJob));
-      ctx->upload(dj, jobs.data(), nj * sizeof(SmallProjJob));
-      if (p->precision == HBP_PREC_FP64)
-        launch_project_small<double>(ctx->stream, dj, nj, scratch, slab, grid);
-      else
-        launch_project_small<float>(ctx->stream, dj, nj, scratch, slab, grid);
-      ctx->release(dj, nj * sizeof(SmallProjJob));
-      ctx->release(scratch, scratch_bytes);
-      for (auto &t : tmp) ctx->release(t.first, t.second);
-      for (int i = 0; i < nj; ++i)
-        set_resident(ctx, p, made[i].first, jobs[i].out, made[i].second);
+    j.z = z;
+    size_t n_old = p->phys[s], n_new = p->phys[s];
+    for (int k = 0; k < z; ++k) {
+      const int m = p->bonds[s][k];
+      j.dims[k] = p->bond_dim[m];
+      j.keep[k] = keep[m];
+      j.trans[k] = p->ea[m] == s ? 0 : 1;
+      j.P[k] = proj + (p->ea[m] == s ? ej[m].pa_off : ej[m].pb_off);
+      n_old *= (size_t)p->bond_dim[m];
+      n_new *= (size_t)keep[m];
     }
+    j.dims[z] = p->phys[s];
+    const size_t bytes = n_new * p->elem_bytes();
+    j.out = ctx->alloc(bytes);
+    b.slab = std::max<int64_t>(b.slab, (int64_t)n_old);
+    b.jobs.push_back(j);
+    b.site.emplace_back(p, s);
+    b.bytes.push_back(bytes);
+    b.any = p;
+    small[s] = 1;
   }
+}
+
+static void run_small_projections(Ctx *ctx, SmallProjBatch &b) {
+  if (b.jobs.empty()) return;
+  absorb_batch(ctx, b.any, b.absorb);
+  const int nj = (int)b.jobs.size();
+  const int grid = small_sweep_grid(0, nj);
+  const size_t scratch_bytes = (size_t)grid * 2 * (size_t)b.slab * b.any->elem_bytes();
+  void *scratch = ctx->alloc(scratch_bytes);
+  SmallProjJob *dj = (SmallProjJob *)ctx->alloc(nj * sizeof(SmallProjJob));
+  ctx->upload(dj, b.jobs.data(), nj * sizeof(SmallProjJob));
+  if (b.any->precision == HBP_PREC_FP64)
+    launch_project_small<double>(ctx->stream, dj, nj, scratch, b.slab, grid);
+  else
+    launch_project_small<float>(ctx->stream, dj, nj, scratch, b.slab, grid);
+  ctx->release(dj, nj * sizeof(SmallProjJob));
+  ctx->release(scratch, scratch_bytes);
+  for (auto &t : b.tmp) ctx->release(t.first, t.second);
+  for (int i = 0; i < nj; ++i)
+    set_resident(ctx, b.site[i].first, b.site[i].second, b.jobs[i].out, b.bytes[i]);
+}
+
+// The other sites: projectors bond by bond (the last one writes the new site
+// tensor), then the normalisation (truncation.h:156-217).
+template <typename S>
+static void apply_projectors_large(Ctx *ctx, Peps *p, const double2 *proj,
+                                   const std::vector<EdgeJob> &ej, const std::vector<int> &keep,
+                                   const std::vector<char> &small,
+                                   const std::vector<int> &old_dim) {
   for (int s = 0; s < p->n_sites; ++s) {
     if (small[s]) continue;
     const int z = p->z(s);
     int32_t sh[HBP_MAX_ORDER];
     for (int k = 0; k < z; ++k) sh[k] = old_dim[p->bonds[s][k]];
     sh[z] = p->phys[s];
-    size_t maxn = 1;
-    {
-      size_t n = p->phys[s];
-      for (int k = 0; k < z; ++k) n *= (size_t)sh[k];
-      maxn = n;
-    }
+    size_t maxn = p->phys[s];
+    for (int k = 0; k < z; ++k) maxn *= (size_t)sh[k];
     ensure_work(ctx, p, std::max(p->buf_bytes, maxn * p->elem_bytes()));
     size_t n = p->phys[s];
     for (int k = 0; k < z; ++k) n *= (size_t)keep[p->bonds[s][k]];
@@ -1692,7 +1710,6 @@ static void apply_projectors(Ctx *ctx, Peps *p, const double2 *proj,
     for (int k = 0; k < z; ++k) {
       const int m = p->bonds[s][k];
       const int kp = keep[m];
-      // the last projector writes the new site tensor itself
       void *dst = k + 1 == z ? nv : p->buf[which];
       if (p->ea[m] == s)
         mode_prod<S>(ctx, cur, proj + ej[m].pa_off, dst, sh, z + 1, k, kp, 0);
@@ -1703,7 +1720,6 @@ static void apply_projectors(Ctx *ctx, Peps *p, const double2 *proj,
       which ^= 1;
     }
     if (z == 0) HBP_HIP(hipMemcpyAsync(nv, cur, bytes, hipMemcpyDeviceToDevice, ctx->stream));
-    // normalise the site (truncation.h:185, :216)
     double *dv = (double *)ctx->alloc(sizeof(double));
     device_norm2(ctx, nv, n, p->site_dtype(), dv);
     launch_scale(ctx->stream, nv, n, p->site_dtype(), 1.0, 0.0, dv);
@@ -1712,125 +1728,243 @@ static void apply_projectors(Ctx *ctx, Peps *p, const double2 *proj,
   }
 }
 
+// One PEPS of a (batched) truncation and where its part of the batch lies.
+struct TruncMember {
+  Peps *p = nullptr;
+  int chi = 1;
+  double sv_min = 0.0;
+  int32_t *dims_out = nullptr;
+  double *errs_out = nullptr;
+  int64_t e_base = 0;  // its messages' offset in the batch's message buffer
+  int64_t e_len = 0;   // and their elements
+  int m_base = 0;      // its first message in the batch
+  int e_first = 0;     // its first edge in the batch
+  std::vector<EdgeJob> ej;
+};
+
+// Truncation (truncation.h:17-222) of every PEPS of `mem` at once: one
+// batched SVD of all their messages, one root launch, one T = Ra^T Rb launch,
+// one batched SVD of all T, one projector launch, one read of the kept
+// dimensions, the small sites' projectors in one launch per precision, the
+// others site by site, and all new diag(Lambda) messages in one launch.
+// Every matrix, edge and site is computed by the same kernel code as for a
+// PEPS on its own, so each PEPS's result does not depend on the batch.
+static void truncate_many(Ctx *ctx, std::vector<TruncMember> &mem) {
+  // The batched SVD keeps a launch's matrices in LDS when its largest fits
+  // (bonds <= 64): members whose own bonds fit come first and get that form,
+  // the others a launch of their own, so that every PEPS's matrices take the
+  // form they take when it is truncated alone.
+  auto own_max = [](const Peps *p) {
+    int d = 1;
+    for (int m = 0; m < p->n_edges; ++m) d = std::max(d, p->bond_dim[m]);
+    return d;
+  };
+  auto lds_form = [&](const Peps *p) {
+    const int64_t d = own_max(p);
+    return 2 * d * d <= SVD_LDS_ELEMS;
+  };
+  std::stable_partition(mem.begin(), mem.end(), [&](const TruncMember &t) { return lds_form(t.p); });
+  int nm_lds = 0, ne_lds = 0, maxD_lds = 1, maxD_glob = 1;
+  for (auto &t : mem) {
+    if (lds_form(t.p)) {
+      nm_lds += 2 * t.p->n_edges;
+      ne_lds += t.p->n_edges;
+      maxD_lds = std::max(maxD_lds, own_max(t.p));
+    } else {
+      maxD_glob = std::max(maxD_glob, own_max(t.p));
+    }
+  }
+  int64_t tot = 0;
+  int nm_all = 0, ne_all = 0, maxD = 1;
+  for (auto &t : mem) {
+    Peps *p = t.p;
+    t.e_base = tot;
+    t.e_len = 0;
+    for (int m = 0; m < 2 * p->n_edges; ++m) t.e_len += (int64_t)p->msg_dim(m) * p->msg_dim(m);
+    tot += t.e_len;
+    t.m_base = nm_all;
+    t.e_first = ne_all;
+    nm_all += 2 * p->n_edges;
+    ne_all += p->n_edges;
+    for (int m = 0; m < p->n_edges; ++m) maxD = std::max(maxD, p->bond_dim[m]);
+  }
+  HBP_REQUIRE(maxD <= 512, HBP_ERR_UNSUPPORTED, "truncation: bond > 512");
+  const int64_t totc = std::max<int64_t>(tot, 1);
+  // the messages of every member in one buffer (a lone PEPS's own)
+  double2 *Ecat = nullptr;
+  const double2 *Eall = mem.size() == 1 ? mem[0].p->E : nullptr;
+  if (!Eall) {
+    Ecat = (double2 *)ctx->alloc(totc * 16);
+    for (auto &t : mem)
+      if (t.e_len > 0)
+        HBP_HIP(hipMemcpyAsync(Ecat + t.e_base, t.p->E, (size_t)t.e_len * 16,
+                               hipMemcpyDeviceToDevice, ctx->stream));
+    Eall = Ecat;
+  }
+  double2 *U = (double2 *)ctx->alloc(totc * 16);
+  double2 *Vd = (double2 *)ctx->alloc(totc * 16);
+  double2 *W = (double2 *)ctx->alloc(2 * totc * 16);
+  double *sv = (double *)ctx->alloc((size_t)std::max(nm_all, 1) * maxD * 8);
+  double2 *roots = (double2 *)ctx->alloc(2 * totc * 16);
+  // 1.-4. job tables: SVD of every message, its roots, the edges (T, its
+  // SVD, projectors, new messages); offsets into the shared buffers
+  std::vector<SvdJob> sj, tj;
+  std::vector<RootJob> rj;
+  std::vector<EdgeJob> ej_all;
+  int64_t tt = 0, poff = 0;
+  for (auto &t : mem) {
+    Peps *p = t.p;
+    const int ne = p->n_edges;
+    for (int m = 0; m < 2 * ne; ++m) {
+      const int D = p->msg_dim(m);
+      const int64_t o = t.e_base + p->E_off[m];
+      const int64_t so = (int64_t)(t.m_base + m) * maxD;
+      sj.push_back(SvdJob{o, o, so, o, 2 * o, D, D});
+      rj.push_back(RootJob{o, so, o, 2 * o, 2 * o + (int64_t)D * D, D, t.sv_min});
+    }
+    t.ej.assign(ne, EdgeJob{});
+    for (int m = 0; m < ne; ++m) {
+      const int D = p->bond_dim[m];
+      EdgeJob j{};
+      j.D = D;
+      j.chi = t.chi;
+      j.s_min = t.sv_min;
+      j.ra_off = rj[t.m_base + m].r_off;       // M_{a->b} = E[m]
+      j.sa_off = rj[t.m_base + m].si_off;
+      j.rb_off = rj[t.m_base + m + ne].r_off;  // M_{b->a} = E[m + ne]
+      j.sb_off = rj[t.m_base + m + ne].si_off;
+      j.t_off = tt;
+      j.u_off = tt;
+      j.vd_off = tt;
+      j.s_off = (int64_t)(t.e_first + m) * maxD;
+      j.pa_off = poff;
+      j.pb_off = poff + (int64_t)D * D;
+      j.e_off = poff + 2 * (int64_t)D * D;
+      poff += 3 * (int64_t)D * D;
+      tt += (int64_t)D * D;
+      t.ej[m] = j;
+      ej_all.push_back(j);
+      tj.push_back(SvdJob{j.t_off, j.u_off, j.s_off, j.vd_off, 2 * j.t_off, D, D});
+    }
+  }
+  // one upload: [message SVDs | roots | edges | T SVDs]
+  const size_t b_sj = sj.size() * sizeof(SvdJob), b_rj = rj.size() * sizeof(RootJob),
+               b_ej = ej_all.size() * sizeof(EdgeJob), b_tj = tj.size() * sizeof(SvdJob);
+  const size_t tab_bytes = std::max<size_t>(b_sj + b_rj + b_ej + b_tj, 16);
+  std::vector<char> tab(tab_bytes);
+  std::memcpy(tab.data(), sj.data(), b_sj);
+  std::memcpy(tab.data() + b_sj, rj.data(), b_rj);
+  std::memcpy(tab.data() + b_sj + b_rj, ej_all.data(), b_ej);
+  std::memcpy(tab.data() + b_sj + b_rj + b_ej, tj.data(), b_tj);
+  char *dtab = (char *)ctx->alloc(tab_bytes);
+  ctx->upload(dtab, tab.data(), tab_bytes);
+  const SvdJob *dsj = (const SvdJob *)dtab;
+  const RootJob *drj = (const RootJob *)(dtab + b_sj);
+  const EdgeJob *dej = (const EdgeJob *)(dtab + b_sj + b_rj);
+  const SvdJob *dtj = (const SvdJob *)(dtab + b_sj + b_rj + b_ej);
+  double2 *T = (double2 *)ctx->alloc(std::max<int64_t>(tt, 1) * 16);
+  double2 *proj = (double2 *)ctx->alloc(std::max<int64_t>(poff, 1) * 16);
+  int *dkeep = (int *)ctx->alloc(std::max(ne_all, 1) * sizeof(int));
+  double *derr = (double *)ctx->alloc(std::max(ne_all, 1) * sizeof(double));
+  std::vector<int> keep_all(ne_all);
+  std::vector<double> err_all(ne_all);
+  auto svd_both = [&](const double2 *A, const SvdJob *jobs, int n_all, int n_lds) {
+    if (n_lds > 0) launch_batched_svd(ctx->stream, A, U, sv, Vd, W, jobs, n_lds, maxD_lds, maxD_lds);
+    if (n_all > n_lds)
+      launch_batched_svd(ctx->stream, A, U, sv, Vd, W, jobs + n_lds, n_all - n_lds, maxD_glob,
+                         maxD_glob);
+  };
+  if (nm_all > 0) {
+    svd_both(Eall, dsj, nm_all, nm_lds);
+    hipLaunchKernelGGL(k_roots, dim3(nm_all), dim3(256), 0, ctx->stream, U, sv, Vd, roots, drj);
+    HBP_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_rr, dim3(ne_all), dim3(256), 0, ctx->stream, roots, T, dej);
+    HBP_HIP(hipGetLastError());
+    svd_both(T, dtj, ne_all, ne_lds);
+    // 5. selection and projectors
+    hipLaunchKernelGGL(k_projectors, dim3(ne_all), dim3(256), 0, ctx->stream, roots, U, sv, Vd,
+                       proj, dej, dkeep, derr);
+    HBP_HIP(hipGetLastError());
+    HBP_HIP(hipMemcpyAsync(keep_all.data(), dkeep, ne_all * sizeof(int), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    HBP_HIP(hipMemcpyAsync(err_all.data(), derr, ne_all * sizeof(double), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    ctx->sync();
+  }
+  // 6. projectors into the sites: the small sites of all members in one
+  // launch per precision, the others site by site
+  SmallProjBatch sb[2];
+  std::vector<std::vector<char>> small(mem.size());
+  std::vector<std::vector<int>> keeps(mem.size()), old_dims(mem.size());
+  for (size_t i = 0; i < mem.size(); ++i) {
+    TruncMember &t = mem[i];
+    keeps[i].assign(keep_all.begin() + t.e_first, keep_all.begin() + t.e_first + t.p->n_edges);
+    old_dims[i] = t.p->bond_dim;
+    collect_small_projections(ctx, t.p, proj, t.ej, keeps[i], small[i],
+                              sb[t.p->precision == HBP_PREC_MIXED ? 1 : 0]);
+  }
+  run_small_projections(ctx, sb[0]);
+  run_small_projections(ctx, sb[1]);
+  for (size_t i = 0; i < mem.size(); ++i) {
+    Peps *p = mem[i].p;
+    if (p->precision == HBP_PREC_FP64)
+      apply_projectors_large<double>(ctx, p, proj, mem[i].ej, keeps[i], small[i], old_dims[i]);
+    else
+      apply_projectors_large<float>(ctx, p, proj, mem[i].ej, keeps[i], small[i], old_dims[i]);
+  }
+  // 7. messages = diag(Lambda) in both directions, every member in one launch
+  std::vector<CopyJob> cj;
+  for (size_t i = 0; i < mem.size(); ++i) {
+    TruncMember &t = mem[i];
+    Peps *p = t.p;
+    const int ne = p->n_edges;
+    for (int m = 0; m < ne; ++m) p->bond_dim[m] = keeps[i][m];
+    double2 *nE;
+    size_t nbytes;
+    std::vector<int64_t> noff;
+    layout_messages(ctx, p, &nE, &nbytes, &noff);
+    for (int m = 0; m < ne; ++m)
+      cj.push_back(CopyJob{t.ej[m].e_off, nE + noff[m], nE + noff[m + ne], keeps[i][m] * keeps[i][m]});
+    ctx->release(p->E, p->E_bytes);
+    p->E = nE;
+    p->E_bytes = nbytes;
+    p->E_off = noff;
+    p->E_diag.assign(2 * ne, 1);
+    if (t.dims_out)
+      for (int m = 0; m < ne; ++m) t.dims_out[m] = keeps[i][m];
+    if (t.errs_out)
+      for (int m = 0; m < ne; ++m) t.errs_out[m] = err_all[t.e_first + m];
+  }
+  if (!cj.empty()) {
+    CopyJob *dcj = (CopyJob *)ctx->alloc(cj.size() * sizeof(CopyJob));
+    ctx->upload(dcj, cj.data(), cj.size() * sizeof(CopyJob));
+    hipLaunchKernelGGL(k_copy_twice, dim3((unsigned)cj.size()), dim3(256), 0, ctx->stream, proj,
+                       dcj);
+    HBP_HIP(hipGetLastError());
+    ctx->release(dcj, cj.size() * sizeof(CopyJob));
+  }
+  ctx->release(derr, std::max(ne_all, 1) * sizeof(double));
+  ctx->release(dkeep, std::max(ne_all, 1) * sizeof(int));
+  ctx->release(proj, std::max<int64_t>(poff, 1) * 16);
+  ctx->release(T, std::max<int64_t>(tt, 1) * 16);
+  ctx->release(dtab, tab_bytes);
+  ctx->release(roots, 2 * totc * 16);
+  ctx->release(sv, (size_t)std::max(nm_all, 1) * maxD * 8);
+  ctx->release(W, 2 * totc * 16);
+  ctx->release(Vd, totc * 16);
+  ctx->release(U, totc * 16);
+  if (Ecat) ctx->release(Ecat, totc * 16);
+}
+
 static void truncate(Ctx *ctx, Peps *p, int chi, double sv_min, int32_t *dims_out,
                      double *errs_out) {
-  const int ne = p->n_edges, nm = 2 * ne;
-  // 1. SVD of every message
-  int64_t tot = 0;
-  for (int m = 0; m < nm; ++m) tot += (int64_t)p->msg_dim(m) * p->msg_dim(m);
-  int maxD = 1;
-  for (int m = 0; m < ne; ++m) maxD = std::max(maxD, p->bond_dim[m]);
-  HBP_REQUIRE(maxD <= 512, HBP_ERR_UNSUPPORTED, "truncation: bond > 512");
-  double2 *U = (double2 *)ctx->alloc(tot * 16);
-  double2 *Vd = (double2 *)ctx->alloc(tot * 16);
-  double2 *W = (double2 *)ctx->alloc(2 * tot * 16);
-  double *sv = (double *)ctx->alloc((size_t)nm * maxD * 8);
-  std::vector<SvdJob> sj(nm);
-  for (int m = 0; m < nm; ++m) {
-    const int D = p->msg_dim(m);
-    sj[m] = SvdJob{p->E_off[m], p->E_off[m], (int64_t)m * maxD, p->E_off[m], 2 * p->E_off[m], D, D};
-  }
-  SvdJob *dsj = (SvdJob *)ctx->alloc(nm * sizeof(SvdJob));
-  ctx->upload(dsj, sj.data(), nm * sizeof(SvdJob));
-  launch_batched_svd(ctx->stream, p->E, U, sv, Vd, W, dsj, nm, maxD, maxD);
-  // 2. square roots and pseudo-inverse roots
-  double2 *roots = (double2 *)ctx->alloc(2 * tot * 16);
-  std::vector<RootJob> rj(nm);
-  for (int m = 0; m < nm; ++m)
-    rj[m] = RootJob{p->E_off[m], (int64_t)m * maxD, p->E_off[m], 2 * p->E_off[m],
-                    2 * p->E_off[m] + (int64_t)p->msg_dim(m) * p->msg_dim(m), p->msg_dim(m)};
-  RootJob *drj = (RootJob *)ctx->alloc(nm * sizeof(RootJob));
-  ctx->upload(drj, rj.data(), nm * sizeof(RootJob));
-  hipLaunchKernelGGL(k_roots, dim3(nm), dim3(256), 0, ctx->stream, U, sv, Vd, roots, drj, sv_min);
-  HBP_HIP(hipGetLastError());
-  // 3. T = Ra^T Rb per edge, normalised; 4. SVD of T
-  int64_t tt = 0;
-  std::vector<EdgeJob> ej(ne);
-  int64_t poff = 0;
-  for (int m = 0; m < ne; ++m) {
-    const int D = p->bond_dim[m];
-    EdgeJob j{};
-    j.D = D;
-    j.ra_off = rj[m].r_off;           // M_{a->b} = E[m]
-    j.sa_off = rj[m].si_off;
-    j.rb_off = rj[m + ne].r_off;      // M_{b->a} = E[m + ne]
-    j.sb_off = rj[m + ne].si_off;
-    j.t_off = tt;
-    j.u_off = tt;
-    j.vd_off = tt;
-    j.s_off = (int64_t)m * maxD;
-    j.pa_off = poff;
-    j.pb_off = poff + (int64_t)D * D;
-    j.e_off = poff + 2 * (int64_t)D * D;
-    poff += 3 * (int64_t)D * D;
-    tt += (int64_t)D * D;
-    ej[m] = j;
-  }
-  double2 *T = (double2 *)ctx->alloc(std::max<int64_t>(tt, 1) * 16);
-  EdgeJob *dej = (EdgeJob *)ctx->alloc(ne * sizeof(EdgeJob));
-  ctx->upload(dej, ej.data(), ne * sizeof(EdgeJob));
-  hipLaunchKernelGGL(k_rr, dim3(ne), dim3(256), 0, ctx->stream, roots, T, dej);
-  HBP_HIP(hipGetLastError());
-  std::vector<SvdJob> tj(ne);
-  for (int m = 0; m < ne; ++m) {
-    const int D = p->bond_dim[m];
-    tj[m] = SvdJob{ej[m].t_off, ej[m].u_off, ej[m].s_off, ej[m].vd_off, 2 * ej[m].t_off, D, D};
-  }
-  ctx->upload(dsj, tj.data(), ne * sizeof(SvdJob));
-  launch_batched_svd(ctx->stream, T, U, sv, Vd, W, dsj, ne, maxD, maxD);
-  // 5. selection and projectors
-  double2 *proj = (double2 *)ctx->alloc(std::max<int64_t>(poff, 1) * 16);
-  int *dkeep = (int *)ctx->alloc(ne * sizeof(int));
-  double *derr = (double *)ctx->alloc(ne * sizeof(double));
-  hipLaunchKernelGGL(k_projectors, dim3(ne), dim3(256), 0, ctx->stream, roots, U, sv, Vd, proj,
-                     dej, chi, sv_min, dkeep, derr);
-  HBP_HIP(hipGetLastError());
-  std::vector<int> keep(ne);
-  std::vector<double> err(ne);
-  HBP_HIP(hipMemcpyAsync(keep.data(), dkeep, ne * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-  HBP_HIP(hipMemcpyAsync(err.data(), derr, ne * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-  ctx->sync();
-  // 6. projectors into the sites
-  if (p->precision == HBP_PREC_FP64)
-    apply_projectors<double>(ctx, p, proj, ej, keep);
-  else
-    apply_projectors<float>(ctx, p, proj, ej, keep);
-  // 7. messages = diag(L) in both directions
-  for (int m = 0; m < ne; ++m) p->bond_dim[m] = keep[m];
-  double2 *nE;
-  size_t nbytes;
-  std::vector<int64_t> noff;
-  layout_messages(ctx, p, &nE, &nbytes, &noff);
-  if (ne > 0) {
-    std::vector<CopyJob> cj(ne);
-    for (int m = 0; m < ne; ++m) cj[m] = CopyJob{ej[m].e_off, noff[m], noff[m + ne], keep[m] * keep[m]};
-    CopyJob *dcj = (CopyJob *)ctx->alloc(ne * sizeof(CopyJob));
-    ctx->upload(dcj, cj.data(), ne * sizeof(CopyJob));
-    hipLaunchKernelGGL(k_copy_twice, dim3(ne), dim3(256), 0, ctx->stream, proj, nE, dcj);
-    HBP_HIP(hipGetLastError());
-    ctx->release(dcj, ne * sizeof(CopyJob));
-  }
-  ctx->release(p->E, p->E_bytes);
-  p->E = nE;
-  p->E_bytes = nbytes;
-  p->E_off = noff;
-  p->E_diag.assign(2 * ne, 1);
-  if (dims_out)
-    for (int m = 0; m < ne; ++m) dims_out[m] = keep[m];
-  if (errs_out)
-    for (int m = 0; m < ne; ++m) errs_out[m] = err[m];
-  ctx->release(dkeep, ne * sizeof(int));
-  ctx->release(derr, ne * sizeof(double));
-  ctx->release(proj, std::max<int64_t>(poff, 1) * 16);
-  ctx->release(dej, ne * sizeof(EdgeJob));
-  ctx->release(T, std::max<int64_t>(tt, 1) * 16);
-  ctx->release(drj, nm * sizeof(RootJob));
-  ctx->release(roots, 2 * tot * 16);
-  ctx->release(dsj, nm * sizeof(SvdJob));
-  ctx->release(sv, (size_t)nm * maxD * 8);
-  ctx->release(W, 2 * tot * 16);
-  ctx->release(Vd, tot * 16);
-  ctx->release(U, tot * 16);
+  std::vector<TruncMember> mem(1);
+  mem[0].p = p;
+  mem[0].chi = chi;
+  mem[0].sv_min = sv_min;
+  mem[0].dims_out = dims_out;
+  mem[0].errs_out = errs_out;
+  truncate_many(ctx, mem);
 }
 
 // ------------------------------------------------------------- measure
@@ -3395,6 +3529,30 @@ hbp_status hbp_peps_truncate(hbp_ctx *ctx, hbp_peps *p, int chi_max, double sv_m
   API_BEGIN
   HBP_REQUIRE(chi_max >= 1, HBP_ERR_INVALID_ARGUMENT, "truncate: chi_max < 1");
   truncate(ctx, p, chi_max, sv_min, bond_dims, trunc_errs);
+  API_END
+}
+
+hbp_status hbp_peps_truncate_batched(hbp_ctx *ctx, int n, hbp_peps *const *ps,
+                                     const int32_t *chi_max, const double *sv_min,
+                                     int32_t *bond_dims, double *trunc_errs) {
+  API_BEGIN
+  HBP_REQUIRE(n >= 0 && (n == 0 || (ps && chi_max && sv_min)), HBP_ERR_INVALID_ARGUMENT,
+              "truncate batched: null argument");
+  std::vector<TruncMember> mem(n);
+  int64_t off = 0;
+  for (int i = 0; i < n; ++i) {
+    HBP_REQUIRE(ps[i], HBP_ERR_INVALID_ARGUMENT, "truncate batched: null PEPS");
+    HBP_REQUIRE(chi_max[i] >= 1, HBP_ERR_INVALID_ARGUMENT, "truncate: chi_max < 1");
+    for (int j = 0; j < i; ++j)
+      HBP_REQUIRE(ps[i] != ps[j], HBP_ERR_INVALID_ARGUMENT, "truncate batched: a PEPS appears twice");
+    mem[i].p = ps[i];
+    mem[i].chi = chi_max[i];
+    mem[i].sv_min = sv_min[i];
+    mem[i].dims_out = bond_dims ? bond_dims + off : nullptr;
+    mem[i].errs_out = trunc_errs ? trunc_errs + off : nullptr;
+    off += ps[i]->n_edges;
+  }
+  if (n > 0) truncate_many(ctx, mem);
   API_END
 }
 

@@ -22,7 +22,8 @@ from ._lib import (HBP_ABSORB_AUTO, HBP_ABSORB_EAGER, HBP_ABSORB_LAZY, HBP_BATCH
 from .tci import Context
 
 __all__ = ["Peps", "InitTensorProductState", "AbsorbTPO", "BeliefPropagation",
-           "BeliefPropagationCondition", "bp_loop", "bp_loop_batched", "Truncation", "Measure", "MeasureEdges",
+           "BeliefPropagationCondition", "bp_loop", "bp_loop_batched", "Truncation",
+           "TruncationBatched", "Measure", "MeasureEdges",
            "HBP_PREC_FP64", "HBP_PREC_MIXED", "HBP_MODE_FUSED", "HBP_MODE_PLAIN",
            "HBP_SCHEDULE_JACOBI", "HBP_SCHEDULE_CHECKERBOARD", "HBP_BATCH_ALONE", "HBP_BATCH_SMALL",
            "HBP_BATCH_GRAPH"]
@@ -280,6 +281,29 @@ def Truncation(ctx, peps: Peps, max_dim: int, eps: float):
                                     dims.ctypes.data_as(C.POINTER(C.c_int32)),
                                     errs.ctypes.data_as(C.POINTER(C.c_double))))
     return list(map(int, dims)), list(map(float, errs))
+
+
+def TruncationBatched(ctx, peps_list, max_dims, epss):
+    """``Truncation`` of several PEPS of one context at once
+    (``hbp_peps_truncate_batched``: one launch per stage for all of them) ->
+    list of (bond dims, truncation errors) per PEPS, each bitwise what
+    ``Truncation`` gives that PEPS alone."""
+    n = len(peps_list)
+    if n == 0:
+        return []
+    hs = (C.c_void_p * n)(*[p.h.value if isinstance(p.h, C.c_void_p) else p.h for p in peps_list])
+    ne = [p.n_edges for p in peps_list]
+    dims = np.zeros(max(sum(ne), 1), dtype=np.int32)
+    errs = np.zeros(max(sum(ne), 1), dtype=np.float64)
+    check(ctx.lib.hbp_peps_truncate_batched(
+        ctx.h, n, hs, i32([int(x) for x in max_dims]),
+        np.ascontiguousarray(epss, dtype=np.float64).ctypes.data_as(C.POINTER(C.c_double)),
+        dims.ctypes.data_as(C.POINTER(C.c_int32)), errs.ctypes.data_as(C.POINTER(C.c_double))))
+    out, o = [], 0
+    for k in ne:
+        out.append((list(map(int, dims[o:o + k])), list(map(float, errs[o:o + k]))))
+        o += k
+    return out
 
 
 def Measure(ctx, peps: Peps, sites, ops):

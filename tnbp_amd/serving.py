@@ -36,7 +36,7 @@ import numpy as np
 from . import circuits, pauli
 from ._lib import (HBP_ABSORB_EAGER, HBP_ABSORB_LAZY, HBP_BATCH_GRAPH, HBP_BATCH_SMALL, HBP_PREC_FP64,
                    HBP_PREC_MIXED)
-from .peps import AbsorbTPO, InitTensorProductState, Truncation, bp_loop
+from .peps import AbsorbTPO, InitTensorProductState, Truncation, TruncationBatched, bp_loop
 from .tci import Context, device_memory
 
 
@@ -334,8 +334,9 @@ class BatchedServer:
     all resident jobs run as one ``hbp_peps_bp_run_batched`` call (one
     replay of a HIP graph per BP iteration for all of them: jobs whose bonds
     are <= 16 in one persistent launch over every site of every such job,
-    larger ones with their own fused sweeps recorded in the same graph), then each job
-    truncates; a job whose last slice is done is measured and leaves.  Queued
+    larger ones with their own fused sweeps recorded in the same graph), then
+    all resident jobs truncate in one ``hbp_peps_truncate_batched`` call; a
+    job whose last slice is done is measured and leaves.  Queued
     jobs are admitted by footprint at every round boundary, so a job that
     arrives while others run joins the next round.  A job's result does not
     depend on the others in the batch (every site's messages are computed by
@@ -454,8 +455,9 @@ class BatchedServer:
                         nb += int(bat == HBP_BATCH_SMALL)
                         ng += int(bat == HBP_BATCH_GRAPH)
                 t2 = time.perf_counter()
+                TruncationBatched(ctx, [r["peps"] for r in active],
+                                  [r["job"].chi for r in active], [r["job"].sv_min for r in active])
                 for r in active:
-                    Truncation(ctx, r["peps"], r["job"].chi, r["job"].sv_min)
                     r["layer"] += 1
                 t3 = time.perf_counter()
                 still = []
