@@ -322,6 +322,14 @@ hbp_status hbp_peps_store_sites(hbp_ctx *ctx, hbp_peps *p, hbp_tensor *const *ou
  * V'[(v o)...,p'] = sum_p O[o..,p,p'] V[v..,p]; messages become E (x) I_t. */
 hbp_status hbp_peps_absorb_tpo(hbp_ctx *ctx, hbp_peps *p, const double *const *ops,
                                const int32_t *tpo_shapes);
+/* hbp_peps_absorb_tpo for n PEPS of one context at once (no reference
+ * counterpart): ops[i] and tpo_shapes[i] are PEPS i's arguments.  Every eager
+ * PEPS's operators go up in one copy, all their sites up to 16 M elements are
+ * absorbed in one launch per precision, every message expansion in one
+ * launch; each PEPS's result is bitwise the one hbp_peps_absorb_tpo gives it. */
+hbp_status hbp_peps_absorb_tpo_batched(hbp_ctx *ctx, int n, hbp_peps *const *ps,
+                                       const double *const *const *ops,
+                                       const int32_t *const *tpo_shapes);
 
 /* tnbp::BeliefPropagation over all layers followed by E <- F
  * (bp.h:65-261, apps/kicked_ising_lattice/main.cc:128-141). */

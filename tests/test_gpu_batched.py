@@ -271,3 +271,54 @@ def test_truncate_batched_equals_each_alone(ctx):
     announce(f"  {len(a)} PEPS, bond dims after: {[max(g[0]) for g in got]}")
     for p in a + b:
         p.close()
+
+
+def test_absorb_batched_equals_each_alone(ctx):
+    """hbp_peps_absorb_tpo_batched over PEPS of both precisions, eager and lazy,
+    small (one batched launch) and large sites (a site of 33.5 M elements at
+    bonds 64 absorbs in its own launch): every site tensor and message
+    bitwise as AbsorbTPO leaves each PEPS alone."""
+    from tnbp_amd import circuits as pc
+    from tnbp_amd.kernel_probe import build_state
+    from tnbp_amd.peps import (HBP_ABSORB_EAGER, HBP_ABSORB_LAZY, AbsorbTPO, AbsorbTPOBatched,
+                               InitTensorProductState, Truncation, bp_loop)
+    announce("batched AbsorbTPO vs each PEPS alone")
+    rng = np.random.default_rng(31)
+
+    def states():
+        out, tpos = [], []
+        for k, (edges, prec, mode) in enumerate([(pc.square_lattice(3, 3), 0, HBP_ABSORB_EAGER),
+                                                 (pc.square_lattice(4, 4), 1, HBP_ABSORB_EAGER),
+                                                 (pc.chain(6), 0, HBP_ABSORB_LAZY),
+                                                 (pc.square_lattice(3, 4), 1, HBP_ABSORB_LAZY)]):
+            n = len({s for e in edges for s in e})
+            r = np.random.default_rng(100 + k)
+            layers = [pc.layer_tpo(edges, pc.random_layer(r, edges, n)) for _ in range(3)]
+            p = InitTensorProductState(ctx, edges, [2] * n, prec)
+            p.set_absorb_mode(mode)
+            for t in layers[:-1]:
+                AbsorbTPO(ctx, t, p)
+                bp_loop(ctx, p, 50, 1e-12)
+                Truncation(ctx, p, 8, 1e-12)
+            out.append(p)
+            tpos.append(layers[-1])
+        # bonds 32 -> 64: the interior site leaves the batched launch
+        q = build_state(ctx, 3, 32, 1, seed=9)
+        edges = pc.square_lattice(3, 3)
+        out.append(q)
+        tpos.append(pc.kicked_ising_tpo(edges, 0.125, 0.0, 0.11, math.pi))
+        return out, tpos
+
+    a, ta = states()
+    b, tb = states()
+    AbsorbTPOBatched(ctx, ta, a)
+    for q, t in zip(b, tb):
+        AbsorbTPO(ctx, t, q)
+    for p, q in zip(a, b):
+        for s in range(len(p.sites)):
+            np.testing.assert_array_equal(p.get_site(s), q.get_site(s))
+        for x, y in zip(_state(p), _state(q)):
+            np.testing.assert_array_equal(x, y)
+    assert max(a[-1].site_shape(4)[:-1]) == 64
+    for p in a + b:
+        p.close()

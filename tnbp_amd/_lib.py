@@ -91,6 +91,8 @@ SIGNATURES = {
     "hbp_peps_get_message": [_vp, _vp, C.c_int, _i32p, _dp],
     "hbp_peps_set_message": [_vp, _vp, C.c_int, C.c_int32, _dp],
     "hbp_peps_absorb_tpo": [_vp, _vp, C.POINTER(_dp), _i32p],
+    "hbp_peps_absorb_tpo_batched": [_vp, C.c_int, C.POINTER(_vp), C.POINTER(C.POINTER(_dp)),
+                                    C.POINTER(_i32p)],
     "hbp_peps_bp_sweep": [_vp, _vp],
     "hbp_peps_bp_condition": [_vp, _vp, _dp],
     "hbp_peps_bp_run": [_vp, _vp, C.c_int, C.c_double, C.POINTER(C.c_int), _dp],

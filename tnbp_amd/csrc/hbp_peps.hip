@@ -373,18 +373,17 @@ __global__ __launch_bounds__(256) void k_absorb_batch(const AbsorbJob *__restric
 // message jobs[b] (apply.h:110-128), element (e0 + D i0, e1 + D i1) of the
 // (D t) x (D t) result = E[e0, e1] if i0 == i1 else 0.
 struct ExpandJob {
-  int64_t src, dst;  // offsets (in elements) of the old and the new message
+  const double2 *src;  // the old message (D x D)
+  double2 *dst;        // the new one (D t x D t)
   int D, t;
 };
-__global__ __launch_bounds__(256) void k_expand_messages(const double2 *__restrict__ E,
-                                                         double2 *__restrict__ En,
-                                                         const ExpandJob *__restrict__ jobs) {
+__global__ __launch_bounds__(256) void k_expand_messages(const ExpandJob *__restrict__ jobs) {
   const ExpandJob j = jobs[blockIdx.x];
   const int D = j.D, Dn = j.D * j.t;
   for (int e = threadIdx.x; e < Dn * Dn; e += blockDim.x) {
     const int r = e % Dn, c = e / Dn;
     const int e0 = r % D, i0 = r / D, e1 = c % D, i1 = c / D;
-    En[j.dst + e] = (i0 == i1) ? E[j.src + e0 + D * e1] : make_double2(0.0, 0.0);
+    j.dst[e] = (i0 == i1) ? j.src[e0 + D * e1] : make_double2(0.0, 0.0);
   }
 }
 
@@ -2642,163 +2641,230 @@ hbp_status hbp_peps_store_sites(hbp_ctx *ctx, hbp_peps *p, hbp_tensor *const *ou
   API_END
 }
 
+// AbsorbTPO (apply.h:14-177) of several PEPS at once: the eager sites' operators of
+// every PEPS in one upload, their sites up to kAbsorbBatchMax elements in one
+// launch per precision (larger ones one launch each), every message
+// expansion E <- E (x) I_t in one launch, one wait at the end (the callers'
+// operator arrays may be freed on return).  Each site and message gets the
+// arithmetic of a PEPS absorbed on its own.
+struct AbsorbMember {
+  Peps *p;
+  const double *const *ops;
+  const int32_t *tpo_shapes;
+};
+static void absorb_many(Ctx *ctx, const std::vector<AbsorbMember> &mem) {
+  constexpr size_t kAbsorbBatchMax = (size_t)1 << 24;
+  std::vector<std::vector<int>> tdim(mem.size());
+  std::vector<char> lazy(mem.size(), 0);
+  // checks, and the state before the absorption (a second absorption before
+  // BP composes with the first: that one is stored first)
+  for (size_t i = 0; i < mem.size(); ++i) {
+    Peps *p = mem[i].p;
+    const int32_t *tpo_shapes = mem[i].tpo_shapes;
+    const int ns = p->n_sites, ne = p->n_edges;
+    std::vector<int> &t = tdim[i];
+    t.assign(ne, 0);
+    for (int s = 0; s < ns; ++s) {
+      const int32_t *sh = tpo_shapes + (size_t)s * HBP_MAX_ORDER;
+      const int z = p->z(s);
+      HBP_REQUIRE(sh[z] == p->phys[s], HBP_ERR_INVALID_ARGUMENT, "absorb: d_in differs from phys");
+      for (int k = 0; k < z; ++k) {
+        const int m = p->bonds[s][k];
+        HBP_REQUIRE(sh[k] >= 1, HBP_ERR_INVALID_ARGUMENT, "absorb: TPO bond < 1");
+        HBP_REQUIRE(t[m] == 0 || t[m] == sh[k], HBP_ERR_INVALID_ARGUMENT,
+                    "absorb: TPO bond dims differ across an edge");
+        t[m] = sh[k];
+      }
+    }
+  }
+  for (size_t i = 0; i < mem.size(); ++i) {
+    Peps *p = mem[i].p;
+    drop_pair_cache(ctx, p);
+    for (int s = 0; s < p->n_sites; ++s) make_resident(ctx, p, s);
+    // lazy: keep V_s and O_s, materialise per sweep; eager: store the absorbed
+    // lattice (AUTO: eager while it takes at most half of the free device memory)
+    if (p->mode == HBP_MODE_FUSED && p->absorb_mode != HBP_ABSORB_EAGER) {
+      lazy[i] = 1;
+      if (p->absorb_mode == HBP_ABSORB_AUTO) {
+        size_t total = 0;
+        for (int s = 0; s < p->n_sites; ++s) {
+          const int32_t *osh = mem[i].tpo_shapes + (size_t)s * HBP_MAX_ORDER;
+          size_t n = (size_t)osh[p->z(s) + 1];
+          for (int k = 0; k < p->z(s); ++k) n *= (size_t)p->bond_dim[p->bonds[s][k]] * osh[k];
+          total += n * p->elem_bytes();
+        }
+        // (memory the context pool holds in reserve counts as free)
+        lazy[i] = total > ctx->free_bytes() / 2;
+      }
+    }
+  }
+  // the eager PEPS' operators in one upload (a pageable copy per site would
+  // wait for the previous site's absorption each time)
+  std::vector<std::vector<size_t>> op_off(mem.size());
+  size_t op_total = 0;
+  for (size_t i = 0; i < mem.size(); ++i) {
+    Peps *p = mem[i].p;
+    op_off[i].assign(p->n_sites + 1, 0);
+    for (int s = 0; s < p->n_sites; ++s) {
+      const int32_t *osh = mem[i].tpo_shapes + (size_t)s * HBP_MAX_ORDER;
+      size_t on = 1;
+      for (int k = 0; k <= p->z(s) + 1; ++k) on *= (size_t)osh[k];
+      op_off[i][s + 1] = op_off[i][s] + on;
+    }
+    if (lazy[i]) continue;
+    for (int s = 0; s <= p->n_sites; ++s) op_off[i][s] += op_total;
+    op_total = op_off[i][p->n_sites];
+  }
+  double2 *dAll = nullptr;
+  if (op_total > 0) {
+    std::vector<double2> packed(op_total);
+    for (size_t i = 0; i < mem.size(); ++i) {
+      if (lazy[i]) continue;
+      for (int s = 0; s < mem[i].p->n_sites; ++s)
+        std::memcpy(packed.data() + op_off[i][s], mem[i].ops[s],
+                    (op_off[i][s + 1] - op_off[i][s]) * 16);
+    }
+    dAll = (double2 *)ctx->alloc(op_total * 16);
+    ctx->upload(dAll, packed.data(), op_total * 16);
+  }
+  // eager sites up to kAbsorbBatchMax elements: one launch per precision
+  std::vector<AbsorbJob> batch[2];
+  std::vector<std::pair<Peps *, int>> batch_site[2];
+  std::vector<size_t> batch_bytes[2];
+  for (size_t i = 0; i < mem.size(); ++i) {
+    Peps *p = mem[i].p;
+    for (int s = 0; s < p->n_sites; ++s) {
+      const int32_t *osh = mem[i].tpo_shapes + (size_t)s * HBP_MAX_ORDER;
+      const int z = p->z(s);
+      AbsorbArgs a{};
+      a.z = z;
+      a.din = osh[z];
+      a.dout = osh[z + 1];
+      int64_t vs = 1, os = 1, ns_ = 1;
+      for (int k = 0; k < z; ++k) {
+        const int D = p->bond_dim[p->bonds[s][k]];
+        a.vdim[k] = D;
+        a.vstride[k] = vs;
+        vs *= D;
+        a.odim[k] = osh[k];
+        a.ostride[k] = os;
+        os *= osh[k];
+        a.ndim[k] = D * osh[k];
+        a.nstride[k] = ns_;
+        ns_ *= D * osh[k];
+      }
+      a.vstride[z] = vs;
+      a.ostride[z] = os;
+      a.ostride[z + 1] = os * osh[z];
+      a.ndim[z] = a.dout;
+      a.fast = (uint64_t)ns_ * (uint64_t)a.dout < (uint64_t(1) << 32) ? 1 : 0;
+      for (int k = 0; k < z; ++k) {
+        fast_div_params((uint32_t)a.ndim[k], &a.nm[k], &a.ns[k]);
+        fast_div_params((uint32_t)a.vdim[k], &a.vm[k], &a.vs[k]);
+      }
+      const size_t on = op_off[i][s + 1] - op_off[i][s];
+      const size_t n = (size_t)ns_ * a.dout;
+      p->phys[s] = a.dout;
+      if (lazy[i]) {
+        double2 *dO = (double2 *)ctx->alloc(on * 16);
+        HBP_HIP(hipMemcpyAsync(dO, mem[i].ops[s], on * 16, hipMemcpyHostToDevice, ctx->stream));
+        p->pending[s] = 1;
+        p->pend_O[s] = dO;
+        p->pend_O_bytes[s] = on * 16;
+        p->pend_args[s] = a;
+        if (p->abs_site == s) p->abs_site = -1;
+        continue;
+      }
+      const size_t bytes = n * p->elem_bytes();
+      void *nv = ctx->alloc(bytes);
+      const double2 *dO = dAll + op_off[i][s];
+      if (n <= kAbsorbBatchMax) {
+        // joins the launch over the small sites below (the old tensor is
+        // released once that launch is enqueued)
+        const int g = p->precision == HBP_PREC_MIXED ? 1 : 0;
+        batch[g].push_back(AbsorbJob{p->V[s], dO, nv, n, a});
+        batch_site[g].emplace_back(p, s);
+        batch_bytes[g].push_back(bytes);
+        continue;
+      }
+      if (p->precision == HBP_PREC_FP64)
+        hipLaunchKernelGGL(k_absorb_site<double2>, dim3(gridn(n)), dim3(256), 0, ctx->stream,
+                           (const double2 *)p->V[s], dO, (double2 *)nv, n, a);
+      else
+        hipLaunchKernelGGL(k_absorb_site<float2>, dim3(gridn(n)), dim3(256), 0, ctx->stream,
+                           (const float2 *)p->V[s], dO, (float2 *)nv, n, a);
+      HBP_HIP(hipGetLastError());
+      release_site(ctx, p, s);
+      p->V[s] = nv;
+      p->V_bytes[s] = bytes;
+    }
+  }
+  for (int g = 0; g < 2; ++g) {
+    if (batch[g].empty()) continue;
+    absorb_batch(ctx, batch_site[g][0].first, batch[g]);
+    for (size_t j = 0; j < batch[g].size(); ++j) {
+      Peps *p = batch_site[g][j].first;
+      const int s = batch_site[g][j].second;
+      release_site(ctx, p, s);
+      p->V[s] = batch[g][j].Vn;
+      p->V_bytes[s] = batch_bytes[g][j];
+    }
+  }
+  if (dAll) ctx->release(dAll, op_total * 16);
+  // E_e <- E_e (x) I_t for every message of every PEPS in one launch
+  std::vector<ExpandJob> xj;
+  std::vector<std::pair<double2 *, size_t>> old_E;
+  for (size_t i = 0; i < mem.size(); ++i) {
+    Peps *p = mem[i].p;
+    const int ne = p->n_edges;
+    std::vector<int> old = p->bond_dim;
+    std::vector<int64_t> old_off = p->E_off;
+    for (int m = 0; m < ne; ++m) p->bond_dim[m] = old[m] * tdim[i][m];
+    double2 *nE;
+    size_t nbytes;
+    std::vector<int64_t> noff;
+    layout_messages(ctx, p, &nE, &nbytes, &noff);
+    for (int m = 0; m < 2 * ne; ++m)
+      xj.push_back(ExpandJob{p->E + old_off[m], nE + noff[m], old[m % ne], tdim[i][m % ne]});
+    old_E.emplace_back(p->E, p->E_bytes);
+    p->E = nE;
+    p->E_bytes = nbytes;
+    p->E_off = noff;
+  }
+  if (!xj.empty()) {
+    ExpandJob *dxj = (ExpandJob *)ctx->alloc(xj.size() * sizeof(ExpandJob));
+    ctx->upload(dxj, xj.data(), xj.size() * sizeof(ExpandJob));
+    hipLaunchKernelGGL(k_expand_messages, dim3((unsigned)xj.size()), dim3(256), 0, ctx->stream,
+                       dxj);
+    HBP_HIP(hipGetLastError());
+    ctx->release(dxj, xj.size() * sizeof(ExpandJob));
+  }
+  for (auto &e : old_E) ctx->release(e.first, e.second);
+  ctx->sync();  // the caller's operator arrays may be freed on return
+}
+
 hbp_status hbp_peps_absorb_tpo(hbp_ctx *ctx, hbp_peps *p, const double *const *ops,
                                const int32_t *tpo_shapes) {
   API_BEGIN
-  const int ns = p->n_sites, ne = p->n_edges;
-  // TPO bond dims per edge, checked at both ends
-  std::vector<int> t(ne, 0);
-  for (int s = 0; s < ns; ++s) {
-    const int32_t *sh = tpo_shapes + (size_t)s * HBP_MAX_ORDER;
-    const int z = p->z(s);
-    HBP_REQUIRE(sh[z] == p->phys[s], HBP_ERR_INVALID_ARGUMENT, "absorb: d_in differs from phys");
-    for (int k = 0; k < z; ++k) {
-      const int m = p->bonds[s][k];
-      HBP_REQUIRE(sh[k] >= 1, HBP_ERR_INVALID_ARGUMENT, "absorb: TPO bond < 1");
-      HBP_REQUIRE(t[m] == 0 || t[m] == sh[k], HBP_ERR_INVALID_ARGUMENT,
-                  "absorb: TPO bond dims differ across an edge");
-      t[m] = sh[k];
-    }
+  absorb_many(ctx, {AbsorbMember{p, ops, tpo_shapes}});
+  API_END
+}
+
+hbp_status hbp_peps_absorb_tpo_batched(hbp_ctx *ctx, int n, hbp_peps *const *ps,
+                                       const double *const *const *ops,
+                                       const int32_t *const *tpo_shapes) {
+  API_BEGIN
+  HBP_REQUIRE(n >= 0 && (n == 0 || (ps && ops && tpo_shapes)), HBP_ERR_INVALID_ARGUMENT,
+              "absorb batched: null argument");
+  std::vector<AbsorbMember> mem;
+  for (int i = 0; i < n; ++i) {
+    HBP_REQUIRE(ps[i] && ops[i] && tpo_shapes[i], HBP_ERR_INVALID_ARGUMENT,
+                "absorb batched: null PEPS or operator");
+    for (int j = 0; j < i; ++j)
+      HBP_REQUIRE(ps[i] != ps[j], HBP_ERR_INVALID_ARGUMENT, "absorb batched: a PEPS appears twice");
+    mem.push_back(AbsorbMember{ps[i], ops[i], tpo_shapes[i]});
   }
-  drop_pair_cache(ctx, p);
-  // a second absorption before BP composes with the first: store the first
-  for (int s = 0; s < ns; ++s) make_resident(ctx, p, s);
-  // lazy: keep V_s and O_s, materialise per sweep; eager: store the absorbed
-  // lattice (AUTO: eager while it takes at most half of the free device memory)
-  bool lazy = false;
-  if (p->mode == HBP_MODE_FUSED && p->absorb_mode != HBP_ABSORB_EAGER) {
-    lazy = true;
-    if (p->absorb_mode == HBP_ABSORB_AUTO) {
-      size_t total = 0;
-      for (int s = 0; s < ns; ++s) {
-        const int32_t *osh = tpo_shapes + (size_t)s * HBP_MAX_ORDER;
-        size_t n = (size_t)osh[p->z(s) + 1];
-        for (int k = 0; k < p->z(s); ++k) n *= (size_t)p->bond_dim[p->bonds[s][k]] * osh[k];
-        total += n * p->elem_bytes();
-      }
-      // (memory the context pool holds in reserve counts as free)
-      lazy = total > ctx->free_bytes() / 2;
-    }
-  }
-  // eager: every site's operator in one upload (a pageable copy per site
-  // would wait for the previous site's absorption each time)
-  std::vector<size_t> op_off(ns + 1, 0);
-  for (int s = 0; s < ns; ++s) {
-    const int32_t *osh = tpo_shapes + (size_t)s * HBP_MAX_ORDER;
-    size_t on = 1;
-    for (int k = 0; k <= p->z(s) + 1; ++k) on *= (size_t)osh[k];
-    op_off[s + 1] = op_off[s] + on;
-  }
-  double2 *dAll = nullptr;
-  // eager sites up to kAbsorbBatchMax elements are absorbed in one launch
-  constexpr size_t kAbsorbBatchMax = (size_t)1 << 24;
-  std::vector<AbsorbJob> batch;
-  std::vector<int> batch_site;
-  std::vector<size_t> batch_bytes;
-  if (!lazy) {
-    std::vector<double2> packed(std::max<size_t>(op_off[ns], 1));
-    for (int s = 0; s < ns; ++s)
-      std::memcpy(packed.data() + op_off[s], ops[s], (op_off[s + 1] - op_off[s]) * 16);
-    dAll = (double2 *)ctx->alloc(packed.size() * 16);
-    ctx->upload(dAll, packed.data(), packed.size() * 16);
-  }
-  for (int s = 0; s < ns; ++s) {
-    const int32_t *osh = tpo_shapes + (size_t)s * HBP_MAX_ORDER;
-    const int z = p->z(s);
-    AbsorbArgs a{};
-    a.z = z;
-    a.din = osh[z];
-    a.dout = osh[z + 1];
-    int64_t vs = 1, os = 1, ns_ = 1;
-    size_t on = 1;
-    for (int k = 0; k < z; ++k) {
-      const int D = p->bond_dim[p->bonds[s][k]];
-      a.vdim[k] = D;
-      a.vstride[k] = vs;
-      vs *= D;
-      a.odim[k] = osh[k];
-      a.ostride[k] = os;
-      os *= osh[k];
-      a.ndim[k] = D * osh[k];
-      a.nstride[k] = ns_;
-      ns_ *= D * osh[k];
-    }
-    a.vstride[z] = vs;
-    a.ostride[z] = os;
-    a.ostride[z + 1] = os * osh[z];
-    a.ndim[z] = a.dout;
-    a.fast = (uint64_t)ns_ * (uint64_t)a.dout < (uint64_t(1) << 32) ? 1 : 0;
-    for (int k = 0; k < z; ++k) {
-      fast_div_params((uint32_t)a.ndim[k], &a.nm[k], &a.ns[k]);
-      fast_div_params((uint32_t)a.vdim[k], &a.vm[k], &a.vs[k]);
-    }
-    for (int k = 0; k <= z + 1; ++k) on *= (size_t)osh[k];
-    const size_t n = (size_t)ns_ * a.dout;
-    p->phys[s] = a.dout;
-    if (lazy) {
-      double2 *dO = (double2 *)ctx->alloc(on * 16);
-      HBP_HIP(hipMemcpyAsync(dO, ops[s], on * 16, hipMemcpyHostToDevice, ctx->stream));
-      p->pending[s] = 1;
-      p->pend_O[s] = dO;
-      p->pend_O_bytes[s] = on * 16;
-      p->pend_args[s] = a;
-      if (p->abs_site == s) p->abs_site = -1;
-      continue;
-    }
-    const size_t bytes = n * p->elem_bytes();
-    void *nv = ctx->alloc(bytes);
-    const double2 *dO = dAll + op_off[s];
-    if (n <= kAbsorbBatchMax) {
-      // joins the one launch over the small sites below (the old tensor is
-      // released once that launch is enqueued)
-      batch.push_back(AbsorbJob{p->V[s], dO, nv, n, a});
-      batch_site.push_back(s);
-      batch_bytes.push_back(bytes);
-      continue;
-    }
-    if (p->precision == HBP_PREC_FP64)
-      hipLaunchKernelGGL(k_absorb_site<double2>, dim3(gridn(n)), dim3(256), 0, ctx->stream,
-                         (const double2 *)p->V[s], dO, (double2 *)nv, n, a);
-    else
-      hipLaunchKernelGGL(k_absorb_site<float2>, dim3(gridn(n)), dim3(256), 0, ctx->stream,
-                         (const float2 *)p->V[s], dO, (float2 *)nv, n, a);
-    HBP_HIP(hipGetLastError());
-    release_site(ctx, p, s);
-    p->V[s] = nv;
-    p->V_bytes[s] = bytes;
-  }
-  if (!batch.empty()) {
-    absorb_batch(ctx, p, batch);
-    const int nj = (int)batch.size();
-    for (int j = 0; j < nj; ++j) {
-      const int s = batch_site[j];
-      release_site(ctx, p, s);
-      p->V[s] = batch[j].Vn;
-      p->V_bytes[s] = batch_bytes[j];
-    }
-  }
-  if (dAll) ctx->release(dAll, std::max<size_t>(op_off[ns], 1) * 16);
-  std::vector<int> old = p->bond_dim;
-  std::vector<int64_t> old_off = p->E_off;
-  for (int m = 0; m < ne; ++m) p->bond_dim[m] = old[m] * t[m];
-  double2 *nE;
-  size_t nbytes;
-  std::vector<int64_t> noff;
-  layout_messages(ctx, p, &nE, &nbytes, &noff);
-  if (ne > 0) {
-    std::vector<ExpandJob> xj(2 * ne);
-    for (int m = 0; m < 2 * ne; ++m) xj[m] = ExpandJob{old_off[m], noff[m], old[m % ne], t[m % ne]};
-    ExpandJob *dxj = (ExpandJob *)ctx->alloc(2 * ne * sizeof(ExpandJob));
-    ctx->upload(dxj, xj.data(), 2 * ne * sizeof(ExpandJob));
-    hipLaunchKernelGGL(k_expand_messages, dim3(2 * ne), dim3(256), 0, ctx->stream, p->E, nE, dxj);
-    HBP_HIP(hipGetLastError());
-    ctx->release(dxj, 2 * ne * sizeof(ExpandJob));
-  }
-  ctx->release(p->E, p->E_bytes);
-  p->E = nE;
-  p->E_bytes = nbytes;
-  p->E_off = noff;
-  ctx->sync();  // the caller's operator arrays may be freed on return
+  if (!mem.empty()) absorb_many(ctx, mem);
   API_END
 }
 

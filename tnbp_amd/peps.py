@@ -22,7 +22,7 @@ from ._lib import (HBP_ABSORB_AUTO, HBP_ABSORB_EAGER, HBP_ABSORB_LAZY, HBP_BATCH
 from .tci import Context
 
 __all__ = ["Peps", "InitTensorProductState", "AbsorbTPO", "BeliefPropagation",
-           "BeliefPropagationCondition", "bp_loop", "bp_loop_batched", "Truncation",
+           "BeliefPropagationCondition", "bp_loop", "bp_loop_batched", "Truncation", "AbsorbTPOBatched",
            "TruncationBatched", "Measure", "MeasureEdges",
            "HBP_PREC_FP64", "HBP_PREC_MIXED", "HBP_MODE_FUSED", "HBP_MODE_PLAIN",
            "HBP_SCHEDULE_JACOBI", "HBP_SCHEDULE_CHECKERBOARD", "HBP_BATCH_ALONE", "HBP_BATCH_SMALL",
@@ -196,17 +196,37 @@ def InitTensorProductState(ctx, edges, pdim=None, precision=HBP_PREC_FP64) -> Pe
     return Peps(ctx, edges, pdim, precision)
 
 
-def AbsorbTPO(ctx, O, peps: Peps) -> None:
-    """``AbsorbTPO`` (apply.h:14-177); ``O[s]`` is site s's operator
-    (t_1..t_z, d_in, d_out) as a numpy complex array."""
+def _tpo_args(O):
+    """(keep-alive arrays, pointer array, flat shapes) of a TPO for the C-ABI."""
     arrs = [np.ascontiguousarray(np.asarray(o, dtype=np.complex128).reshape(-1, order="F"))
             for o in O]
     ptrs = (C.POINTER(C.c_double) * len(arrs))(*[a.ctypes.data_as(C.POINTER(C.c_double)) for a in arrs])
     shapes = []
     for o in O:
-        sh = list(np.shape(o)) + [0] * (HBP_MAX_ORDER - np.ndim(o))
-        shapes += sh
-    check(ctx.lib.hbp_peps_absorb_tpo(ctx.h, peps.h, ptrs, i32(shapes)))
+        shapes += list(np.shape(o)) + [0] * (HBP_MAX_ORDER - np.ndim(o))
+    return arrs, ptrs, i32(shapes)
+
+
+def AbsorbTPO(ctx, O, peps: Peps) -> None:
+    """``AbsorbTPO`` (apply.h:14-177); ``O[s]`` is site s's operator
+    (t_1..t_z, d_in, d_out) as a numpy complex array."""
+    _, ptrs, shapes = _tpo_args(O)
+    check(ctx.lib.hbp_peps_absorb_tpo(ctx.h, peps.h, ptrs, shapes))
+
+
+def AbsorbTPOBatched(ctx, Os, peps_list) -> None:
+    """``AbsorbTPO`` of several PEPS of one context at once
+    (``hbp_peps_absorb_tpo_batched``); ``Os[i]`` is PEPS i's operator list.
+    Each PEPS ends bitwise as ``AbsorbTPO`` leaves it alone."""
+    n = len(peps_list)
+    if n == 0:
+        return
+    args = [_tpo_args(O) for O in Os]
+    hs = (C.c_void_p * n)(*[p.h.value if isinstance(p.h, C.c_void_p) else p.h for p in peps_list])
+    ops = (C.POINTER(C.POINTER(C.c_double)) * n)(*[C.cast(a[1], C.POINTER(C.POINTER(C.c_double)))
+                                                   for a in args])
+    shp = (C.POINTER(C.c_int32) * n)(*[C.cast(a[2], C.POINTER(C.c_int32)) for a in args])
+    check(ctx.lib.hbp_peps_absorb_tpo_batched(ctx.h, n, hs, ops, shp))
 
 
 def BeliefPropagation(ctx, peps: Peps) -> None:

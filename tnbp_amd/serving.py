@@ -36,7 +36,8 @@ import numpy as np
 from . import circuits, pauli
 from ._lib import (HBP_ABSORB_EAGER, HBP_ABSORB_LAZY, HBP_BATCH_GRAPH, HBP_BATCH_SMALL, HBP_PREC_FP64,
                    HBP_PREC_MIXED)
-from .peps import AbsorbTPO, InitTensorProductState, Truncation, TruncationBatched, bp_loop
+from .peps import (AbsorbTPO, AbsorbTPOBatched, InitTensorProductState, Truncation,
+                   TruncationBatched, bp_loop)
 from .tci import Context, device_memory
 
 
@@ -330,7 +331,8 @@ class BatchedServer:
     context (no reference counterpart).
 
     One scheduler thread keeps every admitted job's PEPS resident and runs
-    rounds: each resident job absorbs its next TPO slice, then the BP loops of
+    rounds: every resident job absorbs its next TPO slice (one
+    ``hbp_peps_absorb_tpo_batched`` call), then the BP loops of
     all resident jobs run as one ``hbp_peps_bp_run_batched`` call (one
     replay of a HIP graph per BP iteration for all of them: jobs whose bonds
     are <= 16 in one persistent launch over every site of every such job,
@@ -441,8 +443,8 @@ class BatchedServer:
                 if not active:
                     continue
                 t0 = time.perf_counter()
-                for r in active:
-                    AbsorbTPO(ctx, r["job"].tpos[r["layer"]], r["peps"])
+                AbsorbTPOBatched(ctx, [r["job"].tpos[r["layer"]] for r in active],
+                                 [r["peps"] for r in active])
                 t1 = time.perf_counter()
                 groups = {}
                 for r in active:
