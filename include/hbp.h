@@ -269,8 +269,10 @@ hbp_status hbp_peps_set_absorb_mode(hbp_peps *p, int mode);
  * ~64); MERGED is PER_SITE with the last mode product of each outgoing
  * message computed inside the message kernel (bonds in (32, 64] on both of
  * the two bonds involved; other messages as PER_SITE), which saves writing
- * and re-reading that product.  AUTO (default) is MERGED (which falls back
- * to PER_SITE launches wherever bonds are outside (32, 64]).  GROUPED (one
+ * and re-reading that product, and with every resident site whose bonds are
+ * all <= 16 (degree <= 4) in one launch of the small-site kernel per sweep
+ * (one workgroup per site, FP64 sums).  AUTO (default) is MERGED (which
+ * falls back to PER_SITE launches wherever bonds are outside (32, 64]).  GROUPED (one
  * launch per sweep level for all sites) is reserved and rejected with
  * HBP_ERR_UNSUPPORTED: its first GPU run left the device unusable, the cause
  * was not found and its code was taken out; hbp_peps_bp_run_batched covers

@@ -135,6 +135,30 @@ def test_svd(ctx, shape, nrows):
     np.testing.assert_allclose(Um.conj().T @ Um, np.eye(k), atol=1e-12)
 
 
+def test_svd_rank_deficient(ctx):
+    """Rank-deficient matrices, the shape of truncation's Ra^T Rb when the
+    messages have low rank: the Jacobi rotations turn some columns into
+    rounding noise, and pairs of such columns count as converged; rotating
+    them by the relative orthogonality test alone exhausts the sweep limit
+    and leaves singular values wrong by up to 1e-5 on such inputs."""
+    from tnbp_amd import tci as T
+    announce("svd of rank-deficient matrices (LDS and global-memory forms)")
+    rng = np.random.default_rng(7)
+    shapes = [(n, int(rng.integers(1, n))) for n in rng.integers(3, 9, size=40)] + [(80, 20), (72, 5)]
+    for i, (n, r) in enumerate(shapes):
+        a = _rand(rng, (n, r)) @ _rand(rng, (r, n))
+        if i % 2:
+            a[rng.integers(0, n), :] = 0
+        a /= np.linalg.norm(a)
+        u, s, vd = T.svd(ctx, T.Tensor.from_numpy(ctx, a), 1)
+        U, S, Vd = u.to_numpy(), s.to_numpy(), vd.to_numpy()
+        _, s_ref, _ = ot.svd(a, 1)
+        np.testing.assert_allclose(S, s_ref, rtol=0, atol=1e-13, err_msg=f"matrix {i} ({n}x{n}, rank {r})")
+        np.testing.assert_allclose(U @ np.diag(S) @ Vd, a, rtol=0, atol=1e-13)
+        Ur = U[:, :r]
+        np.testing.assert_allclose(Ur.conj().T @ Ur, np.eye(r), atol=1e-12)
+
+
 def test_trunc_svd_rule(ctx):
     from tnbp_amd import tci as T
     rng = np.random.default_rng(3)

@@ -92,10 +92,15 @@ __device__ void small_message(const typename CT<S>::type *__restrict__ T,
   __syncthreads();
 }
 
+// E_base / raw_base (both null, or both set): the messages every job reads
+// and the raw messages it writes, the jobs' E and raw then unused (the
+// small sites of one PEPS inside its fused sweep)
 template <typename S>
 __global__ __launch_bounds__(256) void k_sweep_small(const SmallSiteJob *__restrict__ jobs,
                                                      int njobs, void *__restrict__ scratch,
-                                                     int64_t slab_elems) {
+                                                     int64_t slab_elems,
+                                                     const double2 *__restrict__ E_base,
+                                                     double2 *__restrict__ raw_base) {
   using C = typename CT<S>::type;
   __shared__ double2 Es[SMALL_D_MAX * SMALL_D_MAX];
   __shared__ double2 red[256];
@@ -114,7 +119,7 @@ __global__ __launch_bounds__(256) void k_sweep_small(const SmallSiteJob *__restr
         for (int d = 0; d < l; ++d) L *= job.dims[d];
         for (int d = l + 1; d <= z; ++d) R *= job.dims[d];
         const int D = job.dims[l];
-        const double2 *E = job.E + job.in_off[l];
+        const double2 *E = (E_base ? E_base : job.E) + job.in_off[l];
         for (int e = threadIdx.x; e < D * D; e += blockDim.x) Es[e] = E[e];
         __syncthreads();
         small_mode_product<S>(cur, nxt, L, D, D, R, Es);
@@ -125,7 +130,8 @@ __global__ __launch_bounds__(256) void k_sweep_small(const SmallSiteJob *__restr
       int64_t L = 1, R = 1;
       for (int d = 0; d < k; ++d) L *= job.dims[d];
       for (int d = k + 1; d <= z; ++d) R *= job.dims[d];
-      small_message<S>(cur, V, job.raw + job.out_off[k], L, job.dims[k], R, red);
+      small_message<S>(cur, V, (raw_base ? raw_base : job.raw) + job.out_off[k], L, job.dims[k], R,
+                       red);
     }
   }
 }
@@ -323,15 +329,15 @@ template void launch_project_small<float>(hipStream_t, const SmallProjJob *, int
 
 template <typename S>
 void launch_sweep_small(hipStream_t st, const SmallSiteJob *jobs_dev, int njobs, void *scratch,
-                        int64_t slab_elems, int grid) {
+                        int64_t slab_elems, int grid, const double2 *E_base, double2 *raw_base) {
   if (njobs <= 0) return;
   hipLaunchKernelGGL(k_sweep_small<S>, dim3((unsigned)grid), dim3(256), 0, st, jobs_dev, njobs,
-                     scratch, slab_elems);
+                     scratch, slab_elems, E_base, raw_base);
   HBP_HIP(hipGetLastError());
 }
 template void launch_sweep_small<double>(hipStream_t, const SmallSiteJob *, int, void *, int64_t,
-                                         int);
+                                         int, const double2 *, double2 *);
 template void launch_sweep_small<float>(hipStream_t, const SmallSiteJob *, int, void *, int64_t,
-                                        int);
+                                        int, const double2 *, double2 *);
 
 }  // namespace hbp

@@ -267,7 +267,8 @@ struct SmallSiteJob {
 };
 template <typename S>
 void launch_sweep_small(hipStream_t st, const SmallSiteJob *jobs_dev, int njobs, void *scratch,
-                        int64_t slab_elems, int grid);
+                        int64_t slab_elems, int grid, const double2 *E_base = nullptr,
+                        double2 *raw_base = nullptr);
 // Truncation of one site with bonds <= SMALL_D_MAX (truncation.h:156-217):
 // out = normalise(V x_0 P_0 x_1 ... x_{z-1} P_{z-1}) with the projector of
 // every bond (dims[k] x keep[k], complex double: P[i + dims[k] j], or
@@ -316,7 +317,8 @@ int small_sweep_grid(int variant, int njobs);
 // the stream is being captured.
 template <typename S>
 int launch_sweep_small_tuned(hipStream_t st, const SmallSiteJob *jobs_dev, int njobs,
-                             void *scratch, int64_t slab_elems);
+                             void *scratch, int64_t slab_elems, const double2 *E_base = nullptr,
+                             double2 *raw_base = nullptr);
 
 // the autotuner's choices so far, one line per shape class
 std::string autotune_report();

@@ -1488,15 +1488,29 @@ __global__ __launch_bounds__(256) void k_batched_svd(const double2 *__restrict__
   double2 *W = IN_LDS ? svd_lds : work + job.work_off;  // m x n
   double2 *Vm = W + (int64_t)m * n;                     // n x n
   const double2 *A = Ain + job.a_off;
-  for (int64_t e = threadIdx.x; e < (int64_t)m * n; e += blockDim.x) W[e] = A[e];
+  double fro = 0.0;
+  for (int64_t e = threadIdx.x; e < (int64_t)m * n; e += blockDim.x) {
+    const double2 x = A[e];
+    W[e] = x;
+    fro += x.x * x.x + x.y * x.y;
+  }
   for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += blockDim.x) {
     const int i = (int)(e % n), j = (int)(e / n);
     Vm[e] = make_double2(i == j ? 1.0 : 0.0, 0.0);
   }
   __shared__ int rotated;
+  __shared__ double fro_part[4];
+  fro = wave_sum(fro);
+  if (lane == 0) fro_part[wave] = fro;
   __syncthreads();
   const int np = (n + 1) & ~1;  // players incl. a dummy when n is odd
   const double tol = 1.0e-15;
+  // A column whose squared norm is below 1e-30 ||A||_F^2 is rounding noise:
+  // rotating it against another column changes nothing at the scale of A,
+  // and the relative orthogonality test would keep such pairs rotating to
+  // the sweep limit (rank-deficient inputs such as Ra^T Rb of two roots of
+  // rank-2 messages), so such pairs count as converged.
+  const double small2 = 1.0e-30 * (fro_part[0] + fro_part[1] + fro_part[2] + fro_part[3]);
   for (int sweep = 0; sweep < 60; ++sweep) {
     if (threadIdx.x == 0) rotated = 0;
     __syncthreads();
@@ -1519,9 +1533,13 @@ __global__ __launch_bounds__(256) void k_batched_svd(const double2 *__restrict__
         gr = wave_sum(gr);
         gi = wave_sum(gi);
         const double g = sqrt(gr * gr + gi * gi);
-        if (!(g > tol * sqrt(al * be)) || al == 0.0 || be == 0.0) continue;
+        if (al <= small2 || be <= small2 || !(g > tol * sqrt(al * be))) continue;
         const double zeta = (be - al) / (2.0 * g);
-        const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+        // tan of the rotation angle, the root of t^2 + 2 zeta t - 1 = 0 of
+        // smaller modulus; 1/(2 zeta) once zeta^2 would overflow
+        const double t = fabs(zeta) < 1.0e150
+                             ? (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta))
+                             : 0.5 / zeta;
         const double c = 1.0 / sqrt(1.0 + t * t), sn = c * t;
         const double2 w = make_double2(gr / g, -gi / g);  // e^{-i phi}
         for (int i = lane; i < m; i += 64) {
@@ -1778,19 +1796,21 @@ int small_sweep_grid(int variant, int njobs) {
 
 template <typename S>
 int launch_sweep_small_tuned(hipStream_t st, const SmallSiteJob *jobs_dev, int njobs,
-                             void *scratch, int64_t slab_elems) {
+                             void *scratch, int64_t slab_elems, const double2 *E_base,
+                             double2 *raw_base) {
   const TuneKey key{2, (int)sizeof(S), 0, ilog2(std::max(1, njobs)), ilog2(std::max<int64_t>(1, slab_elems))};
   int chosen = 0;
   tuned_launch(st, key, SMALL_GRID_VARIANTS, [&](int v) {
     chosen = v;
-    launch_sweep_small<S>(st, jobs_dev, njobs, scratch, slab_elems, small_sweep_grid(v, njobs));
+    launch_sweep_small<S>(st, jobs_dev, njobs, scratch, slab_elems, small_sweep_grid(v, njobs),
+                          E_base, raw_base);
   });
   return chosen;
 }
 template int launch_sweep_small_tuned<double>(hipStream_t, const SmallSiteJob *, int, void *,
-                                              int64_t);
+                                              int64_t, const double2 *, double2 *);
 template int launch_sweep_small_tuned<float>(hipStream_t, const SmallSiteJob *, int, void *,
-                                             int64_t);
+                                             int64_t, const double2 *, double2 *);
 
 std::string autotune_report() {
   std::lock_guard<std::mutex> g(g_tune_mu);

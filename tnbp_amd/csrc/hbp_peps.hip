@@ -727,6 +727,21 @@ struct Peps {
   std::vector<void *> Vpair;
   std::vector<size_t> Vpair_bytes;
   int pair_cache = -1;  // -1 undecided (decided once per set of sites), 0 off, 1 on
+  // Small-site pass of the fused sweep: device job tables of the sites with
+  // bonds <= SMALL_D_MAX (per colour filter: all, colour 0, colour 1) and the
+  // host copies they were made from; a table is uploaded again only when its
+  // jobs change (site tensors or message layout), so the sweeps of a BP loop
+  // reuse it (and a recorded sweep needs no upload).
+  SmallSiteJob *stab[3] = {nullptr, nullptr, nullptr};
+  size_t stab_bytes[3] = {0, 0, 0};
+  std::vector<SmallSiteJob> stab_host[3];
+  // The lazily absorbed sites that pass takes, materialised together into one
+  // buffer (site s at byte offset sabs_off[s]) on the first sweep after the
+  // sites change and reused by the later sweeps; dropped with the pair-front
+  // copies by anything that changes a site.
+  void *sabs = nullptr;
+  size_t sabs_bytes = 0;
+  std::vector<int64_t> sabs_off;
   // caller's choice (hbp_peps_set_pair_cache): -1 by free device memory, 0 never, 1 always
   int pair_cache_policy = -1;
   int mode = HBP_MODE_FUSED;
@@ -883,6 +898,8 @@ static void make_resident(Ctx *ctx, Peps *p, int s) {
   if (p->abs_site == s) p->abs_site = -1;
 }
 
+// Drops the copies derived from the sites: pair-front site tensors and the
+// small-site pass's materialised lazily absorbed sites.
 static void drop_pair_cache(Ctx *ctx, Peps *p) {
   for (size_t s = 0; s < p->Vpair.size(); ++s)
     if (p->Vpair[s]) {
@@ -891,6 +908,12 @@ static void drop_pair_cache(Ctx *ctx, Peps *p) {
       p->Vpair_bytes[s] = 0;
     }
   p->pair_cache = -1;
+  if (p->sabs) {
+    ctx->release(p->sabs, p->sabs_bytes);
+    p->sabs = nullptr;
+    p->sabs_bytes = 0;
+  }
+  p->sabs_off.clear();
 }
 
 // Replace site s's tensor (already allocated at the current shape) and drop
@@ -1294,6 +1317,102 @@ static void pair_messages_diag(Ctx *ctx, Peps *p, int s, const int32_t *sh, int 
   if (wo) ctx->release(wo, (size_t)nw * 16);
 }
 
+// Sites the small-site pass takes: degree 1..4, every bond <= SMALL_D_MAX.
+static bool small_site(const Peps *p, int s) {
+  const int z = p->z(s);
+  if (z < 1 || z > 4) return false;
+  for (int k = 0; k < z; ++k)
+    if (p->bond_dim[p->bonds[s][k]] > SMALL_D_MAX) return false;
+  return true;
+}
+
+// The absorbed tensor of every pending small site in p->sabs (one launch of
+// k_absorb_batch, the same arithmetic per element as materialize(), so the
+// pass reads bitwise what eager absorption would have stored).
+static void materialize_small_pending(Ctx *ctx, Peps *p) {
+  if (p->sabs || !p->sabs_off.empty()) return;
+  p->sabs_off.assign(p->n_sites, -1);
+  std::vector<AbsorbJob> aj;
+  size_t total = 0;
+  for (int s = 0; s < p->n_sites; ++s) {
+    if (!p->pending[s] || !small_site(p, s)) continue;
+    p->sabs_off[s] = (int64_t)total;
+    total += (p->site_size(s) * p->elem_bytes() + 255) & ~(size_t)255;
+    aj.push_back(AbsorbJob{p->V[s], (const double2 *)p->pend_O[s], nullptr, p->site_size(s),
+                           p->pend_args[s]});
+  }
+  if (aj.empty()) return;
+  p->sabs = ctx->alloc(total);
+  p->sabs_bytes = total;
+  size_t i = 0;
+  for (int s = 0; s < p->n_sites; ++s)
+    if (p->sabs_off[s] >= 0) aj[i++].Vn = (char *)p->sabs + p->sabs_off[s];
+  absorb_batch(ctx, p, aj);
+}
+
+// The sites of a fused sweep whose bonds are all <= SMALL_D_MAX (degree
+// 1..4; `only` / `colour`: the checkerboard's colour filter) in one launch of
+// the batched small-site kernel (hbp_batched.hip: one workgroup per site, FP64
+// sums in a fixed order) instead of a dozen launches each; they are marked in
+// `done`.  Lazily absorbed sites are read from their materialised copies, so
+// lazy and eager absorption give bitwise the same messages.
+// HBP_SWEEP_PER_SITE keeps every site on its own launches.
+template <typename S>
+static void small_sites_sweep(Ctx *ctx, Peps *p, double2 *raw, const std::vector<int64_t> &raw_off,
+                              const std::vector<char> *only, char colour, std::vector<char> *done) {
+  if (p->sweep_kernel == HBP_SWEEP_PER_SITE) return;
+  materialize_small_pending(ctx, p);
+  std::vector<SmallSiteJob> jobs;
+  std::vector<int> sites;
+  int64_t slab = 1;
+  for (int s = 0; s < p->n_sites; ++s) {
+    if (only && (*only)[s] != colour) continue;
+    if (!small_site(p, s)) continue;
+    const int z = p->z(s);
+    SmallSiteJob j{};
+    j.V = p->V[s];
+    if (p->pending[s]) {
+      HBP_REQUIRE(p->sabs && p->sabs_off[s] >= 0, HBP_ERR_LOGIC,
+                  "small-site sweep: pending site not materialised");
+      j.V = (char *)p->sabs + p->sabs_off[s];
+    }
+    j.z = z;
+    for (int k = 0; k < z; ++k) {
+      j.dims[k] = p->bond_dim[p->bonds[s][k]];
+      j.in_off[k] = p->E_off[p->in_msg(s, k)];
+      j.out_off[k] = raw_off[p->out_msg(s, k)];
+    }
+    j.dims[z] = p->phys[s];
+    jobs.push_back(j);
+    sites.push_back(s);
+    slab = std::max<int64_t>(slab, (int64_t)p->site_size(s));
+  }
+  if (jobs.empty()) return;
+  const int slot = only ? 1 + colour : 0;
+  const size_t bytes = jobs.size() * sizeof(SmallSiteJob);
+  std::vector<SmallSiteJob> &host = p->stab_host[slot];
+  if (host.size() != jobs.size() || std::memcmp(host.data(), jobs.data(), bytes) != 0) {
+    if (p->stab[slot] && p->stab_bytes[slot] != bytes) {
+      ctx->release(p->stab[slot], p->stab_bytes[slot]);
+      p->stab[slot] = nullptr;
+    }
+    if (!p->stab[slot]) {
+      p->stab[slot] = (SmallSiteJob *)ctx->alloc(bytes);
+      p->stab_bytes[slot] = bytes;
+    }
+    ctx->upload(p->stab[slot], jobs.data(), bytes);
+    host = jobs;
+  }
+  const int nj = (int)jobs.size();
+  const size_t scratch_bytes =
+      (size_t)small_sweep_grid(SMALL_GRID_VARIANTS - 1, nj) * 2 * (size_t)slab * 2 * sizeof(S);
+  void *scratch = ctx->alloc(scratch_bytes);
+  launch_sweep_small_tuned<S>(ctx->stream, p->stab[slot], nj, scratch, slab, p->E, raw);
+  ctx->release(scratch, scratch_bytes);
+  done->assign(p->n_sites, 0);
+  for (int s : sites) (*done)[s] = 1;
+}
+
 template <typename S>
 static void sweep_raw(Ctx *ctx, Peps *p, double2 *raw, const std::vector<int64_t> &raw_off,
                       const std::vector<char> *only = nullptr, char colour = 0) {
@@ -1308,9 +1427,12 @@ static void sweep_raw(Ctx *ctx, Peps *p, double2 *raw, const std::vector<int64_t
   }();
   Reducer reducer(ctx);
   Reducer *red = batch ? &reducer : nullptr;
+  std::vector<char> small;
+  small_sites_sweep<S>(ctx, p, raw, raw_off, only, colour, &small);
   int32_t sh[HBP_MAX_ORDER];
   for (int s = 0; s < p->n_sites; ++s) {
     if (only && (*only)[s] != colour) continue;
+    if (!small.empty() && small[s]) continue;
     const int z = p->z(s), order = z + 1;
     p->shape(s, sh);
     if (diag_env(p, s, -1)) {
@@ -2386,6 +2508,8 @@ hbp_status hbp_peps_destroy(hbp_ctx *ctx, hbp_peps *p) {
   }
   if (p->abs_buf) ctx->release(p->abs_buf, p->abs_bytes);
   drop_pair_cache(ctx, p);
+  for (int i = 0; i < 3; ++i)
+    if (p->stab[i]) ctx->release(p->stab[i], p->stab_bytes[i]);
   ctx->release(p->E, p->E_bytes);
   for (int i = 0; i < 3; ++i)
     if (p->buf[i]) ctx->release(p->buf[i], p->buf_bytes);
