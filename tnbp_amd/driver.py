@@ -30,15 +30,19 @@ def op2(u):
 def run_kicked(ctx, L, num_steps, Jz, hz, hx, dt, max_bp_iterations, bp_tolerance,
                max_bond_dim, sv_min, lattice="square", two_site_keep_zero=False,
                precision=HBP_PREC_FP64, measure="every", observables=("x", "z", "zz"),
-               log=None, mode=HBP_MODE_FUSED):
+               log=None, mode=HBP_MODE_FUSED, schedule=0, accel=0):
     """Kicked-Ising Floquet evolution; returns per-step records like the oracle's.
     ``mode`` HBP_MODE_PLAIN runs the reference's op sequence one TCI-level
-    call at a time (bp.h:128-175, :290-350) instead of the fused kernels."""
+    call at a time (bp.h:128-175, :290-350) instead of the fused kernels;
+    ``schedule`` 1 the checkerboard BP schedule, ``accel`` > 0 Anderson
+    mixing of that depth (defaults: the reference's Jacobi loop)."""
     edges = circuits.square_lattice(L, L) if lattice == "square" else circuits.chain(L)
     sites = sorted({s for e in edges for s in e})
     tpo = circuits.kicked_ising_tpo(edges, Jz, hz, hx, dt * math.pi, two_site_keep_zero)
     peps = InitTensorProductState(ctx, edges, [2] * len(sites), precision)
     peps.set_mode(mode)
+    peps.set_schedule(schedule)
+    peps.set_accel(accel)
     steps = []
     try:
         for step in range(num_steps):
