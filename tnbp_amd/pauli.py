@@ -16,6 +16,7 @@ sum_i c_i <P_i> is returned beside the per-term values.
 """
 from __future__ import annotations
 
+import re
 from collections import deque
 
 import numpy as np
@@ -70,10 +71,13 @@ def load_sparse_pauli(path):
         return parse_sparse_pauli(f.read())
 
 
+_NON_IDENTITY = re.compile("[^I]")
+
+
 def local_ops(label):
     """Non-identity (qubit, op) pairs, ascending qubit, qubit 0 = last character."""
     n = len(label)
-    return [(i, label[n - 1 - i]) for i in range(n) if label[n - 1 - i] != "I"]
+    return [(n - 1 - m.start(), m.group()) for m in _NON_IDENTITY.finditer(label)][::-1]
 
 
 def _adjacency(edges):
@@ -105,10 +109,12 @@ def _shortest(adj, src, dst):
     return path[::-1]
 
 
-def support_path(edges, qubits):
+def support_path(edges, qubits, adj=None):
     """A simple lattice path through the given sites (ascending order), joined
-    by shortest paths; raises ValueError if it would revisit a site."""
-    adj = _adjacency(edges)
+    by shortest paths; raises ValueError if it would revisit a site.  ``adj``:
+    the lattice's adjacency (``_adjacency(edges)``) when the caller has it."""
+    if adj is None:
+        adj = _adjacency(edges)
     qs = sorted(qubits)
     path = [qs[0]]
     for q in qs[1:]:
@@ -119,25 +125,28 @@ def support_path(edges, qubits):
     return path
 
 
-def term_path_ops(edges, label):
+def term_path_ops(edges, label, adj=None):
     """(path, operators) of one Pauli term; empty for the identity."""
     ops = dict(local_ops(label))
     if not ops:
         return [], []
-    path = support_path(edges, list(ops))
+    path = support_path(edges, list(ops), adj)
     return path, [op_tensor(PAULI[ops.get(s, "I")]) for s in path]
 
 
-def estimate(ctx, peps, edges, terms):
-    """(sum_i c_i <P_i>, [<P_i>]) for terms [(c_i, label_i)] on the PEPS.
+def classify(edges, terms):
+    """How ``estimate`` measures each term (host work only, so a server can do
+    it when a job arrives): ((indices, sites, operators) of the weight-1
+    terms, (indices, site pairs, operators) of the weight-2 terms on lattice
+    edges, (indices, paths, operator lists) of the others).
 
     As ClassifyOps does (paulitotpo.h:98-153), weight-1 terms go to the site
     measurement and weight-2 terms on a lattice edge to the edge measurement
     (operator index order (a_in, b_in, a_out, b_out), qubits ascending,
     paulitotpo.h:62-92); every other term is measured on its support path."""
     edge_set = {frozenset(e) for e in edges}
+    adj = _adjacency(edges)
     one, two, paths = ([], [], []), ([], [], []), ([], [], [])
-    vals = np.ones(len(terms), dtype=np.complex128)
     for i, (_, label) in enumerate(terms):
         lops = local_ops(label)
         if len(lops) == 1:
@@ -148,8 +157,16 @@ def estimate(ctx, peps, edges, terms):
             u = np.kron(PAULI[ob], PAULI[oa])  # qubit qa is the low bit
             two[0].append(i), two[1].append((qa, qb)), two[2].append(circuits.op_tensor(u))
         elif lops:
-            path, o = term_path_ops(edges, label)
+            path, o = term_path_ops(edges, label, adj)
             paths[0].append(i), paths[1].append(path), paths[2].append(o)
+    return one, two, paths
+
+
+def estimate(ctx, peps, edges, terms, plan=None):
+    """(sum_i c_i <P_i>, [<P_i>]) for terms [(c_i, label_i)] on the PEPS;
+    ``plan``: ``classify(edges, terms)`` when the caller already has it."""
+    one, two, paths = plan if plan is not None else classify(edges, terms)
+    vals = np.ones(len(terms), dtype=np.complex128)
     if one[0]:
         vals[one[0]] = Measure(ctx, peps, one[1], one[2])
     if two[0]:

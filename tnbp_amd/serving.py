@@ -57,6 +57,8 @@ class EstimatorJob:
     name: str = ""
     arrival: float = field(default=0.0, repr=False)
     bypassed: int = field(default=0, repr=False)
+    # pauli.classify(edges, terms), made by submit() on the caller's thread
+    plan: object = field(default=None, repr=False)
 
 
 def _incident(edges):
@@ -182,7 +184,7 @@ def run_job(ctx: Context, job: EstimatorJob):
             it, _ = bp_loop(ctx, peps, job.max_bp_iterations, job.bp_tolerance)
             iters.append(it)
             Truncation(ctx, peps, job.chi, job.sv_min)
-        total, vals = pauli.estimate(ctx, peps, job.edges, job.terms)
+        total, vals = pauli.estimate(ctx, peps, job.edges, job.terms, job.plan)
     finally:
         peps.close()
     return {"value": total, "values": vals, "bp_iterations": iters, "n_values": len(vals)}
@@ -216,12 +218,20 @@ class Server:
 
     # ---------------------------------------------------------------- API
     def submit(self, job: EstimatorJob) -> Future:
+        arrival = time.perf_counter()
         fp = footprint_bytes(job)
         if fp > self.budget:
             raise MemoryError(f"job {job.name!r} needs {fp / 2**30:.1f} GiB, budget "
                               f"{self.budget / 2**30:.1f} GiB")
+        # the observable's host work here, off the device-feeding threads (a
+        # term that cannot be measured raises again there, into the future)
+        if job.plan is None:
+            try:
+                job.plan = pauli.classify(job.edges, job.terms)
+            except Exception:
+                job.plan = None
         fut = Future()
-        job.arrival = time.perf_counter()
+        job.arrival = arrival
         with self._cv:
             if self._closing:
                 raise RuntimeError("server is closed")
@@ -504,7 +514,7 @@ class BatchedServer:
     def _retire(self, ctx, r, concurrent):
         job = r["job"]
         try:
-            total, vals = pauli.estimate(ctx, r["peps"], job.edges, job.terms)
+            total, vals = pauli.estimate(ctx, r["peps"], job.edges, job.terms, job.plan)
         except BaseException as e:
             r["peps"].close()
             self._finish_fp(r["fp"])
@@ -538,7 +548,7 @@ def run_job_batched(ctx, job: EstimatorJob):
             (it, _, _, _), = bp_loop_batched(ctx, [peps], job.max_bp_iterations, job.bp_tolerance)
             iters.append(it)
             Truncation(ctx, peps, job.chi, job.sv_min)
-        total, vals = pauli.estimate(ctx, peps, job.edges, job.terms)
+        total, vals = pauli.estimate(ctx, peps, job.edges, job.terms, job.plan)
     finally:
         peps.close()
     return {"value": total, "values": vals, "bp_iterations": iters, "n_values": len(vals)}
