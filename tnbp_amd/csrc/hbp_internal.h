@@ -59,6 +59,18 @@ struct Ctx {
   size_t arena_bytes = 0;
   // pinned host staging for small results
   double *host_small = nullptr;
+  // Pinned staging of uploads: kStageSlots slots of kStageSlotBytes used in
+  // turn.  An upload that fits a slot is copied there and its DMA ordered on
+  // the stream, with the slot's event recorded after it; the host waits only
+  // when the slot's previous copy (kStageSlots uploads back) is still in
+  // flight, instead of for the whole stream.  Larger uploads wait for the
+  // stream as before.
+  static constexpr int kStageSlots = 64;
+  static constexpr size_t kStageSlotBytes = 256 << 10;
+  char *stage = nullptr;
+  hipEvent_t stage_ev[kStageSlots] = {};
+  bool stage_busy[kStageSlots] = {};
+  int stage_next = 0;
 
   void *alloc(size_t bytes);
   void release(void *p, size_t bytes);

@@ -1465,26 +1465,46 @@ __device__ inline double2 cmul(double2 a, double2 b) {
 }
 __device__ inline double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
 
-// One block of 256 threads per matrix.  Hestenes one-sided Jacobi: pairs of
-// columns (p, q) are rotated until mutually orthogonal; the rotations are
-// accumulated in V so that A V = [sigma_j u_j].  Round-robin pairing puts
-// n'/2 disjoint pairs in each round; each wave takes pairs w, w+4, ...
+// One block of SVD_THREADS threads per matrix.  Hestenes one-sided Jacobi:
+// pairs of columns (p, q) are rotated until mutually orthogonal; the
+// rotations are accumulated in V so that A V = [sigma_j u_j].  Round-robin
+// pairing puts n'/2 disjoint pairs in each round.  A pair is taken by a group
+// of G lanes (G from the matrix's own row count: 4 to 6 rows per lane), so
+// every pair of a round (32 at n = 64) is rotated at once and a round costs
+// one dot-product reduction over G lanes and one rotation; a matrix's
+// arithmetic does not depend on the batch it is in.  The group sums are xor
+// butterflies, which leave bitwise the same sum in every lane of the group
+// (floating-point addition is commutative), so the whole group takes the
+// same skip decision and the same rotation.
 // IN_LDS: the working columns W (m x n) and V (n x n) live in the block's
-// LDS (m n + n^2 <= SVD_LDS_ELEMS, i.e. up to 64 x 64): every round is a
-// chain of dependent dot products and rotations, so their latency decides
-// the time; the same operations in the same order as the global-memory form,
-// so the results are bitwise those of the global form.
+// LDS (m n + n^2 <= SVD_LDS_ELEMS, i.e. up to 64 x 64); the same operations
+// in the same order as the global-memory form, so the results are bitwise
+// those of the global form.
+constexpr int SVD_THREADS = 512;
+
+__device__ inline int svd_group_lanes(int m) {
+  return m <= 48 ? 8 : m <= 96 ? 16 : m <= 192 ? 32 : 64;
+}
+
+// Sum over the aligned group of G lanes (G a power of two dividing 64).
+__device__ inline double group_sum(double v, int G) {
+  for (int o = G >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
 template <bool IN_LDS>
-__global__ __launch_bounds__(256) void k_batched_svd(const double2 *__restrict__ Ain,
-                                                      double2 *__restrict__ U,
-                                                      double *__restrict__ s,
-                                                      double2 *__restrict__ Vd,
-                                                      double2 *__restrict__ work,
-                                                      const SvdJob *__restrict__ jobs) {
+__global__ __launch_bounds__(SVD_THREADS) void k_batched_svd(const double2 *__restrict__ Ain,
+                                                              double2 *__restrict__ U,
+                                                              double *__restrict__ s,
+                                                              double2 *__restrict__ Vd,
+                                                              double2 *__restrict__ work,
+                                                              const SvdJob *__restrict__ jobs) {
   extern __shared__ double2 svd_lds[];
   const SvdJob job = jobs[blockIdx.x];
   const int m = job.m, n = job.n;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int G = svd_group_lanes(m);
+  const int grp = threadIdx.x / G, gl = threadIdx.x % G, ngrp = blockDim.x / G;
   double2 *W = IN_LDS ? svd_lds : work + job.work_off;  // m x n
   double2 *Vm = W + (int64_t)m * n;                     // n x n
   const double2 *A = Ain + job.a_off;
@@ -1499,10 +1519,12 @@ __global__ __launch_bounds__(256) void k_batched_svd(const double2 *__restrict__
     Vm[e] = make_double2(i == j ? 1.0 : 0.0, 0.0);
   }
   __shared__ int rotated;
-  __shared__ double fro_part[4];
+  __shared__ double fro_part[SVD_THREADS / 64];
   fro = wave_sum(fro);
   if (lane == 0) fro_part[wave] = fro;
   __syncthreads();
+  double fro_all = 0.0;
+  for (int w = 0; w < nw; ++w) fro_all += fro_part[w];
   const int np = (n + 1) & ~1;  // players incl. a dummy when n is odd
   const double tol = 1.0e-15;
   // A column whose squared norm is below 1e-30 ||A||_F^2 is rounding noise:
@@ -1510,28 +1532,28 @@ __global__ __launch_bounds__(256) void k_batched_svd(const double2 *__restrict__
   // and the relative orthogonality test would keep such pairs rotating to
   // the sweep limit (rank-deficient inputs such as Ra^T Rb of two roots of
   // rank-2 messages), so such pairs count as converged.
-  const double small2 = 1.0e-30 * (fro_part[0] + fro_part[1] + fro_part[2] + fro_part[3]);
+  const double small2 = 1.0e-30 * fro_all;
   for (int sweep = 0; sweep < 60; ++sweep) {
     if (threadIdx.x == 0) rotated = 0;
     __syncthreads();
     for (int round = 0; round < np - 1; ++round) {
-      for (int pi = wave; pi < np / 2; pi += nw) {
+      for (int pi = grp; pi < np / 2; pi += ngrp) {
         int a = (pi == 0) ? 0 : ((pi - 1 + round) % (np - 1)) + 1;
         int b = ((np - 2 - pi + round) % (np - 1)) + 1;
         if (a >= n || b >= n) continue;
         if (a > b) { const int t = a; a = b; b = t; }
         double al = 0.0, be = 0.0, gr = 0.0, gi = 0.0;
-        for (int i = lane; i < m; i += 64) {
+        for (int i = gl; i < m; i += G) {
           const double2 x = W[i + (int64_t)m * a], y = W[i + (int64_t)m * b];
           al += x.x * x.x + x.y * x.y;
           be += y.x * y.x + y.y * y.y;
           gr += x.x * y.x + x.y * y.y;  // conj(x) y
           gi += x.x * y.y - x.y * y.x;
         }
-        al = wave_sum(al);
-        be = wave_sum(be);
-        gr = wave_sum(gr);
-        gi = wave_sum(gi);
+        al = group_sum(al, G);
+        be = group_sum(be, G);
+        gr = group_sum(gr, G);
+        gi = group_sum(gi, G);
         const double g = sqrt(gr * gr + gi * gi);
         if (al <= small2 || be <= small2 || !(g > tol * sqrt(al * be))) continue;
         const double zeta = (be - al) / (2.0 * g);
@@ -1542,17 +1564,17 @@ __global__ __launch_bounds__(256) void k_batched_svd(const double2 *__restrict__
                              : 0.5 / zeta;
         const double c = 1.0 / sqrt(1.0 + t * t), sn = c * t;
         const double2 w = make_double2(gr / g, -gi / g);  // e^{-i phi}
-        for (int i = lane; i < m; i += 64) {
+        for (int i = gl; i < m; i += G) {
           const double2 x = W[i + (int64_t)m * a], y = cmul(w, W[i + (int64_t)m * b]);
           W[i + (int64_t)m * a] = make_double2(c * x.x - sn * y.x, c * x.y - sn * y.y);
           W[i + (int64_t)m * b] = make_double2(sn * x.x + c * y.x, sn * x.y + c * y.y);
         }
-        for (int i = lane; i < n; i += 64) {
+        for (int i = gl; i < n; i += G) {
           const double2 x = Vm[i + (int64_t)n * a], y = cmul(w, Vm[i + (int64_t)n * b]);
           Vm[i + (int64_t)n * a] = make_double2(c * x.x - sn * y.x, c * x.y - sn * y.y);
           Vm[i + (int64_t)n * b] = make_double2(sn * x.x + c * y.x, sn * x.y + c * y.y);
         }
-        if (lane == 0) rotated = 1;
+        if (gl == 0) rotated = 1;
       }
       __syncthreads();
     }
@@ -1563,14 +1585,14 @@ __global__ __launch_bounds__(256) void k_batched_svd(const double2 *__restrict__
   // column norms, descending order (ties keep the lower index first)
   __shared__ double sig[512];
   __shared__ int ord[512];
-  for (int j = wave; j < n; j += nw) {
+  for (int j = grp; j < n; j += ngrp) {
     double al = 0.0;
-    for (int i = lane; i < m; i += 64) {
+    for (int i = gl; i < m; i += G) {
       const double2 x = W[i + (int64_t)m * j];
       al += x.x * x.x + x.y * x.y;
     }
-    al = wave_sum(al);
-    if (lane == 0) sig[j] = sqrt(al);
+    al = group_sum(al, G);
+    if (gl == 0) sig[j] = sqrt(al);
   }
   __syncthreads();
   for (int j = threadIdx.x; j < n; j += blockDim.x) {
@@ -1581,17 +1603,17 @@ __global__ __launch_bounds__(256) void k_batched_svd(const double2 *__restrict__
   }
   __syncthreads();
   const int k = min(m, n);
-  for (int c = wave; c < k; c += nw) {
+  for (int c = grp; c < k; c += ngrp) {
     const int j = ord[c];
     const double sv = sig[j];
     const double inv = sv > 0.0 ? 1.0 / sv : 0.0;
-    for (int i = lane; i < m; i += 64) {
+    for (int i = gl; i < m; i += G) {
       const double2 x = W[i + (int64_t)m * j];
       U[job.u_off + i + (int64_t)m * c] = make_double2(x.x * inv, x.y * inv);
     }
-    for (int i = lane; i < n; i += 64)
+    for (int i = gl; i < n; i += G)
       Vd[job.vd_off + c + (int64_t)k * i] = cconj(Vm[i + (int64_t)n * j]);
-    if (lane == 0) s[job.s_off + c] = sv;
+    if (gl == 0) s[job.s_off + c] = sv;
   }
 }
 
@@ -2268,11 +2290,11 @@ void launch_batched_svd(hipStream_t st, const double2 *A, double2 *U, double *s,
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)(SVD_LDS_ELEMS * sizeof(double2)));
     });
-    hipLaunchKernelGGL(k_batched_svd<true>, dim3(batch), dim3(256), bytes, st, A, U, s, Vd, work,
-                       jobs_dev);
+    hipLaunchKernelGGL(k_batched_svd<true>, dim3(batch), dim3(SVD_THREADS), bytes, st, A, U, s, Vd,
+                       work, jobs_dev);
   } else {
-    hipLaunchKernelGGL(k_batched_svd<false>, dim3(batch), dim3(256), 0, st, A, U, s, Vd, work,
-                       jobs_dev);
+    hipLaunchKernelGGL(k_batched_svd<false>, dim3(batch), dim3(SVD_THREADS), 0, st, A, U, s, Vd,
+                       work, jobs_dev);
   }
   HBP_HIP(hipGetLastError());
 }

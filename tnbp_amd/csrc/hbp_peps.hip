@@ -2891,7 +2891,7 @@ static void absorb_many(Ctx *ctx, const std::vector<AbsorbMember> &mem) {
       p->phys[s] = a.dout;
       if (lazy[i]) {
         double2 *dO = (double2 *)ctx->alloc(on * 16);
-        HBP_HIP(hipMemcpyAsync(dO, mem[i].ops[s], on * 16, hipMemcpyHostToDevice, ctx->stream));
+        ctx->upload(dO, mem[i].ops[s], on * 16);
         p->pending[s] = 1;
         p->pend_O[s] = dO;
         p->pend_O_bytes[s] = on * 16;

@@ -267,6 +267,23 @@ void Ctx::prof_end(int idx) {
 
 void Ctx::upload(void *dst, const void *src, size_t bytes) {
   HBP_REQUIRE(!graph_capturing(), HBP_ERR_UNSUPPORTED, "upload while a graph is recorded");
+  if (bytes == 0) return;
+  if (bytes <= kStageSlotBytes) {
+    if (!stage) {
+      HBP_HIP(hipHostMalloc((void **)&stage, kStageSlots * kStageSlotBytes, hipHostMallocDefault));
+      for (int k = 0; k < kStageSlots; ++k)
+        HBP_HIP(hipEventCreateWithFlags(&stage_ev[k], hipEventDisableTiming));
+    }
+    const int k = stage_next;
+    stage_next = (k + 1) % kStageSlots;
+    if (stage_busy[k]) HBP_HIP(hipEventSynchronize(stage_ev[k]));
+    char *h = stage + (size_t)k * kStageSlotBytes;
+    std::memcpy(h, src, bytes);
+    HBP_HIP(hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, stream));
+    HBP_HIP(hipEventRecord(stage_ev[k], stream));
+    stage_busy[k] = true;
+    return;
+  }
   HBP_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
   HBP_HIP(hipStreamSynchronize(stream));
 }
@@ -726,6 +743,11 @@ hbp_status hbp_ctx_destroy(hbp_ctx *ctx) {
   ctx->flush_blocks();
   HBP_HIP(hipStreamSynchronize(ctx->stream));
   ctx->prof_clear();
+  if (ctx->stage) {
+    for (int k = 0; k < Ctx::kStageSlots; ++k) (void)hipEventDestroy(ctx->stage_ev[k]);
+    (void)hipHostFree(ctx->stage);
+    ctx->stage = nullptr;
+  }
   // give the reserve back before the pool goes, so that a context created
   // next in this process sees the memory as free
   HBP_HIP(hipMemPoolTrimTo(ctx->pool, 0));
