@@ -158,3 +158,35 @@ def test_no_device_fails_jobs_instead_of_hanging():
     with pytest.raises(RuntimeError):
         fut.result(timeout=10)
     srv.close()
+
+
+def test_packed_tpo_layout(monkeypatch):
+    """PackedTPO (the form submit() prepares): pointer i reads site i's
+    operator in column-major order, shape row i holds its shape padded with
+    zeros; the server's job keeps the packed layers."""
+    import ctypes as C
+
+    from tnbp_amd import circuits
+    from tnbp_amd._lib import HBP_MAX_ORDER
+    from tnbp_amd.peps import PackedTPO
+    rng = np.random.default_rng(3)
+    edges = circuits.square_lattice(3, 3)
+    layer = circuits.layer_tpo(edges, circuits.random_layer(rng, edges, 9))
+    pk = PackedTPO(layer)
+    assert pk.n == len(layer)
+    for i, o in enumerate(layer):
+        a = np.asarray(o, dtype=np.complex128)
+        addr = C.cast(pk.ptrs[i], C.c_void_p).value
+        got = np.frombuffer((C.c_double * (2 * a.size)).from_address(addr), dtype=np.complex128)
+        np.testing.assert_array_equal(got, a.reshape(-1, order="F"))
+        row = list(pk.shapes[i * HBP_MAX_ORDER:(i + 1) * HBP_MAX_ORDER])
+        assert row[:a.ndim] == list(a.shape) and not any(row[a.ndim:])
+    job = serving.EstimatorJob(edges=edges, tpos=[layer, layer], chi=4, terms=[], name="p")
+    monkeypatch.setattr(serving, "footprint_bytes", lambda j: 0)
+    srv = serving.Server.__new__(serving.Server)
+    srv.budget, srv._closing, srv._pending = 1 << 62, False, []
+    srv._cv = threading.Condition()
+    srv.slots = 1
+    serving.Server.submit(srv, job)
+    assert len(job.packed) == 2 and all(isinstance(p, PackedTPO) for p in job.packed)
+    assert serving._layers(job) is job.packed

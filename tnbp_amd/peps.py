@@ -196,20 +196,48 @@ def InitTensorProductState(ctx, edges, pdim=None, precision=HBP_PREC_FP64) -> Pe
     return Peps(ctx, edges, pdim, precision)
 
 
+class PackedTPO:
+    """A TPO layer in the C-ABI's form, built once: every site's operator
+    (column-major complex128) in one contiguous buffer, the array of pointers
+    to them and the flat shape array (n x HBP_MAX_ORDER).  ``AbsorbTPO`` and
+    ``AbsorbTPOBatched`` take it in place of the operator list, so a layer
+    absorbed again (or by a server, packed when the job arrives) costs no
+    per-site host work."""
+
+    __slots__ = ("buf", "ptrs", "shapes", "n")
+
+    def __init__(self, O):
+        flats = [np.asarray(o, dtype=np.complex128).ravel(order="F") for o in O]
+        n = len(flats)
+        self.n = n
+        sizes = np.fromiter((f.size for f in flats), dtype=np.int64, count=n)
+        self.buf = np.concatenate(flats) if n else np.zeros(1, dtype=np.complex128)
+        offs = np.zeros(n, dtype=np.int64)
+        if n > 1:
+            np.cumsum(sizes[:-1], out=offs[1:])
+        addr = (self.buf.ctypes.data + 16 * offs).astype(np.uint64)
+        self.ptrs = (C.POINTER(C.c_double) * max(n, 1))()
+        C.memmove(self.ptrs, addr.ctypes.data, n * C.sizeof(C.c_void_p))
+        shp = np.zeros((max(n, 1), HBP_MAX_ORDER), dtype=np.int32)
+        for i, o in enumerate(O):
+            nd = np.ndim(o)
+            if nd > HBP_MAX_ORDER:
+                raise ValueError("TPO operator of too high an order")
+            shp[i, :nd] = np.shape(o)
+        self.shapes = (C.c_int32 * shp.size).from_buffer_copy(shp.tobytes())
+
+
 def _tpo_args(O):
-    """(keep-alive arrays, pointer array, flat shapes) of a TPO for the C-ABI."""
-    arrs = [np.ascontiguousarray(np.asarray(o, dtype=np.complex128).reshape(-1, order="F"))
-            for o in O]
-    ptrs = (C.POINTER(C.c_double) * len(arrs))(*[a.ctypes.data_as(C.POINTER(C.c_double)) for a in arrs])
-    shapes = []
-    for o in O:
-        shapes += list(np.shape(o)) + [0] * (HBP_MAX_ORDER - np.ndim(o))
-    return arrs, ptrs, i32(shapes)
+    """(keep-alive object, pointer array, flat shapes) of a TPO for the C-ABI;
+    ``O`` an operator list or a ``PackedTPO``."""
+    pk = O if isinstance(O, PackedTPO) else PackedTPO(O)
+    return pk, pk.ptrs, pk.shapes
 
 
 def AbsorbTPO(ctx, O, peps: Peps) -> None:
     """``AbsorbTPO`` (apply.h:14-177); ``O[s]`` is site s's operator
-    (t_1..t_z, d_in, d_out) as a numpy complex array."""
+    (t_1..t_z, d_in, d_out) as a numpy complex array (or ``O`` a
+    ``PackedTPO`` of such a list)."""
     _, ptrs, shapes = _tpo_args(O)
     check(ctx.lib.hbp_peps_absorb_tpo(ctx.h, peps.h, ptrs, shapes))
 

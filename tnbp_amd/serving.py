@@ -36,7 +36,7 @@ import numpy as np
 from . import circuits, pauli
 from ._lib import (HBP_ABSORB_EAGER, HBP_ABSORB_LAZY, HBP_BATCH_GRAPH, HBP_BATCH_SMALL, HBP_PREC_FP64,
                    HBP_PREC_MIXED)
-from .peps import (AbsorbTPO, AbsorbTPOBatched, InitTensorProductState, Truncation,
+from .peps import (AbsorbTPO, AbsorbTPOBatched, InitTensorProductState, PackedTPO, Truncation,
                    TruncationBatched, bp_loop)
 from .tci import Context, device_memory
 
@@ -57,8 +57,16 @@ class EstimatorJob:
     name: str = ""
     arrival: float = field(default=0.0, repr=False)
     bypassed: int = field(default=0, repr=False)
-    # pauli.classify(edges, terms), made by submit() on the caller's thread
+    # pauli.classify(edges, terms) and the layers as PackedTPOs, made by
+    # submit() on the caller's thread
     plan: object = field(default=None, repr=False)
+    packed: list = field(default=None, repr=False)
+
+
+def _layers(job: EstimatorJob):
+    """The job's circuit layers in the form AbsorbTPO takes: packed when
+    submit() packed them."""
+    return job.packed if job.packed is not None else job.tpos
 
 
 def _incident(edges):
@@ -179,7 +187,7 @@ def run_job(ctx: Context, job: EstimatorJob):
     peps.set_pair_cache(1 if pair_cache_bytes(job) > 0 else 0)
     iters = []
     try:
-        for tpo in job.tpos:
+        for tpo in _layers(job):
             AbsorbTPO(ctx, tpo, peps)
             it, _ = bp_loop(ctx, peps, job.max_bp_iterations, job.bp_tolerance)
             iters.append(it)
@@ -230,6 +238,11 @@ class Server:
                 job.plan = pauli.classify(job.edges, job.terms)
             except Exception:
                 job.plan = None
+        if job.packed is None:
+            try:
+                job.packed = [PackedTPO(O) for O in job.tpos]
+            except Exception:
+                job.packed = None
         fut = Future()
         job.arrival = arrival
         with self._cv:
@@ -453,7 +466,7 @@ class BatchedServer:
                 if not active:
                     continue
                 t0 = time.perf_counter()
-                AbsorbTPOBatched(ctx, [r["job"].tpos[r["layer"]] for r in active],
+                AbsorbTPOBatched(ctx, [_layers(r["job"])[r["layer"]] for r in active],
                                  [r["peps"] for r in active])
                 t1 = time.perf_counter()
                 groups = {}
@@ -543,7 +556,7 @@ def run_job_batched(ctx, job: EstimatorJob):
     peps.set_pair_cache(1 if pair_cache_bytes(job) > 0 else 0)
     iters = []
     try:
-        for tpo in job.tpos:
+        for tpo in _layers(job):
             AbsorbTPO(ctx, tpo, peps)
             (it, _, _, _), = bp_loop_batched(ctx, [peps], job.max_bp_iterations, job.bp_tolerance)
             iters.append(it)
