@@ -106,11 +106,17 @@ __global__ __launch_bounds__(256) void k_sweep_small(const SmallSiteJob *__restr
   __shared__ double2 red[256];
   C *bufA = (C *)scratch + (int64_t)blockIdx.x * 2 * slab_elems;
   C *bufB = bufA + slab_elems;
-  for (int w = blockIdx.x; w < njobs; w += gridDim.x) {
-    const SmallSiteJob job = jobs[w];
+  // work item w = (site w / SMALL_SWEEP_ITEMS, outgoing message w % ...):
+  // each message is computed on its own from V (no shared partial
+  // products), so the split only spreads a site over more workgroups
+  const int nitems = SMALL_SWEEP_ITEMS * njobs;
+  for (int w = blockIdx.x; w < nitems; w += gridDim.x) {
+    const SmallSiteJob job = jobs[w / SMALL_SWEEP_ITEMS];
     const int z = job.z;
+    const int k = w % SMALL_SWEEP_ITEMS;
+    if (k >= z) continue;
     const C *V = (const C *)job.V;
-    for (int k = 0; k < z; ++k) {
+    {
       const C *cur = V;
       C *nxt = bufA;
       for (int l = 0; l < z; ++l) {

@@ -323,6 +323,8 @@ void launch_chain_small(hipStream_t st, const SmallChainSite *sites, const Small
 // (variant v), at most one per work item; scratch for the largest.
 constexpr int SMALL_GRID_VARIANTS = 3;
 int small_sweep_grid(int variant, int njobs);
+// k_sweep_small's work items per site: one per outgoing message (degree <= 4)
+constexpr int SMALL_SWEEP_ITEMS = 4;
 // The variant for (precision, njobs class, slab class): timed on first use by
 // running every variant on the real work (each computes the same messages;
 // the chosen one runs last), remembered for the process.  Must not run while

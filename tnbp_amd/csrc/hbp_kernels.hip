@@ -1824,8 +1824,8 @@ int launch_sweep_small_tuned(hipStream_t st, const SmallSiteJob *jobs_dev, int n
   int chosen = 0;
   tuned_launch(st, key, SMALL_GRID_VARIANTS, [&](int v) {
     chosen = v;
-    launch_sweep_small<S>(st, jobs_dev, njobs, scratch, slab_elems, small_sweep_grid(v, njobs),
-                          E_base, raw_base);
+    launch_sweep_small<S>(st, jobs_dev, njobs, scratch, slab_elems,
+                          small_sweep_grid(v, SMALL_SWEEP_ITEMS * njobs), E_base, raw_base);
   });
   return chosen;
 }

@@ -1405,7 +1405,8 @@ static void small_sites_sweep(Ctx *ctx, Peps *p, double2 *raw, const std::vector
   }
   const int nj = (int)jobs.size();
   const size_t scratch_bytes =
-      (size_t)small_sweep_grid(SMALL_GRID_VARIANTS - 1, nj) * 2 * (size_t)slab * 2 * sizeof(S);
+      (size_t)small_sweep_grid(SMALL_GRID_VARIANTS - 1, SMALL_SWEEP_ITEMS * nj) * 2 *
+      (size_t)slab * 2 * sizeof(S);
   void *scratch = ctx->alloc(scratch_bytes);
   launch_sweep_small_tuned<S>(ctx->stream, p->stab[slot], nj, scratch, slab, p->E, raw);
   ctx->release(scratch, scratch_bytes);
@@ -3419,7 +3420,7 @@ template <typename S>
 void launch_batch_sweep(Ctx *ctx, const BatchTables &t, void *scratch, int64_t slab, int variant,
                         double *dist, double *res, bool residual) {
   launch_sweep_small<S>(ctx->stream, t.sites, t.nsites, scratch, slab,
-                        small_sweep_grid(variant, t.nsites));
+                        small_sweep_grid(variant, SMALL_SWEEP_ITEMS * t.nsites));
   hipLaunchKernelGGL(k_finalize_batched, dim3((unsigned)t.nmsgs), dim3(256), 0, ctx->stream,
                      t.msgs, dist);
   HBP_HIP(hipGetLastError());
@@ -3509,7 +3510,7 @@ static void bp_run_batched_impl(Ctx *ctx, std::vector<BatchMember> &mem, int max
       nsites += b->p->n_sites;
       nmsgs += 2 * b->p->n_edges;
     }
-    const int grid_max = small_sweep_grid(SMALL_GRID_VARIANTS - 1, nsites);
+    const int grid_max = small_sweep_grid(SMALL_GRID_VARIANTS - 1, SMALL_SWEEP_ITEMS * nsites);
     g.scratch_bytes = (size_t)grid_max * 2 * (size_t)g.slab * 2 * (g.fp32 ? sizeof(float) : sizeof(double));
     g.scratch = ctx->alloc(g.scratch_bytes);
     g.dist_bytes = (size_t)std::max(nmsgs, 1) * sizeof(double);
