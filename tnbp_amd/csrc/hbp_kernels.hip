@@ -1469,10 +1469,10 @@ __device__ inline double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
 // pairs of columns (p, q) are rotated until mutually orthogonal; the
 // rotations are accumulated in V so that A V = [sigma_j u_j].  Round-robin
 // pairing puts n'/2 disjoint pairs in each round.  A pair is taken by a group
-// of G lanes (G from the matrix's own row count: 4 to 6 rows per lane), so
-// every pair of a round (32 at n = 64) is rotated at once and a round costs
-// one dot-product reduction over G lanes and one rotation; a matrix's
-// arithmetic does not depend on the batch it is in.  The group sums are xor
+// of G lanes (G from the matrix's own row count, svd_group_lanes), so every pair
+// of a round (32 at n = 64) is rotated at once and a round costs one
+// dot-product reduction over G lanes and one rotation; a matrix's arithmetic
+// does not depend on the batch it is in.  The group sums are xor
 // butterflies, which leave bitwise the same sum in every lane of the group
 // (floating-point addition is commutative), so the whole group takes the
 // same skip decision and the same rotation.
@@ -1482,6 +1482,10 @@ __device__ inline double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
 // those of the global form.
 constexpr int SVD_THREADS = 512;
 
+// Lanes per column pair: about 4 to 6 rows of the m-row columns per lane.
+// Fewer lanes per pair (more rows each) leave one wave per SIMD with the
+// latency of its reduction and rotation chain exposed; measured slower at
+// 64 and 96 rows.  It depends on the matrix alone, not on its batch.
 __device__ inline int svd_group_lanes(int m) {
   return m <= 48 ? 8 : m <= 96 ? 16 : m <= 192 ? 32 : 64;
 }
@@ -1490,6 +1494,20 @@ __device__ inline int svd_group_lanes(int m) {
 __device__ inline double group_sum(double v, int G) {
   for (int o = G >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o);
   return v;
+}
+
+// Four group sums at once: every step issues the four exchanges before
+// waiting, so the sums cost one chain of log2(G) exchanges instead of four;
+// each value's additions are those of group_sum, in the same order.
+__device__ inline void group_sum4(double &a, double &b, double &c, double &d, int G) {
+  for (int o = G >> 1; o > 0; o >>= 1) {
+    const double ta = __shfl_xor(a, o), tb = __shfl_xor(b, o);
+    const double tc = __shfl_xor(c, o), td = __shfl_xor(d, o);
+    a += ta;
+    b += tb;
+    c += tc;
+    d += td;
+  }
 }
 
 template <bool IN_LDS>
@@ -1550,20 +1568,32 @@ __global__ __launch_bounds__(SVD_THREADS) void k_batched_svd(const double2 *__re
           gr += x.x * y.x + x.y * y.y;  // conj(x) y
           gi += x.x * y.y - x.y * y.x;
         }
-        al = group_sum(al, G);
-        be = group_sum(be, G);
-        gr = group_sum(gr, G);
-        gi = group_sum(gi, G);
-        const double g = sqrt(gr * gr + gi * gi);
+        group_sum4(al, be, gr, gi, G);
+        const double g2 = gr * gr + gi * gi, g = sqrt(g2);
         if (al <= small2 || be <= small2 || !(g > tol * sqrt(al * be))) continue;
-        const double zeta = (be - al) / (2.0 * g);
-        // tan of the rotation angle, the root of t^2 + 2 zeta t - 1 = 0 of
-        // smaller modulus; 1/(2 zeta) once zeta^2 would overflow
-        const double t = fabs(zeta) < 1.0e150
-                             ? (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta))
-                             : 0.5 / zeta;
-        const double c = 1.0 / sqrt(1.0 + t * t), sn = c * t;
-        const double2 w = make_double2(gr / g, -gi / g);  // e^{-i phi}
+        // The rotation zeroing the pair's inner product: tan = the root of
+        // t^2 + 2 zeta t - 1 = 0 of smaller modulus, zeta = (be - al) / 2g.
+        // With d = be - al, r = sqrt(d^2 + 4 g^2) and q = |d| + r:
+        // c = q / sqrt(q^2 + 4 g^2), s = sign(d) 2g / sqrt(q^2 + 4 g^2)
+        // (two square roots and one division in a row); where the squares
+        // would overflow, the zeta form.
+        const double d = be - al, sgn = d >= 0.0 ? 1.0 : -1.0;
+        const double q = fabs(d) + sqrt(d * d + 4.0 * g2);
+        const double den = sqrt(q * q + 4.0 * g2);
+        double c, sn;
+        if (den < 1.0e300) {
+          const double inv = 1.0 / den;
+          c = q * inv;
+          sn = sgn * 2.0 * g * inv;
+        } else {
+          const double zeta = d / (2.0 * g);
+          const double t = fabs(zeta) < 1.0e150 ? sgn / (fabs(zeta) + sqrt(1.0 + zeta * zeta))
+                                                : 0.5 / zeta;
+          c = 1.0 / sqrt(1.0 + t * t);
+          sn = c * t;
+        }
+        const double ig = 1.0 / g;
+        const double2 w = make_double2(gr * ig, -gi * ig);  // e^{-i phi}
         for (int i = gl; i < m; i += G) {
           const double2 x = W[i + (int64_t)m * a], y = cmul(w, W[i + (int64_t)m * b]);
           W[i + (int64_t)m * a] = make_double2(c * x.x - sn * y.x, c * x.y - sn * y.y);
