@@ -1476,10 +1476,14 @@ __device__ inline double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
 // butterflies, which leave bitwise the same sum in every lane of the group
 // (floating-point addition is commutative), so the whole group takes the
 // same skip decision and the same rotation.
-// IN_LDS: the working columns W (m x n) and V (n x n) live in the block's
-// LDS (m n + n^2 <= SVD_LDS_ELEMS, i.e. up to 64 x 64); the same operations
-// in the same order as the global-memory form, so the results are bitwise
-// those of the global form.
+// FORM: where the working columns W (m x n) and V (n x n) live.
+// SVD_FORM_LDS: both in the block's LDS (m n + n^2 <= SVD_LDS_ELEMS, up to
+// 64 x 64); SVD_FORM_W_LDS: W in LDS and V in the job's work area (m n up to
+// the device's LDS per workgroup less the static LDS: 96 x 96 on 160 KiB),
+// so the dot products and W's rotation never wait on global memory and V's
+// loads can be issued early; SVD_FORM_GLOBAL: both in the work area.  The
+// same operations in the same order in every form, so the results are
+// bitwise those of the global form.
 constexpr int SVD_THREADS = 512;
 
 // Lanes per column pair: about 4 to 6 rows of the m-row columns per lane.
@@ -1510,7 +1514,9 @@ __device__ inline void group_sum4(double &a, double &b, double &c, double &d, in
   }
 }
 
-template <bool IN_LDS>
+constexpr int SVD_FORM_GLOBAL = 0, SVD_FORM_LDS = 1, SVD_FORM_W_LDS = 2;
+
+template <int FORM>
 __global__ __launch_bounds__(SVD_THREADS) void k_batched_svd(const double2 *__restrict__ Ain,
                                                               double2 *__restrict__ U,
                                                               double *__restrict__ s,
@@ -1523,8 +1529,9 @@ __global__ __launch_bounds__(SVD_THREADS) void k_batched_svd(const double2 *__re
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   const int G = svd_group_lanes(m);
   const int grp = threadIdx.x / G, gl = threadIdx.x % G, ngrp = blockDim.x / G;
-  double2 *W = IN_LDS ? svd_lds : work + job.work_off;  // m x n
-  double2 *Vm = W + (int64_t)m * n;                     // n x n
+  double2 *W = FORM != SVD_FORM_GLOBAL ? svd_lds : work + job.work_off;  // m x n
+  double2 *Vm = FORM == SVD_FORM_LDS ? W + (int64_t)m * n                 // n x n
+                                     : work + job.work_off + (int64_t)m * n;
   const double2 *A = Ain + job.a_off;
   double fro = 0.0;
   for (int64_t e = threadIdx.x; e < (int64_t)m * n; e += blockDim.x) {
@@ -2307,24 +2314,57 @@ void launch_trace(hipStream_t st, const double2 *in, double2 *out, int order,
   HBP_HIP(hipGetLastError());
 }
 
+// Complex128 elements of W the SVD_FORM_W_LDS launch can hold on the current
+// device (0: that form is not used): the device's LDS per workgroup less the
+// kernel's static LDS, once the kernel's dynamic-LDS limit has been raised.
+static int64_t svd_w_lds_cap() {
+  static std::once_flag once;
+  static int64_t cap = 0;
+  std::call_once(once, [] {
+    int dev = 0, lds = 0;
+    hipFuncAttributes fa{};
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+            hipSuccess ||
+        hipFuncGetAttributes(&fa, (const void *)k_batched_svd<SVD_FORM_W_LDS>) != hipSuccess) {
+      (void)hipGetLastError();
+      return;
+    }
+    const int64_t bytes = ((int64_t)lds - (int64_t)fa.sharedSizeBytes) / 16 * 16;
+    if (bytes <= (int64_t)SVD_LDS_ELEMS * 16) return;
+    if (hipFuncSetAttribute((const void *)k_batched_svd<SVD_FORM_W_LDS>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) !=
+        hipSuccess) {
+      (void)hipGetLastError();
+      return;
+    }
+    cap = bytes / 16;
+  });
+  return cap;
+}
+
 void launch_batched_svd(hipStream_t st, const double2 *A, double2 *U, double *s, double2 *Vd,
                         double2 *work, const SvdJob *jobs_dev, int batch, int max_n, int max_m) {
   HBP_REQUIRE(max_n <= 512, HBP_ERR_UNSUPPORTED, "svd: more than 512 columns");
   if (batch <= 0) return;
-  const int64_t elems = (int64_t)max_m * max_n + (int64_t)max_n * max_n;
+  const int64_t w_elems = (int64_t)max_m * max_n;
+  const int64_t elems = w_elems + (int64_t)max_n * max_n;
   if (max_m > 0 && elems <= SVD_LDS_ELEMS) {
     const unsigned bytes = (unsigned)(elems * sizeof(double2));
     static std::once_flag once;
     std::call_once(once, [] {
-      (void)hipFuncSetAttribute((const void *)k_batched_svd<true>,
+      (void)hipFuncSetAttribute((const void *)k_batched_svd<SVD_FORM_LDS>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)(SVD_LDS_ELEMS * sizeof(double2)));
     });
-    hipLaunchKernelGGL(k_batched_svd<true>, dim3(batch), dim3(SVD_THREADS), bytes, st, A, U, s, Vd,
-                       work, jobs_dev);
+    hipLaunchKernelGGL(k_batched_svd<SVD_FORM_LDS>, dim3(batch), dim3(SVD_THREADS), bytes, st, A,
+                       U, s, Vd, work, jobs_dev);
+  } else if (max_m > 0 && w_elems <= svd_w_lds_cap()) {
+    hipLaunchKernelGGL(k_batched_svd<SVD_FORM_W_LDS>, dim3(batch), dim3(SVD_THREADS),
+                       (unsigned)(w_elems * sizeof(double2)), st, A, U, s, Vd, work, jobs_dev);
   } else {
-    hipLaunchKernelGGL(k_batched_svd<false>, dim3(batch), dim3(SVD_THREADS), 0, st, A, U, s, Vd,
-                       work, jobs_dev);
+    hipLaunchKernelGGL(k_batched_svd<SVD_FORM_GLOBAL>, dim3(batch), dim3(SVD_THREADS), 0, st, A,
+                       U, s, Vd, work, jobs_dev);
   }
   HBP_HIP(hipGetLastError());
 }
