@@ -190,3 +190,26 @@ def test_packed_tpo_layout(monkeypatch):
     serving.Server.submit(srv, job)
     assert len(job.packed) == 2 and all(isinstance(p, PackedTPO) for p in job.packed)
     assert serving._layers(job) is job.packed
+
+
+def test_batched_server_trims_only_when_idle():
+    """BatchedServer._admit with nothing resident: the idle hook (the
+    context's trim) runs once after IDLE_TRIM_S with nothing queued, the
+    scheduler keeps waiting, and a job that then arrives is admitted."""
+    srv = serving.BatchedServer.__new__(serving.BatchedServer)
+    srv.budget, srv.max_bypass, srv._in_use, srv._closing = 1 << 40, 8, 0, False
+    srv._pending = serving.deque()
+    srv._cv = threading.Condition()
+    srv.IDLE_TRIM_S = 0.05
+    calls, got = [], []
+    th = threading.Thread(target=lambda: got.append(srv._admit(True, idle=lambda: calls.append(1))))
+    th.start()
+    time.sleep(0.3)
+    assert calls == [1] and not got
+    job = serving.EstimatorJob(edges=[(0, 1)], tpos=[], chi=1, terms=[], name="late")
+    with srv._cv:
+        srv._pending.append((job, 10, None))
+        srv._cv.notify_all()
+    th.join(2)
+    assert not th.is_alive() and got and [j.name for j, _, _ in got[0]] == ["late"]
+    assert calls == [1]

@@ -371,6 +371,11 @@ class BatchedServer:
     # device scratch of the batched sweep (per precision: up to 4 workgroups
     # per CU, two site-sized buffers each at bond 16), kept out of the budget
     SCRATCH_RESERVE = 6 << 30
+    # The blocks a retired job frees stay with the one context and serve the
+    # next jobs' allocations (an allocation that fails first gives the kept
+    # blocks back and retries); they go back to the device only once nothing
+    # has been resident or queued for this long.
+    IDLE_TRIM_S = 0.5
 
     def __init__(self, device=0, memory_budget=None, max_bypass=8, make_context=None):
         self.device = device
@@ -408,9 +413,11 @@ class BatchedServer:
     def __exit__(self, *exc):
         self.close()
 
-    def _admit(self, block):
+    def _admit(self, block, idle=None):
         """Jobs that fit now (FIFO with bounded bypass, like Server._take);
-        waits for arrivals when `block` and nothing is resident."""
+        waits for arrivals when `block` and nothing is resident.  ``idle``
+        runs once (outside the lock) when nothing has been queued for
+        IDLE_TRIM_S seconds while waiting."""
         out = []
         with self._cv:
             while True:
@@ -431,6 +438,15 @@ class BatchedServer:
                     return out
                 if self._closing and not self._pending:
                     return None
+                if idle is not None and not self._pending:
+                    if not self._cv.wait(timeout=self.IDLE_TRIM_S) and not self._pending:
+                        self._cv.release()
+                        try:
+                            idle()
+                        finally:
+                            self._cv.acquire()
+                        idle = None
+                    continue
                 self._cv.wait()
 
     def _loop(self):
@@ -446,7 +462,7 @@ class BatchedServer:
         active = []
         try:
             while True:
-                new = self._admit(block=not active)
+                new = self._admit(block=not active, idle=ctx.trim)
                 if new is None:
                     return
                 for job, fp, fut in new:
@@ -491,11 +507,6 @@ class BatchedServer:
                         still.append(r)
                         continue
                     self._retire(ctx, r, len(active))
-                if not still:
-                    # nothing resident: give the context's kept blocks back to
-                    # the device (while jobs stay resident the blocks a retired
-                    # job freed serve the others' allocations)
-                    ctx.trim()
                 t4 = time.perf_counter()
                 # wall seconds of the round's phases (each call returns once its
                 # work is done on the device)
