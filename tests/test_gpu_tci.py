@@ -117,7 +117,10 @@ def test_elementwise_and_reductions(ctx):
     assert T.get_elem(ctx, T.Tensor.from_numpy(ctx, a), [1, 2, 3]) == a[1, 2, 3]
 
 
-@pytest.mark.parametrize("shape,nrows", [((6, 6), 1), ((4, 3, 5), 2), ((64, 64), 1), ((2, 7), 1), ((9, 4), 1)])
+# (64, 64): W and V in LDS; (96, 96): W in LDS, V in global memory; (128, 100):
+# both in global memory
+@pytest.mark.parametrize("shape,nrows", [((6, 6), 1), ((4, 3, 5), 2), ((64, 64), 1), ((96, 96), 1),
+                                         ((128, 100), 1), ((2, 7), 1), ((9, 4), 1)])
 def test_svd(ctx, shape, nrows):
     from tnbp_amd import tci as T
     announce(f"svd {shape} {nrows}")
@@ -140,11 +143,14 @@ def test_svd_rank_deficient(ctx):
     messages have low rank: the Jacobi rotations turn some columns into
     rounding noise, and pairs of such columns count as converged; rotating
     them by the relative orthogonality test alone exhausts the sweep limit
-    and leaves singular values wrong by up to 1e-5 on such inputs."""
+    and leaves singular values wrong by up to 1e-5 on such inputs.  Sizes
+    cover the kernel's three memory forms (up to 64 x 64 in LDS, 80 x 80 with
+    W in LDS, 120 x 120 in global memory)."""
     from tnbp_amd import tci as T
-    announce("svd of rank-deficient matrices (LDS and global-memory forms)")
+    announce("svd of rank-deficient matrices (all three memory forms)")
     rng = np.random.default_rng(7)
-    shapes = [(n, int(rng.integers(1, n))) for n in rng.integers(3, 9, size=40)] + [(80, 20), (72, 5)]
+    shapes = ([(n, int(rng.integers(1, n))) for n in rng.integers(3, 9, size=40)]
+              + [(80, 20), (72, 5), (120, 30)])
     for i, (n, r) in enumerate(shapes):
         a = _rand(rng, (n, r)) @ _rand(rng, (r, n))
         if i % 2:
