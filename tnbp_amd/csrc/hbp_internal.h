@@ -57,8 +57,6 @@ struct Ctx {
   // grow-only arena for partials whose reduction is deferred (batched)
   void *arena = nullptr;
   size_t arena_bytes = 0;
-  // pinned host staging for small results
-  double *host_small = nullptr;
   // Pinned staging of uploads: kStageSlots slots of kStageSlotBytes used in
   // turn.  An upload that fits a slot is copied there and its DMA ordered on
   // the stream, with the slot's event recorded after it; the host waits only
