@@ -271,7 +271,7 @@ hbp_status hbp_peps_set_absorb_mode(hbp_peps *p, int mode);
  * the two bonds involved; other messages as PER_SITE), which saves writing
  * and re-reading that product, and with every resident site whose bonds are
  * all <= 16 (degree <= 4) in one launch of the small-site kernel per sweep
- * (one workgroup per site, FP64 sums).  AUTO (default) is MERGED (which
+ * (one workgroup per outgoing message, FP64 sums).  AUTO (default) is MERGED (which
  * falls back to PER_SITE launches wherever bonds are outside (32, 64]).  GROUPED (one
  * launch per sweep level for all sites) is reserved and rejected with
  * HBP_ERR_UNSUPPORTED: its first GPU run left the device unusable, the cause
@@ -351,8 +351,8 @@ hbp_status hbp_peps_bp_run(hbp_ctx *ctx, hbp_peps *p, int max_iter, double tol, 
  * schedule and no mixing iterate in lockstep, one BP iteration of all of
  * them per replay of a HIP graph: those with site degree <= 4 and every
  * bond <= 16 (HBP_BATCH_SMALL) in one persistent launch per precision over
- * every site of every such PEPS (one workgroup per site and stride, launch
- * grid picked by the autotuner), a finalisation and a residual launch; the
+ * every site of every such PEPS (one workgroup per outgoing message and
+ * stride, launch grid picked by the autotuner), a finalisation and a residual launch; the
  * others (HBP_BATCH_GRAPH) with their own fused sweeps (per-site launches,
  * variants picked by the autotuner), bitwise as hbp_peps_bp_run.  The first
  * iteration runs unrecorded; the graph of an iteration is recorded once per
