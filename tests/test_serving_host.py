@@ -114,6 +114,10 @@ def test_footprint_model_scales(monkeypatch):
     # one step from |0...0> never leaves bonds of 2
     k1 = serving.kicked_job(16, 32, 1, 0.7, precision=HBP_PREC_MIXED)
     assert serving.footprint_bytes(k1) < 0.5e9
+    # the small-bond sweep's scratch at bond 16 on 16x16: 1024 workgroups (the
+    # largest grid) of two 16^4 * 2-element complex64 buffers
+    assert serving._small_scratch_bytes(k, [16] * len(k.edges)) == 1024 * 2 * 16 ** 4 * 2 * 8
+    assert serving._small_scratch_bytes(k, [17] * len(k.edges)) == 0
 
 
 def test_small_jobs_absorb_eagerly(monkeypatch):

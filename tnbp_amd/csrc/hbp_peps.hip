@@ -1352,9 +1352,9 @@ static void materialize_small_pending(Ctx *ctx, Peps *p) {
 
 // The sites of a fused sweep whose bonds are all <= SMALL_D_MAX (degree
 // 1..4; `only` / `colour`: the checkerboard's colour filter) in one launch of
-// the batched small-site kernel (hbp_batched.hip: one workgroup per site, FP64
-// sums in a fixed order) instead of a dozen launches each; they are marked in
-// `done`.  Lazily absorbed sites are read from their materialised copies, so
+// the batched small-site kernel (hbp_batched.hip: one work item per outgoing
+// message, FP64 sums in a fixed order) instead of a dozen launches each; they
+// are marked in `done`.  Lazily absorbed sites are read from their materialised copies, so
 // lazy and eager absorption give bitwise the same messages.
 // HBP_SWEEP_PER_SITE keeps every site on its own launches.
 template <typename S>

@@ -119,6 +119,28 @@ def _site_bytes(job, dims):
     return out
 
 
+# The small-bond kernels (sites whose bonds are all <= 16, degree <= 4) take
+# scratch of two site-sized buffers per workgroup: up to four work items per
+# site and at most four workgroups per CU on a 256-CU device.
+SMALL_D_MAX = 16
+SMALL_WORKGROUPS_MAX = 4 * 256
+
+
+def _small_scratch_bytes(job, dims):
+    """Scratch of the small-bond sweep over the sites with bond dims `dims`."""
+    elem = 16 if job.precision == HBP_PREC_FP64 else 8
+    n_small, slab = 0, 0
+    for s, ms in _incident(job.edges).items():
+        if len(ms) > 4 or any(dims[m] > SMALL_D_MAX for m in ms):
+            continue
+        n = 2
+        for m in ms:
+            n *= dims[m]
+        n_small += 1
+        slab = max(slab, n)
+    return min(SMALL_WORKGROUPS_MAX, 4 * n_small) * 2 * slab * elem
+
+
 def absorbed_bytes(job: EstimatorJob) -> int:
     """Bytes of the largest absorbed lattice over the job's layers."""
     return max((sum(_site_bytes(job, ab).values()) for _, ab in bond_bounds(job)), default=0)
@@ -156,8 +178,8 @@ def footprint_bytes(job: EstimatorJob) -> int:
     layer: resident sites, the absorbed lattice when it absorbs eagerly (the
     plain mode always does) and then also the pair-front copies of the merged
     sweep, the messages (three sets during a sweep), the work buffers of the
-    per-site sweep (four of the largest absorbed site), split-K partials, and
-    slack."""
+    per-site sweep (four of the largest absorbed site), the small-bond
+    sweep's scratch, split-K partials, and slack."""
     peak = 0
     for res, ab in bond_bounds(job):
         resident = max(sum(_site_bytes(job, res).values()),
@@ -166,7 +188,8 @@ def footprint_bytes(job: EstimatorJob) -> int:
         msgs = 3 * 2 * sum(x * x for x in ab) * 16
         dmax = max(ab)
         partials = 1024 * min(dmax, 128) ** 2 * 16 * 2
-        peak = max(peak, resident + work + msgs + partials)
+        small = _small_scratch_bytes(job, ab)
+        peak = max(peak, resident + work + msgs + partials + small)
     if absorbs_eagerly(job):
         peak += absorbed_bytes(job) + pair_cache_bytes(job)
     # + the context's fixed arenas (deferred split-K partials: 64 MiB) and tables
