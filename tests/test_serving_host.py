@@ -118,6 +118,9 @@ def test_footprint_model_scales(monkeypatch):
     # largest grid) of two 16^4 * 2-element complex64 buffers
     assert serving._small_scratch_bytes(k, [16] * len(k.edges)) == 1024 * 2 * 16 ** 4 * 2 * 8
     assert serving._small_scratch_bytes(k, [17] * len(k.edges)) == 0
+    # a lazily absorbing PEPS also keeps its small sites' absorbed copies
+    assert serving._small_lazy_bytes(k, [16] * len(k.edges)) == sum(
+        2 * 16 ** len(ms) * 8 for ms in serving._incident(k.edges).values())
 
 
 def test_small_jobs_absorb_eagerly(monkeypatch):

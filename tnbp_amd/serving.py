@@ -126,19 +126,31 @@ SMALL_D_MAX = 16
 SMALL_WORKGROUPS_MAX = 4 * 256
 
 
-def _small_scratch_bytes(job, dims):
-    """Scratch of the small-bond sweep over the sites with bond dims `dims`."""
-    elem = 16 if job.precision == HBP_PREC_FP64 else 8
-    n_small, slab = 0, 0
+def _small_sites(job, dims):
+    """Element counts of the sites the small-bond sweep takes at bond dims `dims`."""
+    out = []
     for s, ms in _incident(job.edges).items():
         if len(ms) > 4 or any(dims[m] > SMALL_D_MAX for m in ms):
             continue
         n = 2
         for m in ms:
             n *= dims[m]
-        n_small += 1
-        slab = max(slab, n)
-    return min(SMALL_WORKGROUPS_MAX, 4 * n_small) * 2 * slab * elem
+        out.append(n)
+    return out
+
+
+def _small_scratch_bytes(job, dims):
+    """Scratch of the small-bond sweep over the sites with bond dims `dims`."""
+    elem = 16 if job.precision == HBP_PREC_FP64 else 8
+    sizes = _small_sites(job, dims)
+    return min(SMALL_WORKGROUPS_MAX, 4 * len(sizes)) * 2 * max(sizes, default=0) * elem
+
+
+def _small_lazy_bytes(job, dims):
+    """The absorbed copies of the small sites that a lazily absorbing PEPS
+    keeps during a BP loop (the small-bond sweep reads them)."""
+    elem = 16 if job.precision == HBP_PREC_FP64 else 8
+    return sum(_small_sites(job, dims)) * elem
 
 
 def absorbed_bytes(job: EstimatorJob) -> int:
@@ -179,7 +191,9 @@ def footprint_bytes(job: EstimatorJob) -> int:
     plain mode always does) and then also the pair-front copies of the merged
     sweep, the messages (three sets during a sweep), the work buffers of the
     per-site sweep (four of the largest absorbed site), the small-bond
-    sweep's scratch, split-K partials, and slack."""
+    sweep's scratch (and, absorbing lazily, its absorbed copies of the small
+    sites), split-K partials, and slack."""
+    eager = absorbs_eagerly(job)
     peak = 0
     for res, ab in bond_bounds(job):
         resident = max(sum(_site_bytes(job, res).values()),
@@ -189,8 +203,10 @@ def footprint_bytes(job: EstimatorJob) -> int:
         dmax = max(ab)
         partials = 1024 * min(dmax, 128) ** 2 * 16 * 2
         small = _small_scratch_bytes(job, ab)
+        if not eager:
+            small += _small_lazy_bytes(job, ab)
         peak = max(peak, resident + work + msgs + partials + small)
-    if absorbs_eagerly(job):
+    if eager:
         peak += absorbed_bytes(job) + pair_cache_bytes(job)
     # + the context's fixed arenas (deferred split-K partials: 64 MiB) and tables
     return int(1.15 * peak) + (128 << 20)
