@@ -1816,11 +1816,17 @@ void tuned_launch(hipStream_t st, const TuneKey &key, int nv, F run) {
   hipEvent_t a, b;
   HBP_HIP(hipEventCreate(&a));
   HBP_HIP(hipEventCreate(&b));
-  int best = 0;
-  float best_ms = 1e30f;
+  // Every variant is timed three times, in rounds over all variants, and
+  // keeps the least of its times: a slow moment of the device (clocks
+  // ramping up, a dispatch delayed on the host) then costs one timing of
+  // each variant rather than every timing of one.
+  constexpr int kRounds = 3;
+  float least[32];
+  HBP_REQUIRE(nv <= 32, HBP_ERR_LOGIC, "autotuner: too many variants");
+  for (int v = 0; v < nv; ++v) least[v] = 1e30f;
   launch_tuning() = true;  // the trial launches do not take a profiled bracket's events
-  for (int v = 0; v < nv; ++v) {
-    for (int rep = 0; rep < 2; ++rep) {
+  for (int round = 0; round < kRounds; ++round) {
+    for (int v = 0; v < nv; ++v) {
       HBP_HIP(hipEventRecord(a, st));
       run(v);
       HBP_HIP(hipEventRecord(b, st));
@@ -1830,13 +1836,13 @@ void tuned_launch(hipStream_t st, const TuneKey &key, int nv, F run) {
       if (tune_log())
         fprintf(stderr, "[hbp autotune] kind=%d elem=%d ncb=%d log2Q=%d aux=%d variant %d: %.4f ms\n",
                 key.kind, key.elem, key.ncb, key.qlog, key.aux, v, ms);
-      if (ms < best_ms) {
-        best_ms = ms;
-        best = v;
-      }
+      least[v] = std::min(least[v], ms);
     }
   }
   launch_tuning() = false;
+  int best = 0;
+  for (int v = 1; v < nv; ++v)
+    if (least[v] < least[best]) best = v;
   HBP_HIP(hipEventDestroy(a));
   HBP_HIP(hipEventDestroy(b));
   {
