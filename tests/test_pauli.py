@@ -59,3 +59,33 @@ def test_checkpoint_rejects_foreign_files(tmp_path):
     old.write_bytes(b"HBPP" + (0).to_bytes(4, "little") + bytes(12))
     with pytest.raises(ValueError):
         load_peps(None, old)
+
+
+def test_classify_routes_each_term():
+    """classify (the host part of estimate, done by a server at submit):
+    weight-1 terms go to the site measurement, weight-2 terms on a lattice
+    edge to the edge measurement with U_b (x) U_a (qubit a the low bit,
+    paulitotpo.h:62-92), every other term to its support path padded with
+    identities; the identity string to none (its value stays 1)."""
+    edges = circuits.square_lattice(3, 3)
+    n = 9
+
+    def label(ops):
+        return "".join(ops.get(n - 1 - i, "I") for i in range(n))
+
+    terms = [(1.0, label({4: "Z"})), (0.5, label({0: "X", 1: "Y"})), (2.0, label({0: "X", 4: "Z"})),
+             (1j, label({0: "Z", 1: "Z", 2: "X"})), (3.0, "I" * n)]
+    one, two, paths = pauli.classify(edges, terms)
+    assert one[0] == [0] and one[1] == [4]
+    np.testing.assert_array_equal(one[2][0], pauli.op_tensor(pauli.PAULI["Z"]))
+    assert two[0] == [1] and two[1] == [(0, 1)]
+    np.testing.assert_array_equal(
+        two[2][0], circuits.op_tensor(np.kron(pauli.PAULI["Y"], pauli.PAULI["X"])))
+    assert paths[0] == [2, 3]
+    p = paths[1][0]  # 0 and 4 are diagonal neighbours: a path through 1 or 3
+    assert p[0] == 0 and p[-1] == 4 and len(p) == 3
+    np.testing.assert_array_equal(paths[2][0][1], np.eye(2))
+    assert paths[1][1] == [0, 1, 2]
+    # the same plan, whoever builds it
+    one2, two2, paths2 = pauli.classify(edges, terms)
+    assert one2[:2] == one[:2] and two2[:2] == two[:2] and paths2[:2] == paths[:2]
