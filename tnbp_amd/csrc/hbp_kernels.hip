@@ -147,7 +147,6 @@ __device__ __forceinline__ void mode_product_tiles(
     S (*er)[16 * NCB + 1], S (*ei)[16 * NCB + 1], double *red) {
   using C = typename Cplx<S>::type;
   using M = Mfma<S>;
-  using v4 = typename M::v4;
   constexpr int NC = 16 * NCB;
   constexpr int64_t ROWS = 4 * 16 * RB;  // rows per tile
 
@@ -297,7 +296,6 @@ __device__ __forceinline__ void mode_product_tiles_pf(
     S (*er)[16 * NCB + 1], S (*ei)[16 * NCB + 1], double *red) {
   using C = typename Cplx<S>::type;
   using M = Mfma<S>;
-  using v4 = typename M::v4;
   constexpr int NC = 16 * NCB;
   constexpr int64_t ROWS = 4 * 16 * RB;
   constexpr int KS = MP_KC / 4;  // k-steps of a tile
@@ -444,8 +442,6 @@ __device__ __forceinline__ void mode_product_tiles_buf(
     typename Cplx<S>::type *__restrict__ Out, int64_t L, int64_t Q, S scale, int64_t b0,
     int64_t nb, S (*er)[65], S (*ei)[65]) {
   using C = typename Cplx<S>::type;
-  using M = Mfma<S>;
-  using v4 = typename M::v4;
   constexpr int D = 64, KS = 16;
   constexpr unsigned ES = sizeof(C);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
