@@ -106,3 +106,37 @@ def test_trace_and_diag():
     np.testing.assert_allclose(tci.trace(a, [(0, 1)]), a[0, 0] + a[1, 1])
     d = tci.diag(np.array([1.0, 2.0, 3.0]))
     assert d.reshape(-1, order="F")[1 + 3 * 1] == 2.0
+
+
+def test_trace_pairs_transpose_and_small_ops_brute_force():
+    """The index bookkeeping of the small TCI operations against nested loops:
+    trace over two bond pairs given in either order (axes shift as pairs are
+    removed), transpose (axis k of the result is axis new_order[k]),
+    linear_combine, cplx_conj, to_cplx, fill and get_elem of an order-0
+    tensor."""
+    rng = np.random.default_rng(11)
+    a = _rand(rng, (2, 3, 2, 3, 4))
+    want = np.zeros(4, dtype=complex)
+    for i in range(2):
+        for j in range(3):
+            for k in range(4):
+                want[k] += a[i, j, i, j, k]
+    np.testing.assert_allclose(tci.trace(a, [(0, 2), (1, 3)]), want, atol=1e-14)
+    np.testing.assert_allclose(tci.trace(a, [(1, 3), (0, 2)]), want, atol=1e-14)
+    b = _rand(rng, (2, 3, 4))
+    order = [2, 0, 1]
+    t = tci.transpose(b, order)
+    assert t.shape == (4, 2, 3)
+    for x in itertools.product(range(4), range(2), range(3)):
+        y = [0, 0, 0]
+        for k in range(3):
+            y[order[k]] = x[k]
+        assert t[x] == b[tuple(y)]
+    c = _rand(rng, (2, 3, 4))
+    np.testing.assert_allclose(tci.linear_combine([b, c], [2.0, -1j]), 2.0 * b - 1j * c, atol=1e-14)
+    np.testing.assert_array_equal(tci.cplx_conj(np.array([1 + 2j, -3j])), np.array([1 - 2j, 3j]))
+    z = tci.to_cplx(np.array([1.5, -2.0]))
+    assert z.dtype == np.complex128 and list(z) == [1.5 + 0j, -2.0 + 0j]
+    f = tci.fill((2, 3), 0.5 - 1j)
+    assert f.shape == (2, 3) and np.all(f == 0.5 - 1j)
+    assert tci.get_elem(np.array(2.5 - 1j), []) == 2.5 - 1j
